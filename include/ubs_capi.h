@@ -1,0 +1,220 @@
+/*
+ * ubs_capi.h -- C ABI of libubs_hip.so, the gfx950 (MI355X) implementation
+ * of the differentiable Beta-splat tile rasteriser that universal-beta-
+ * splatting reaches through its gsplat fork.
+ *
+ * Conventions
+ *   - Every pointer is a device pointer to contiguous row-major data, fp32
+ *     unless the type says otherwise.  Shapes are given in the comments.
+ *   - The caller allocates every output and workspace; no entry point
+ *     allocates device memory.  Gradient outputs are zeroed by the callee.
+ *   - `stream` is a hipStream_t (NULL = default stream).  Work is enqueued
+ *     asynchronously; nothing synchronises with the host.
+ *   - Return value: UBS_OK or an error code; ubs_last_error() gives the
+ *     message of the calling thread's last failure.
+ *   - Reentrant.  The only process-wide state is the error string (thread
+ *     local) and one-time kernel attribute setup.
+ *
+ * Each entry point cites the reference binding it replaces (file:line under
+ * submodules/gsplat/cuda/csrc/ of the reference).
+ */
+#ifndef UBS_CAPI_H
+#define UBS_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UBS_OK 0
+#define UBS_ERR_INVALID 1   /* bad argument: shape, size or unsupported option */
+#define UBS_ERR_HIP 2       /* HIP runtime error */
+#define UBS_ERR_WORKSPACE 3 /* workspace smaller than required */
+
+/* Library version (major*10000 + minor*100 + patch). */
+int ubs_version(void);
+
+/* Message of the calling thread's last failed call ("" if none). */
+const char *ubs_last_error(void);
+
+/* ------------------------------------------------------------------------
+ * Covariance construction.
+ * ------------------------------------------------------------------------ */
+
+/* R = I + A from the skew parameters (a01, a02, a12).
+ * l_triangle [N,3] -> rotmat [N,3,3].
+ * Replaces l_triangle_to_rotmat_fwd_tensor, l_triagnle_to_rotmat_fwd.cu:38-67. */
+int ubs_l_triangle_to_rotmat_fwd(int64_t N, const float *l_triangle, float *rotmat, void *stream);
+
+/* v_rotmat [N,3,3] -> v_l_triangle [N,3].
+ * Replaces l_triangle_to_rotmat_bwd_tensor, l_triagnle_to_rotmat_bwd.cu:38-67. */
+int ubs_l_triangle_to_rotmat_bwd(int64_t N, const float *v_rotmat, float *v_l_triangle, void *stream);
+
+/* Sigma = L L^T, L[:3,:3] = rot diag(scale[:3]), L[i,i] = scale[i] (i>=3),
+ * L[rest_i[m], rest_j[m]] = l_triangle[3+m].  rest_i/rest_j: device int32
+ * [M_rest], grouped by row with ascending columns.  3 <= D <= 11.
+ * covar: [N,D,D], or [N,3,3] when spatial_block != 0 or D == 3.
+ * Replaces rot_scale_l_triangle_to_covar_fwd_tensor,
+ * rot_scale_l_triangle_to_covar_fwd.cu:191-235. */
+int ubs_rot_scale_l_triangle_to_covar_fwd(int64_t N, int64_t D, const float *rot, const float *scale,
+                                          const float *l_triangle, const int32_t *rest_i,
+                                          const int32_t *rest_j, int64_t M_rest, int spatial_block,
+                                          float *covar, void *stream);
+
+/* Backward: v_covar [N,dim,dim] -> v_rot [N,3,3], v_scale [N,D],
+ * v_l_triangle [N,D(D-1)/2] (first three entries zero).
+ * Replaces rot_scale_l_triangle_to_covar_bwd_tensor,
+ * rot_scale_l_triangle_to_covar_bwd.cu:238-293. */
+int ubs_rot_scale_l_triangle_to_covar_bwd(int64_t N, int64_t D, const float *rot, const float *scale,
+                                          const float *l_triangle, const int32_t *rest_i,
+                                          const int32_t *rest_j, int64_t M_rest, int spatial_block,
+                                          const float *v_covar, float *v_rot, float *v_scale,
+                                          float *v_l_triangle, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Conditioning of a D-dim Beta primitive on its C = D-3 query dims
+ * (view direction, time).  4 <= D <= 11.
+ * ------------------------------------------------------------------------ */
+
+/* means [N,D], covars [N,D,D], opacities [N,1], betas [N,C], query [N,C]
+ * -> out_means [N,3], out_covars [N,3,3], out_opacities [N,1].
+ * Replaces cond_mean_convariance_opacity_fwd_tensor,
+ * cond_mean_convariance_opacity_fwd.cu:292-349. */
+int ubs_cond_mean_convariance_opacity_fwd(int64_t N, int64_t D, const float *means, const float *covars,
+                                          const float *opacities, const float *betas,
+                                          const float *query, float *out_means, float *out_covars,
+                                          float *out_opacities, void *stream);
+
+/* Backward (the query receives no gradient, as in the reference):
+ * v_means [N,3], v_covars [N,3,3], v_opacities [N,1] -> g_means [N,D],
+ * g_covars [N,D,D], g_opacities [N,1], g_betas [N,C].
+ * Replaces cond_mean_convariance_opacity_bwd_tensor,
+ * cond_mean_convariance_opacity_bwd.cu:565-630. */
+int ubs_cond_mean_convariance_opacity_bwd(int64_t N, int64_t D, const float *means, const float *covars,
+                                          const float *opacities, const float *betas,
+                                          const float *query, const float *v_means,
+                                          const float *v_covars, const float *v_opacities,
+                                          float *g_means, float *g_covars, float *g_opacities,
+                                          float *g_betas, void *stream);
+
+/* ------------------------------------------------------------------------
+ * EWA projection.
+ * ------------------------------------------------------------------------ */
+
+/* means [N,3]; either covars [N,6] (upper triangle) or quats [N,4] + scales
+ * [N,3]; viewmats [C,4,4] world->camera; Ks [C,3,3].
+ * Outputs radii int32 [C,N], means2d [C,N,2], depths [C,N], conics [C,N,3],
+ * compensations [C,N] (may be NULL).  Culled entries get radii = 0 and leave
+ * the other outputs untouched.
+ * Replaces fully_fused_projection_fwd_tensor, fully_fused_projection_fwd.cu:179-256. */
+int ubs_fully_fused_projection_fwd(uint32_t C, uint32_t N, const float *means, const float *covars,
+                                   const float *quats, const float *scales, const float *viewmats,
+                                   const float *Ks, int32_t width, int32_t height, float eps2d,
+                                   float near_plane, float far_plane, float radius_clip, int ortho,
+                                   int32_t *radii, float *means2d, float *depths, float *conics,
+                                   float *compensations, void *stream);
+
+/* Backward.  v_compensations and compensations may be NULL.  Outputs (zeroed
+ * here): v_means [N,3]; v_covars [N,6] when covars given, else v_quats [N,4]
+ * and v_scales [N,3]; v_viewmats [C,4,4] or NULL.
+ * Replaces fully_fused_projection_bwd_tensor, fully_fused_projection_bwd.cu:242-348. */
+int ubs_fully_fused_projection_bwd(uint32_t C, uint32_t N, const float *means, const float *covars,
+                                   const float *quats, const float *scales, const float *viewmats,
+                                   const float *Ks, int32_t width, int32_t height, float eps2d,
+                                   int ortho, const int32_t *radii, const float *conics,
+                                   const float *compensations, const float *v_means2d,
+                                   const float *v_depths, const float *v_conics,
+                                   const float *v_compensations, float *v_means, float *v_covars,
+                                   float *v_quats, float *v_scales, float *v_viewmats, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Tile intersection (isect_tiles.cu:99-285 split at its host sync).
+ * ------------------------------------------------------------------------ */
+
+/* Workspace bytes for ubs_isect_tiles_count. */
+size_t ubs_isect_count_workspace_bytes(uint32_t C, uint32_t N);
+
+/* Pass 1: tiles_per_gauss int32 [C,N] and their inclusive cumulative sum
+ * cum_tiles int64 [C*N]; *n_isects_dev (device int64) receives the total.
+ * The host reads n_isects to size pass 2. */
+int ubs_isect_tiles_count(uint32_t C, uint32_t N, const float *means2d, const int32_t *radii,
+                          uint32_t tile_size, uint32_t tile_width, uint32_t tile_height,
+                          int32_t *tiles_per_gauss, int64_t *cum_tiles, int64_t *n_isects_dev,
+                          void *workspace, size_t workspace_bytes, void *stream);
+
+/* Pass 2: isect_ids int64 [n_isects] = cam | tile | depth bits (unsorted,
+ * primitive-major) and flatten_ids int32 [n_isects]. */
+int ubs_isect_tiles_emit(uint32_t C, uint32_t N, const float *means2d, const int32_t *radii,
+                         const float *depths, const int64_t *cum_tiles, uint32_t tile_size,
+                         uint32_t tile_width, uint32_t tile_height, int64_t *isect_ids,
+                         int32_t *flatten_ids, void *stream);
+
+/* Bits the (cam, tile) prefix of an isect id occupies: 32 + this = end_bit. */
+uint32_t ubs_isect_prefix_bits(uint32_t C, uint32_t tile_width, uint32_t tile_height);
+
+/* ------------------------------------------------------------------------
+ * Onesweep LSD radix sort of (int64 key, int32 value) pairs on key bits
+ * [begin_bit, end_bit), stable (what cub::DeviceRadixSort::SortPairs does at
+ * isect_tiles.cu:224-278).  Double-buffered: the result lands in
+ * (keys, vals) or (keys_alt, vals_alt); *result_in_alt tells which.
+ * n <= 2^30 - 1.
+ * ------------------------------------------------------------------------ */
+size_t ubs_radix_sort_workspace_bytes(int64_t n, int begin_bit, int end_bit);
+int ubs_radix_sort_pairs(int64_t n, int64_t *keys, int64_t *keys_alt, int32_t *vals,
+                         int32_t *vals_alt, int begin_bit, int end_bit, void *workspace,
+                         size_t workspace_bytes, int *result_in_alt, void *stream);
+/* Reads the sort's device-side status word (0 = fine, nonzero = a look-back
+ * wait hit its spin limit and the result is invalid).  Synchronises stream. */
+int ubs_radix_sort_status(const void *workspace, int *status, void *stream);
+
+/* offsets int32 [C, tile_height, tile_width]: first sorted index of each
+ * (camera, tile); empty tiles get the next start.  n_isects = 0 -> zeros.
+ * Replaces isect_offset_encode_tensor, isect_tiles.cu:335-368. */
+int ubs_isect_offset_encode(int64_t n_isects, const int64_t *isect_ids, uint32_t C,
+                            uint32_t tile_width, uint32_t tile_height, int32_t *offsets,
+                            void *stream);
+
+/* ------------------------------------------------------------------------
+ * Tile blending.  tile_size in {8, 16, 32}; any channel count >= 1.
+ * ------------------------------------------------------------------------ */
+
+/* means2d [C,N,2], conics [C,N,3], colors [C,N,channels], opacities [C,N],
+ * betas [C,N], backgrounds [C,channels] or NULL, masks uint8 [C,th,tw] or
+ * NULL, tile_offsets [C,th,tw], flatten_ids [n_isects]
+ * -> render_colors [C,H,W,channels], render_alphas [C,H,W,1],
+ *    last_ids int32 [C,H,W], T_final [C,H,W] (transmittance after the last
+ *    contributor, kept exact for the backward; may be NULL).
+ * Replaces rasterize_to_pixels_fwd_tensor, rasterize_to_pixels_fwd.cu:193-361. */
+int ubs_rasterize_to_pixels_fwd(uint32_t C, uint32_t N, int64_t n_isects, uint32_t channels,
+                                const float *means2d, const float *conics, const float *colors,
+                                const float *opacities, const float *betas,
+                                const float *backgrounds, const uint8_t *masks, uint32_t width,
+                                uint32_t height, uint32_t tile_size, uint32_t tile_width,
+                                uint32_t tile_height, const int32_t *tile_offsets,
+                                const int32_t *flatten_ids, float *render_colors,
+                                float *render_alphas, int32_t *last_ids, float *T_final,
+                                void *stream);
+
+/* Backward.  T_final may be NULL (then 1 - render_alphas is used, as the
+ * reference does).  Outputs (zeroed here): v_means2d [C,N,2], v_conics
+ * [C,N,3], v_colors [C,N,channels], v_opacities [C,N], v_betas [C,N].
+ * Replaces rasterize_to_pixels_bwd_tensor, rasterize_to_pixels_bwd.cu:278-479. */
+int ubs_rasterize_to_pixels_bwd(uint32_t C, uint32_t N, int64_t n_isects, uint32_t channels,
+                                const float *means2d, const float *conics, const float *colors,
+                                const float *opacities, const float *betas,
+                                const float *backgrounds, const uint8_t *masks, uint32_t width,
+                                uint32_t height, uint32_t tile_size, uint32_t tile_width,
+                                uint32_t tile_height, const int32_t *tile_offsets,
+                                const int32_t *flatten_ids, const float *render_alphas,
+                                const int32_t *last_ids, const float *T_final,
+                                const float *v_render_colors, const float *v_render_alphas,
+                                float *v_means2d, float *v_conics, float *v_colors,
+                                float *v_opacities, float *v_betas, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* UBS_CAPI_H */

@@ -1,0 +1,340 @@
+"""TEST INFRASTRUCTURE ONLY -- numpy front end of the CPU oracle.
+
+Loads ``oracle/libubs_oracle.so`` (built from ``oracle/ubs_oracle.cpp`` by
+``oracle/Makefile``) through ctypes.  Only ``tests/``, ``__graft_entry__.smoke()``
+and ``bench.py``'s ``cpu_baseline`` leg may import this module, and only as
+the checker; the product path never does.
+
+Every function takes numpy arrays; float32 inputs run the float
+instantiation (bit-exact twin of the HIP kernels for projection, tile
+intersection, sort and ranges), float64 inputs run the FP64 oracle.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+import subprocess
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libubs_oracle.so")
+_lib = None
+
+_f32p = ctypes.POINTER(ctypes.c_float)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle library with make (g++)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.check_call(["make", "-s", "-C", _HERE, "-B" if force else "all"])
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+    return _lib
+
+
+def _ptr(a: Optional[np.ndarray]):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "oracle inputs must be C-contiguous"
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _fn(name: str, dtype) -> ctypes._CFuncPtr:
+    suf = "_f64" if np.dtype(dtype) == np.float64 else "_f32"
+    f = getattr(lib(), name + suf)
+    return f
+
+
+def _c(a, dtype):
+    return None if a is None else np.ascontiguousarray(a, dtype=dtype)
+
+
+def num_threads() -> int:
+    return int(lib().ora_num_threads())
+
+
+# ----------------------------------------------------------------------------
+# l_triangle_to_rotmat, rot_scale_l_triangle_to_covar
+# ----------------------------------------------------------------------------
+def rotmat_fwd(l):
+    dt = l.dtype
+    l = _c(l, dt)
+    N = l.shape[0]
+    R = np.empty((N, 3, 3), dt)
+    _fn("ora_rotmat_fwd", dt)(ctypes.c_int64(N), _ptr(l), _ptr(R))
+    return R
+
+
+def rotmat_bwd(vR):
+    dt = vR.dtype
+    vR = _c(vR, dt)
+    N = vR.shape[0]
+    out = np.empty((N, 3), dt)
+    _fn("ora_rotmat_bwd", dt)(ctypes.c_int64(N), _ptr(vR), _ptr(out))
+    return out
+
+
+def covar_fwd(rot, scale, ltri, rest_i, rest_j, spatial_block=False):
+    dt = scale.dtype
+    rot, scale, ltri = _c(rot, dt), _c(scale, dt), _c(ltri, dt)
+    ri, rj = _c(rest_i, np.int32), _c(rest_j, np.int32)
+    N, D = scale.shape
+    dim = 3 if (spatial_block or D == 3) else D
+    out = np.empty((N, dim, dim), dt)
+    _fn("ora_covar_fwd", dt)(ctypes.c_int64(N), ctypes.c_int64(D), _ptr(rot), _ptr(scale),
+                             _ptr(ltri), _ptr(ri), _ptr(rj), ctypes.c_int64(ri.size),
+                             ctypes.c_int(int(spatial_block)), _ptr(out))
+    return out
+
+
+def covar_bwd(rot, scale, ltri, rest_i, rest_j, spatial_block, v_covar):
+    dt = scale.dtype
+    rot, scale, ltri, v_covar = _c(rot, dt), _c(scale, dt), _c(ltri, dt), _c(v_covar, dt)
+    ri, rj = _c(rest_i, np.int32), _c(rest_j, np.int32)
+    N, D = scale.shape
+    vr = np.empty((N, 3, 3), dt)
+    vs = np.empty((N, D), dt)
+    vl = np.empty((N, D * (D - 1) // 2), dt)
+    _fn("ora_covar_bwd", dt)(ctypes.c_int64(N), ctypes.c_int64(D), _ptr(rot), _ptr(scale),
+                             _ptr(ltri), _ptr(ri), _ptr(rj), ctypes.c_int64(ri.size),
+                             ctypes.c_int(int(spatial_block)), _ptr(v_covar), _ptr(vr), _ptr(vs),
+                             _ptr(vl))
+    return vr, vs, vl
+
+
+# ----------------------------------------------------------------------------
+# cond_mean_convariance_opacity
+# ----------------------------------------------------------------------------
+def cond_fwd(means, covars, opacities, betas, query):
+    dt = means.dtype
+    means, covars, opacities, betas, query = (_c(a, dt) for a in (means, covars, opacities, betas, query))
+    N, D = means.shape
+    om = np.empty((N, 3), dt)
+    oV = np.empty((N, 3, 3), dt)
+    oo = np.empty((N, 1), dt)
+    _fn("ora_cond_fwd", dt)(ctypes.c_int64(N), ctypes.c_int64(D), _ptr(means), _ptr(covars),
+                            _ptr(opacities), _ptr(betas), _ptr(query), _ptr(om), _ptr(oV), _ptr(oo))
+    return om, oV, oo
+
+
+def cond_bwd(means, covars, opacities, betas, query, v_m, v_V, v_o):
+    dt = means.dtype
+    arrs = [_c(a, dt) for a in (means, covars, opacities, betas, query, v_m, v_V, v_o)]
+    N, D = arrs[0].shape
+    C = D - 3
+    gm = np.empty((N, D), dt)
+    gV = np.empty((N, D, D), dt)
+    go = np.empty((N, 1), dt)
+    gb = np.empty((N, C), dt)
+    _fn("ora_cond_bwd", dt)(ctypes.c_int64(N), ctypes.c_int64(D), *[_ptr(a) for a in arrs],
+                            _ptr(gm), _ptr(gV), _ptr(go), _ptr(gb))
+    return gm, gV, go, gb
+
+
+# ----------------------------------------------------------------------------
+# fully_fused_projection
+# ----------------------------------------------------------------------------
+def proj_fwd(means, covars, quats, scales, viewmats, Ks, width, height, eps2d=0.3,
+             near_plane=0.01, far_plane=1e10, radius_clip=0.0, calc_compensations=False,
+             ortho=False):
+    dt = means.dtype
+    means, covars, quats, scales, viewmats, Ks = (_c(a, dt) for a in (means, covars, quats, scales, viewmats, Ks))
+    C, N = viewmats.shape[0], means.shape[0]
+    radii = np.zeros((C, N), np.int32)
+    means2d = np.zeros((C, N, 2), dt)
+    depths = np.zeros((C, N), dt)
+    conics = np.zeros((C, N, 3), dt)
+    comp = np.zeros((C, N), dt) if calc_compensations else None
+    ct = ctypes.c_double if dt == np.float64 else ctypes.c_float
+    _fn("ora_proj_fwd", dt)(ctypes.c_int64(C), ctypes.c_int64(N), _ptr(means), _ptr(covars),
+                            _ptr(quats), _ptr(scales), _ptr(viewmats), _ptr(Ks),
+                            ctypes.c_int32(width), ctypes.c_int32(height), ct(eps2d),
+                            ct(near_plane), ct(far_plane), ct(radius_clip), ctypes.c_int(int(ortho)),
+                            _ptr(radii), _ptr(means2d), _ptr(depths), _ptr(conics), _ptr(comp))
+    return radii, means2d, depths, conics, comp
+
+
+def proj_bwd(means, covars, quats, scales, viewmats, Ks, width, height, eps2d, radii, conics,
+             comp, v_means2d, v_depths, v_conics, v_comp=None, ortho=False,
+             viewmats_requires_grad=False):
+    dt = means.dtype
+    means, covars, quats, scales, viewmats, Ks, conics, comp, v_means2d, v_depths, v_conics, v_comp = (
+        _c(a, dt) for a in (means, covars, quats, scales, viewmats, Ks, conics, comp, v_means2d,
+                            v_depths, v_conics, v_comp))
+    radii = _c(radii, np.int32)
+    C, N = viewmats.shape[0], means.shape[0]
+    v_means = np.zeros((N, 3), dt)
+    v_covars = np.zeros((N, 6), dt) if covars is not None else None
+    v_quats = np.zeros((N, 4), dt) if covars is None else None
+    v_scales = np.zeros((N, 3), dt) if covars is None else None
+    v_viewmats = np.zeros((C, 4, 4), dt) if viewmats_requires_grad else None
+    ct = ctypes.c_double if dt == np.float64 else ctypes.c_float
+    _fn("ora_proj_bwd", dt)(ctypes.c_int64(C), ctypes.c_int64(N), _ptr(means), _ptr(covars),
+                            _ptr(quats), _ptr(scales), _ptr(viewmats), _ptr(Ks),
+                            ctypes.c_int32(width), ctypes.c_int32(height), ct(eps2d),
+                            ctypes.c_int(int(ortho)), _ptr(radii), _ptr(conics), _ptr(comp),
+                            _ptr(v_means2d), _ptr(v_depths), _ptr(v_conics), _ptr(v_comp),
+                            _ptr(v_means), _ptr(v_covars), _ptr(v_quats), _ptr(v_scales),
+                            _ptr(v_viewmats))
+    return v_means, v_covars, v_quats, v_scales, v_viewmats
+
+
+# ----------------------------------------------------------------------------
+# isect_tiles / sort / isect_offset_encode (float32 structure only)
+# ----------------------------------------------------------------------------
+def isect_tiles(means2d, radii, depths, tile_size, tile_width, tile_height, sort=True):
+    means2d = _c(means2d, np.float32)
+    radii = _c(radii, np.int32)
+    depths = _c(depths, np.float32)
+    C, N = radii.shape
+    tpg = np.zeros((C, N), np.int32)
+    cum = np.zeros((C * N,), np.int64)
+    L = lib()
+    L.ora_isect_count.restype = ctypes.c_int64
+    n = int(L.ora_isect_count(ctypes.c_int64(C), ctypes.c_int64(N), _ptr(means2d), _ptr(radii),
+                              ctypes.c_uint32(tile_size), ctypes.c_uint32(tile_width),
+                              ctypes.c_uint32(tile_height), _ptr(tpg), _ptr(cum)))
+    ids = np.zeros((n,), np.int64)
+    flat = np.zeros((n,), np.int32)
+    L.ora_isect_emit(ctypes.c_int64(C), ctypes.c_int64(N), _ptr(means2d), _ptr(radii),
+                     _ptr(depths), _ptr(cum), ctypes.c_uint32(tile_size),
+                     ctypes.c_uint32(tile_width), ctypes.c_uint32(tile_height),
+                     ctypes.c_int(int(sort)), _ptr(ids), _ptr(flat))
+    return tpg, ids, flat
+
+
+def sort_pairs(keys, vals, begin_bit, end_bit):
+    keys = np.array(keys, dtype=np.int64, copy=True)
+    vals = np.array(vals, dtype=np.int32, copy=True)
+    lib().ora_sort_pairs(ctypes.c_int64(keys.size), _ptr(keys), _ptr(vals), ctypes.c_int(begin_bit),
+                         ctypes.c_int(end_bit))
+    return keys, vals
+
+
+def isect_offset_encode(isect_ids, n_cameras, tile_width, tile_height):
+    isect_ids = _c(isect_ids, np.int64)
+    off = np.zeros((n_cameras, tile_height, tile_width), np.int32)
+    lib().ora_offsets(ctypes.c_int64(isect_ids.size), _ptr(isect_ids), ctypes.c_int64(n_cameras),
+                      ctypes.c_uint32(tile_width), ctypes.c_uint32(tile_height), _ptr(off))
+    return off
+
+
+def tile_bits(tile_width, tile_height):
+    return int(tile_width * tile_height).bit_length()
+
+
+# ----------------------------------------------------------------------------
+# rasterize_to_pixels
+# ----------------------------------------------------------------------------
+def raster_fwd(means2d, conics, colors, opacities, betas, width, height, tile_size,
+               isect_offsets, flatten_ids, backgrounds=None, masks=None, margins=False):
+    dt = means2d.dtype
+    means2d, conics, colors, opacities, betas, backgrounds = (
+        _c(a, dt) for a in (means2d, conics, colors, opacities, betas, backgrounds))
+    isect_offsets = _c(isect_offsets, np.int32)
+    flatten_ids = _c(flatten_ids, np.int32)
+    masks = None if masks is None else np.ascontiguousarray(masks, dtype=np.uint8)
+    C, N = opacities.shape
+    D = colors.shape[-1]
+    th, tw = isect_offsets.shape[1:3]
+    out_c = np.zeros((C, height, width, D), dt)
+    out_a = np.zeros((C, height, width, 1), dt)
+    last = np.zeros((C, height, width), np.int32)
+    mT = np.zeros((C, height, width), np.float64) if margins else None
+    mS = np.zeros((C, height, width), np.float64) if margins else None
+    _fn("ora_raster_fwd", dt)(ctypes.c_int64(C), ctypes.c_int64(N),
+                              ctypes.c_int64(flatten_ids.size), ctypes.c_int64(D), _ptr(means2d),
+                              _ptr(conics), _ptr(colors), _ptr(opacities), _ptr(betas),
+                              _ptr(backgrounds), _ptr(masks), ctypes.c_int32(width),
+                              ctypes.c_int32(height), ctypes.c_int32(tile_size), ctypes.c_int32(tw),
+                              ctypes.c_int32(th), _ptr(isect_offsets), _ptr(flatten_ids),
+                              _ptr(out_c), _ptr(out_a), _ptr(last), _ptr(mT), _ptr(mS))
+    if margins:
+        return out_c, out_a, last, mT, mS
+    return out_c, out_a, last
+
+
+def raster_bwd(means2d, conics, colors, opacities, betas, width, height, tile_size,
+               isect_offsets, flatten_ids, render_alphas, last_ids, v_render_colors,
+               v_render_alphas, backgrounds=None, masks=None, T_final=None):
+    dt = means2d.dtype
+    means2d, conics, colors, opacities, betas, backgrounds, render_alphas, v_render_colors, v_render_alphas, T_final = (
+        _c(a, dt) for a in (means2d, conics, colors, opacities, betas, backgrounds, render_alphas,
+                            v_render_colors, v_render_alphas, T_final))
+    isect_offsets = _c(isect_offsets, np.int32)
+    flatten_ids = _c(flatten_ids, np.int32)
+    last_ids = _c(last_ids, np.int32)
+    masks = None if masks is None else np.ascontiguousarray(masks, dtype=np.uint8)
+    C, N = opacities.shape
+    D = colors.shape[-1]
+    th, tw = isect_offsets.shape[1:3]
+    v_means2d = np.zeros((C, N, 2), dt)
+    v_conics = np.zeros((C, N, 3), dt)
+    v_colors = np.zeros((C, N, D), dt)
+    v_opac = np.zeros((C, N), dt)
+    v_betas = np.zeros((C, N), dt)
+    _fn("ora_raster_bwd", dt)(ctypes.c_int64(C), ctypes.c_int64(N),
+                              ctypes.c_int64(flatten_ids.size), ctypes.c_int64(D), _ptr(means2d),
+                              _ptr(conics), _ptr(colors), _ptr(opacities), _ptr(betas),
+                              _ptr(backgrounds), _ptr(masks), ctypes.c_int32(width),
+                              ctypes.c_int32(height), ctypes.c_int32(tile_size), ctypes.c_int32(tw),
+                              ctypes.c_int32(th), _ptr(isect_offsets), _ptr(flatten_ids),
+                              _ptr(render_alphas), _ptr(last_ids), _ptr(v_render_colors),
+                              _ptr(v_render_alphas), _ptr(T_final), _ptr(v_means2d),
+                              _ptr(v_conics), _ptr(v_colors), _ptr(v_opac), _ptr(v_betas))
+    return v_means2d, v_conics, v_colors, v_opac, v_betas
+
+
+# ----------------------------------------------------------------------------
+# rasterization() orchestration (submodules/gsplat/rendering.py:17-232), RGB
+# mode, covars given as [N,3,3] (the BetaModel call) or quats/scales.
+# ----------------------------------------------------------------------------
+TRIU = ([0, 0, 0, 1, 1, 2], [0, 1, 2, 1, 2, 2])
+
+
+def rasterization(means, covars33, opacities, betas, colors, viewmats, Ks, width, height,
+                  near_plane=0.01, far_plane=1e10, radius_clip=0.0, eps2d=0.3, tile_size=16,
+                  backgrounds=None, structure=None, quats=None, scales=None):
+    """Forward of rendering.py:rasterization for one dtype.
+
+    ``structure`` lets an FP64 evaluation reuse the float32 pipeline's tile
+    lists (isect_offsets, flatten_ids) so both blend in the same order.
+    Returns (render_colors, render_alphas, meta).
+    """
+    dt = means.dtype
+    C = viewmats.shape[0]
+    covars6 = None if covars33 is None else np.ascontiguousarray(covars33[:, TRIU[0], TRIU[1]])
+    radii, means2d, depths, conics, _ = proj_fwd(means, covars6, quats, scales, viewmats, Ks, width,
+                                                 height, eps2d, near_plane, far_plane, radius_clip)
+    opac = np.ascontiguousarray(np.broadcast_to(opacities[None], (C,) + opacities.shape))
+    bet = np.ascontiguousarray(np.broadcast_to(betas[None], (C,) + betas.shape))
+    cols = colors if colors.ndim == 3 else np.ascontiguousarray(np.broadcast_to(colors[None], (C,) + colors.shape))
+    tw = math.ceil(width / float(tile_size))
+    th = math.ceil(height / float(tile_size))
+    if structure is None:
+        tpg, ids, flat = isect_tiles(means2d.astype(np.float32), radii, depths.astype(np.float32),
+                                     tile_size, tw, th)
+        offs = isect_offset_encode(ids, C, tw, th)
+    else:
+        tpg, ids, flat, offs = structure
+    out_c, out_a, last = raster_fwd(means2d, conics, cols, opac, bet, width, height, tile_size, offs,
+                                    flat, backgrounds)
+    meta = dict(radii=radii, means2d=means2d, depths=depths, conics=conics, opacities=opac,
+                betas=bet, colors=cols, tile_width=tw, tile_height=th, tiles_per_gauss=tpg,
+                isect_ids=ids, flatten_ids=flat, isect_offsets=offs, last_ids=last,
+                covars6=covars6, width=width, height=height, tile_size=tile_size, n_cameras=C)
+    return out_c, out_a, meta

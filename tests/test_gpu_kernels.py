@@ -1,0 +1,235 @@
+"""GPU parity of every kernel against the oracle (same seeded inputs).
+
+Bit-exact: rotmat, covariance forward, conditioned mean/covariance, projection
+(radii, means2d, depths, conics), tiles_per_gauss, isect ids, flatten ids,
+sort order, tile offsets.  Tolerances (stated per test) for transcendental and
+atomically accumulated outputs.
+"""
+import numpy as np
+import pytest
+import torch
+
+from gsplat import scenes
+from gsplat.cuda import _wrapper as W
+from oracle import ubs_oracle as O
+
+from util import assert_same, conditioned_scene, opaque_scene, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def cu(a, dtype=None):
+    t = torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    return t if dtype is None else t.to(dtype)
+
+
+def host(t):
+    return t.detach().cpu().numpy()
+
+
+# ------------------------------------------------------------------ covariance
+def test_rotmat_fwd_bwd_exact(device):
+    g = np.random.Generator(np.random.PCG64(1))
+    l = g.normal(0, 1, (5000, 3)).astype(np.float32)
+    vR = g.normal(0, 1, (5000, 3, 3)).astype(np.float32)
+    lt = cu(l).requires_grad_(True)
+    R = W.l_triangle_to_rotmat(lt)
+    assert_same(host(R), O.rotmat_fwd(l), "rotmat")
+    R.backward(cu(vR))
+    assert_same(host(lt.grad), O.rotmat_bwd(vR), "rotmat_bwd")
+
+
+@pytest.mark.parametrize("D,spatial", [(6, False), (7, False), (6, True), (3, False), (9, False)])
+def test_covar_fwd_exact_bwd_close(device, D, spatial):
+    g = np.random.Generator(np.random.PCG64(D))
+    N = 3000
+    rot = scenes.quat_to_rotmat_np(g.normal(0, 1, (N, 4))).astype(np.float32)
+    s = np.exp(g.normal(-1, 0.5, (N, D))).astype(np.float32)
+    lt = g.normal(0, 0.2, (N, D * (D - 1) // 2)).astype(np.float32)
+    ri, rj = scenes.rest_pattern(D)
+    want = O.covar_fwd(rot, s, lt, ri, rj, spatial)
+    tr, ts_, tl = cu(rot).requires_grad_(True), cu(s).requires_grad_(True), cu(lt).requires_grad_(True)
+    got = W.rot_scale_l_triangle_to_covar(tr, ts_, tl, cu(ri), cu(rj), spatial)
+    assert_same(host(got), want, f"covar D={D}")
+    G = g.normal(0, 1, want.shape).astype(np.float32)
+    got.backward(cu(G))
+    wr, ws, wl = O.covar_bwd(rot.astype(np.float64), s.astype(np.float64), lt.astype(np.float64), ri, rj,
+                             spatial, G.astype(np.float64))
+    for a, b, n in ((tr.grad, wr, "v_rot"), (ts_.grad, ws, "v_scale"), (tl.grad, wl, "v_ltri")):
+        assert rel_l2(host(a), b) < 1e-5, (n, rel_l2(host(a), b))
+
+
+# ---------------------------------------------------------------- conditioning
+@pytest.mark.parametrize("D", [6, 7])
+def test_cond_fwd_exact_bwd_close(device, D):
+    c = conditioned_scene(4000, D, seed=3)
+    a = c["act"]
+    ins = [a["mean"], c["cov"], a["opac"], np.ascontiguousarray(a["beta"][:, 1:]), c["query"]]
+    m, V, o = O.cond_fwd(*ins)
+    tins = [cu(x).requires_grad_(i < 4) for i, x in enumerate(ins)]
+    gm, gV, go = W.cond_mean_convariance_opacity(*tins)
+    assert_same(host(gm), m, "m_cond")
+    assert_same(host(gV), V, "V_cond")
+    # tanh / pow differ between the GPU and libm by a few ulp
+    np.testing.assert_allclose(host(go), o, rtol=2e-5, atol=1e-30)
+    g = np.random.Generator(np.random.PCG64(11))
+    vm = g.normal(0, 1, m.shape).astype(np.float32)
+    vV = g.normal(0, 1, V.shape).astype(np.float32)
+    vo = g.normal(0, 1, o.shape).astype(np.float32)
+    torch.autograd.backward([gm, gV, go], [cu(vm), cu(vV), cu(vo)])
+    want = O.cond_bwd(*[x.astype(np.float64) for x in ins], vm.astype(np.float64), vV.astype(np.float64),
+                      vo.astype(np.float64))
+    for t, w, n in zip(tins[:4], want, ("g_means", "g_covars", "g_opac", "g_betas")):
+        assert rel_l2(host(t.grad), w) < 2e-4, (n, rel_l2(host(t.grad), w))
+
+
+# ------------------------------------------------------------------ projection
+@pytest.mark.parametrize("mode,C", [("covars", 1), ("covars", 3), ("quats", 1), ("quats", 2)])
+def test_projection_fwd_exact(device, mode, C):
+    s = opaque_scene(20000, W=320, H=200, n_cams=C, seed=C)
+    cov = s["covars6"] if mode == "covars" else None
+    q = s["quats"] if mode == "quats" else None
+    sc = s["scales"] if mode == "quats" else None
+    want = O.proj_fwd(s["means"], cov, q, sc, s["viewmats"], s["Ks"], 320, 200, calc_compensations=True)
+    got = W.fully_fused_projection(cu(s["means"]), None if cov is None else cu(cov), None if q is None else cu(q),
+                                   None if sc is None else cu(sc), cu(s["viewmats"]), cu(s["Ks"]), 320, 200,
+                                   calc_compensations=True)
+    radii = host(got[0])
+    assert_same(radii, want[0], "radii")
+    vis = radii > 0
+    assert vis.sum() > 100
+    assert_same(host(got[1])[vis], want[1][vis], "means2d")
+    assert_same(host(got[2])[vis], want[2][vis], "depths")
+    assert_same(host(got[3])[vis], want[3][vis], "conics")
+    assert_same(host(got[4])[vis], want[4][vis], "compensations")
+
+
+@pytest.mark.parametrize("mode,C", [("covars", 1), ("covars", 3), ("quats", 2)])
+def test_projection_bwd_close(device, mode, C):
+    s = opaque_scene(5000, W=320, H=200, n_cams=C, seed=10 + C)
+    cov = s["covars6"] if mode == "covars" else None
+    q = s["quats"] if mode == "quats" else None
+    sc = s["scales"] if mode == "quats" else None
+    tm = cu(s["means"]).requires_grad_(True)
+    tc = None if cov is None else cu(cov).requires_grad_(True)
+    tq = None if q is None else cu(q).requires_grad_(True)
+    ts_ = None if sc is None else cu(sc).requires_grad_(True)
+    tv = cu(s["viewmats"]).requires_grad_(True)
+    radii, m2, dp, cn, cp = W.fully_fused_projection(tm, tc, tq, ts_, tv, cu(s["Ks"]), 320, 200,
+                                                      calc_compensations=True)
+    g = np.random.Generator(np.random.PCG64(5))
+    vis = host(radii) > 0
+    vm2 = (g.normal(0, 1, host(m2).shape) * vis[..., None]).astype(np.float32)
+    vdp = (g.normal(0, 1, host(dp).shape) * vis).astype(np.float32)
+    vcn = (g.normal(0, 1, host(cn).shape) * vis[..., None]).astype(np.float32)
+    vcp = (g.normal(0, 1, host(cp).shape) * vis).astype(np.float32)
+    torch.autograd.backward([m2, dp, cn, cp], [cu(vm2), cu(vdp), cu(vcn), cu(vcp)])
+    d = np.float64
+    want = O.proj_bwd(s["means"].astype(d), None if cov is None else cov.astype(d),
+                      None if q is None else q.astype(d), None if sc is None else sc.astype(d),
+                      s["viewmats"].astype(d), s["Ks"].astype(d), 320, 200, 0.3, host(radii),
+                      host(cn).astype(d), host(cp).astype(d), vm2.astype(d), vdp.astype(d), vcn.astype(d),
+                      vcp.astype(d), viewmats_requires_grad=True)
+    pairs = [(tm.grad, want[0], "v_means"), (tv.grad, want[4], "v_viewmats")]
+    if cov is not None:
+        pairs.append((tc.grad, want[1], "v_covars"))
+    else:
+        pairs += [(tq.grad, want[2], "v_quats"), (ts_.grad, want[3], "v_scales")]
+    for t, w, n in pairs:
+        assert rel_l2(host(t), w) < 1e-4, (n, rel_l2(host(t), w))
+
+
+# ------------------------------------------------------- isect / sort / offsets
+@pytest.mark.parametrize("C", [1, 2])
+def test_isect_and_offsets_exact(device, C):
+    s = opaque_scene(30000, W=500, H=300, n_cams=C, seed=20 + C)
+    r, m2, dp, cn, _ = O.proj_fwd(s["means"], s["covars6"], None, None, s["viewmats"], s["Ks"], 500, 300)
+    tw, th = (500 + 15) // 16, (300 + 15) // 16
+    want_tpg, want_ids, want_flat = O.isect_tiles(m2, r, dp, 16, tw, th)
+    tpg, ids, flat = W.isect_tiles(cu(m2), cu(r), cu(dp), 16, tw, th)
+    assert_same(host(tpg), want_tpg, "tiles_per_gauss")
+    assert_same(host(ids), want_ids, "isect_ids")
+    assert_same(host(flat), want_flat, "flatten_ids")
+    assert want_ids.size > 10000
+    off = W.isect_offset_encode(ids, C, tw, th)
+    assert_same(host(off), O.isect_offset_encode(want_ids, C, tw, th), "offsets")
+
+
+def test_isect_unsorted_and_edges(device):
+    # primitives straddling every image border (negative tile coordinates saturate to 0)
+    means2d = np.array([[[-20.0, -20.0], [5.0, 5.0], [499.0, 299.0], [250.0, -3.0], [-1.0, 150.0],
+                         [510.0, 310.0]]], np.float32)
+    radii = np.array([[30, 3, 40, 5, 2, 20]], np.int32)
+    depths = np.array([[1.0, 2.0, 3.0, 4.0, 5.0, 0.5]], np.float32)
+    tw, th = 32, 19
+    for sort in (False, True):
+        want = O.isect_tiles(means2d, radii, depths, 16, tw, th, sort=sort)
+        got = W.isect_tiles(cu(means2d), cu(radii), cu(depths), 16, tw, th, sort=sort)
+        for a, b, n in zip(got, want, ("tpg", "ids", "flat")):
+            assert_same(host(a), b, f"{n} sort={sort}")
+
+
+@pytest.mark.parametrize("n", [0, 1, 63, 4095, 4096, 4097, 100003])
+@pytest.mark.parametrize("end_bit", [7, 33, 46, 64])
+def test_radix_sort_stable(device, n, end_bit):
+    g = np.random.Generator(np.random.PCG64(n * 131 + end_bit))
+    # few distinct keys -> many ties, so stability is visible
+    keys = g.integers(-(1 << 62), 1 << 62, size=max(n // 7, 1), dtype=np.int64)[g.integers(0, max(n // 7, 1), n)]
+    keys ^= g.integers(0, 4, n, dtype=np.int64) << 40
+    vals = np.arange(n, dtype=np.int32)
+    want_k, want_v = O.sort_pairs(keys, vals, 0, end_bit)
+    got_k, got_v = W.radix_sort_pairs(cu(keys), cu(vals), 0, end_bit, check_status=True)
+    assert_same(host(got_k), want_k, "keys")
+    assert_same(host(got_v), want_v, "values")
+
+
+# --------------------------------------------------------------------- raster
+def _raster_inputs(scene_fn, W_, H_, C=1, **kw):
+    s = scene_fn(W=W_, H=H_, n_cams=C, **kw)
+    r, m2, dp, cn, _ = O.proj_fwd(s["means"], s["covars6"], None, None, s["viewmats"], s["Ks"], W_, H_)
+    tw, th = (W_ + 15) // 16, (H_ + 15) // 16
+    _, ids, flat = O.isect_tiles(m2, r, dp, 16, tw, th)
+    off = O.isect_offset_encode(ids, C, tw, th)
+    op = np.ascontiguousarray(np.broadcast_to(s["opac"], (C, s["opac"].size)))
+    be = np.ascontiguousarray(np.broadcast_to(s["beta"], (C, s["beta"].size)))
+    col = np.ascontiguousarray(np.broadcast_to(s["rgb"], (C,) + s["rgb"].shape))
+    return dict(m2=m2, cn=cn, col=col, op=op, be=be, off=off, flat=flat, tw=tw, th=th, C=C)
+
+
+@pytest.mark.parametrize("bg", [False, True])
+def test_raster_fwd_close(device, bg):
+    W_, H_ = 200, 136
+    x = _raster_inputs(lambda **k: opaque_scene(3000, **k), W_, H_, C=2)
+    bgs = np.random.Generator(np.random.PCG64(3)).uniform(0, 1, (2, 3)).astype(np.float32) if bg else None
+    want_c, want_a, want_last, mT, mS = O.raster_fwd(x["m2"], x["cn"], x["col"], x["op"], x["be"], W_, H_, 16,
+                                                    x["off"], x["flat"], bgs, margins=True)
+    rc, ra, last, Tf = W.raster_fwd_raw(cu(x["m2"]), cu(x["cn"]), cu(x["col"]), cu(x["op"]), cu(x["be"]),
+                                        None if bgs is None else cu(bgs), None, W_, H_, 16, cu(x["off"]),
+                                        cu(x["flat"]))
+    safe = (mT > 1e-3) & (mS > 1e-6)  # pixels far from the T cutoff / sigma=0 branch points
+    assert safe.mean() > 0.95
+    np.testing.assert_allclose(host(rc)[safe], want_c[safe], atol=2e-5)
+    np.testing.assert_allclose(host(ra)[..., 0][safe], want_a[..., 0][safe], atol=2e-5)
+    assert_same(host(last)[safe], want_last[safe], "last_ids")
+    np.testing.assert_allclose(host(Tf)[safe], 1.0 - want_a[..., 0][safe], atol=2e-5)
+    assert (want_a > 0.9).mean() > 0.05  # the cutoff is exercised
+
+
+def test_raster_bwd_close(device):
+    W_, H_ = 160, 120
+    x = _raster_inputs(lambda **k: opaque_scene(1500, opacity=(0.2, 0.9), **k), W_, H_, C=1)
+    d = np.float64
+    want_c, want_a, want_last, mT, mS = O.raster_fwd(*(x[k].astype(d) for k in ("m2", "cn", "col", "op", "be")),
+                                                    W_, H_, 16, x["off"], x["flat"], margins=True)
+    rc, ra, last, Tf = W.raster_fwd_raw(cu(x["m2"]), cu(x["cn"]), cu(x["col"]), cu(x["op"]), cu(x["be"]), None,
+                                        None, W_, H_, 16, cu(x["off"]), cu(x["flat"]))
+    assert (mT > 1e-3).all() and (mS > 1e-6).all(), "pick a seed without branch-point pixels"
+    g = np.random.Generator(np.random.PCG64(9))
+    vrc = g.normal(0, 1, host(rc).shape).astype(np.float32)
+    vra = g.normal(0, 1, host(ra).shape).astype(np.float32)
+    got = W.raster_bwd_raw(cu(x["m2"]), cu(x["cn"]), cu(x["col"]), cu(x["op"]), cu(x["be"]), None, None, W_, H_,
+                           16, cu(x["off"]), cu(x["flat"]), ra, last, Tf, cu(vrc), cu(vra))
+    want = O.raster_bwd(*(x[k].astype(d) for k in ("m2", "cn", "col", "op", "be")), W_, H_, 16, x["off"],
+                        x["flat"], want_a, want_last, vrc.astype(d), vra.astype(d), T_final=1.0 - want_a[..., 0])
+    for a, b, n in zip(got, want, ("v_means2d", "v_conics", "v_colors", "v_opac", "v_betas")):
+        assert rel_l2(host(a), b) < 2e-3, (n, rel_l2(host(a), b))

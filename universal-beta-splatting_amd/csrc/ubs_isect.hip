@@ -1,0 +1,242 @@
+// Tile intersection on gfx950: per-primitive tile count, int64 inclusive scan
+// (reduce-then-scan, three launches, no spin waits), key emission and the
+// per-(camera, tile) range table.
+//
+// The tile box follows isect_tiles.cu:54-66 exactly: tile_radius and the
+// centre are divided by tile_size in fp32, floor/ceil, converted with CUDA's
+// saturating float->uint32 semantics and clamped to the grid.
+#pragma clang fp contract(off)
+
+#include "ubs_common.h"
+
+namespace ubs {
+
+constexpr int kScanItems = 4;                        // items per thread
+constexpr int kScanTile = kThreads * kScanItems;     // elements per block
+
+__device__ __forceinline__ void tile_box(float mx, float my, int32_t rad, uint32_t ts, uint32_t tw,
+                                         uint32_t th, uint32_t &x0, uint32_t &y0, uint32_t &x1,
+                                         uint32_t &y1) {
+    const float tr = (float)rad / (float)ts;
+    const float tx = mx / (float)ts, ty = my / (float)ts;
+    x0 = min(sat_u32(floorf(tx - tr)), tw);
+    y0 = min(sat_u32(floorf(ty - tr)), th);
+    x1 = min(sat_u32(ceilf(tx + tr)), tw);
+    y1 = min(sat_u32(ceilf(ty + tr)), th);
+}
+
+__device__ __forceinline__ int32_t tile_count(const float *__restrict__ means2d, const int32_t *__restrict__ radii,
+                                              uint64_t idx, uint32_t ts, uint32_t tw, uint32_t th) {
+    const int32_t r = radii[idx];
+    if (r <= 0) return 0;
+    uint32_t x0, y0, x1, y1;
+    tile_box(means2d[idx * 2], means2d[idx * 2 + 1], r, ts, tw, th, x0, y0, x1, y1);
+    return (int32_t)((y1 - y0) * (x1 - x0));
+}
+
+// Block-wide exclusive scan of one int64 per thread (256 threads, 4 waves).
+__device__ __forceinline__ int64_t block_exclusive_scan(int64_t v, int64_t *lds_wave, int64_t &total) {
+    const int lane = (int)lane_id(), wave = threadIdx.x >> 6;
+    int64_t inc = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int64_t u = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += u;
+    }
+    if (lane == 63) lds_wave[wave] = inc;
+    __syncthreads();
+    int64_t base = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kThreads / 64; ++w) {
+        const int64_t s = lds_wave[w];
+        if (w < wave) base += s;
+        tot += s;
+    }
+    total = tot;
+    return base + inc - v;
+}
+
+// Pass 1a: tiles_per_gauss and one int64 partial sum per block.
+__global__ void isect_count_kernel(uint64_t total, const float *__restrict__ means2d,
+                                   const int32_t *__restrict__ radii, uint32_t ts, uint32_t tw, uint32_t th,
+                                   int32_t *__restrict__ tpg, int64_t *__restrict__ block_sums) {
+    __shared__ int64_t lds_wave[kThreads / 64];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+    int64_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const uint64_t idx = base + (uint64_t)k * kThreads + threadIdx.x;  // coalesced
+        if (idx < total) {
+            const int32_t n = tile_count(means2d, radii, idx, ts, tw, th);
+            tpg[idx] = n;
+            s += n;
+        }
+    }
+    int64_t tot;
+    block_exclusive_scan(s, lds_wave, tot);
+    if (threadIdx.x == 0) block_sums[blockIdx.x] = tot;
+}
+
+// Pass 1b: exclusive scan of the block sums in one block; the grand total
+// goes to *n_isects.
+__global__ void isect_scan_blocks_kernel(int64_t nb, int64_t *__restrict__ block_sums, int64_t *__restrict__ n_isects) {
+    __shared__ int64_t lds_wave[kThreads / 64];
+    __shared__ int64_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int64_t start = 0; start < nb; start += kThreads) {
+        const int64_t i = start + threadIdx.x;
+        const int64_t v = i < nb ? block_sums[i] : 0;
+        int64_t tot;
+        const int64_t ex = block_exclusive_scan(v, lds_wave, tot);
+        const int64_t c0 = carry;
+        if (i < nb) block_sums[i] = c0 + ex;
+        __syncthreads();
+        if (threadIdx.x == 0) carry = c0 + tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *n_isects = carry;
+}
+
+// Pass 1c: int64 inclusive cumulative sum (blocked: each thread owns 4
+// consecutive elements).
+__global__ void isect_cum_kernel(uint64_t total, const int32_t *__restrict__ tpg,
+                                 const int64_t *__restrict__ block_offsets, int64_t *__restrict__ cum) {
+    __shared__ int64_t lds_wave[kThreads / 64];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+    int32_t v[kScanItems];
+    int64_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        v[k] = (base + k < total) ? tpg[base + k] : 0;
+        s += v[k];
+    }
+    int64_t tot;
+    int64_t run = block_exclusive_scan(s, lds_wave, tot) + block_offsets[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        run += v[k];
+        if (base + k < total) cum[base + k] = run;
+    }
+}
+
+// Pass 2: keys cam | tile | depth bits.  reference: isect_tiles.cu:76-96
+__global__ void isect_emit_kernel(uint32_t C, uint32_t N, const float *__restrict__ means2d,
+                                  const int32_t *__restrict__ radii, const float *__restrict__ depths,
+                                  const int64_t *__restrict__ cum, uint32_t ts, uint32_t tw, uint32_t th,
+                                  uint32_t tile_bits, int64_t *__restrict__ isect_ids,
+                                  int32_t *__restrict__ flatten_ids) {
+    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (uint64_t)C * N) return;
+    const int32_t r = radii[idx];
+    if (r <= 0) return;
+    uint32_t x0, y0, x1, y1;
+    tile_box(means2d[idx * 2], means2d[idx * 2 + 1], r, ts, tw, th, x0, y0, x1, y1);
+    const int64_t cid = (int64_t)(idx / N);
+    const int64_t cid_enc = cid << (32 + tile_bits);
+    const int64_t depth_enc = (int64_t)__float_as_int(depths[idx]);  // sign-extended, as the reference
+    int64_t cur = idx == 0 ? 0 : cum[idx - 1];
+    for (uint32_t i = y0; i < y1; ++i)
+        for (uint32_t j = x0; j < x1; ++j) {
+            const int64_t tile = (int64_t)i * tw + j;
+            isect_ids[cur] = cid_enc | (tile << 32) | depth_enc;
+            flatten_ids[cur] = (int32_t)idx;
+            ++cur;
+        }
+}
+
+// offsets[k] = number of sorted isects whose (cam, tile) index is < k.
+// reference: isect_tiles.cu:287-333
+__global__ void isect_offsets_kernel(int64_t n, const int64_t *__restrict__ ids, uint32_t tile_bits,
+                                     int64_t n_tiles, int64_t total_tiles, int32_t *__restrict__ offsets) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t mask = (1ull << tile_bits) - 1ull;
+    const uint64_t hi = (uint64_t)ids[i] >> 32;
+    const int64_t cur = (int64_t)(hi >> tile_bits) * n_tiles + (int64_t)(hi & mask);
+    if (i == 0) {
+        for (int64_t k = 0; k <= cur && k < total_tiles; ++k) offsets[k] = 0;
+    }
+    if (i == n - 1) {
+        for (int64_t k = cur + 1; k < total_tiles; ++k) offsets[k] = (int32_t)n;
+    }
+    if (i > 0) {
+        const uint64_t hp = (uint64_t)ids[i - 1] >> 32;
+        const int64_t prev = (int64_t)(hp >> tile_bits) * n_tiles + (int64_t)(hp & mask);
+        if (prev == cur) return;
+        for (int64_t k = prev + 1; k <= cur && k < total_tiles; ++k) offsets[k] = (int32_t)i;
+    }
+}
+
+}  // namespace ubs
+
+using namespace ubs;
+
+extern "C" {
+
+size_t ubs_isect_count_workspace_bytes(uint32_t C, uint32_t N) {
+    const uint64_t total = (uint64_t)C * N;
+    const uint64_t nb = (total + kScanTile - 1) / kScanTile;
+    return (size_t)std::max<uint64_t>(nb, 1) * sizeof(int64_t);
+}
+
+uint32_t ubs_isect_prefix_bits(uint32_t C, uint32_t tw, uint32_t th) {
+    return bit_width_u32(tw * th) + bit_width_u32(C);
+}
+
+int ubs_isect_tiles_count(uint32_t C, uint32_t N, const float *means2d, const int32_t *radii, uint32_t ts,
+                          uint32_t tw, uint32_t th, int32_t *tpg, int64_t *cum, int64_t *n_isects_dev,
+                          void *workspace, size_t ws_bytes, void *stream) {
+    hipStream_t st = as_stream(stream);
+    const uint64_t total = (uint64_t)C * N;
+    UBS_REQUIRE(n_isects_dev, "n_isects_dev is null");
+    UBS_REQUIRE(ts > 0, "tile_size must be > 0");
+    UBS_REQUIRE(bit_width_u32(tw * th) + bit_width_u32(C) <= 32, "camera and tile ids need more than 32 bits");
+    if (total == 0) return zero_async(n_isects_dev, sizeof(int64_t), st);
+    UBS_REQUIRE(total < (1ull << 31), "C*N must be < 2^31");
+    UBS_REQUIRE(means2d && radii && tpg && cum, "null pointer");
+    const uint64_t nb = (total + kScanTile - 1) / kScanTile;
+    if (!workspace || ws_bytes < ubs_isect_count_workspace_bytes(C, N)) {
+        set_error("isect workspace too small (%zu < %zu)", ws_bytes, ubs_isect_count_workspace_bytes(C, N));
+        return UBS_ERR_WORKSPACE;
+    }
+    int64_t *bs = reinterpret_cast<int64_t *>(workspace);
+    isect_count_kernel<<<(unsigned)nb, kThreads, 0, st>>>(total, means2d, radii, ts, tw, th, tpg, bs);
+    UBS_LAUNCH_CHECK("isect_count_kernel");
+    isect_scan_blocks_kernel<<<1, kThreads, 0, st>>>((int64_t)nb, bs, n_isects_dev);
+    UBS_LAUNCH_CHECK("isect_scan_blocks_kernel");
+    isect_cum_kernel<<<(unsigned)nb, kThreads, 0, st>>>(total, tpg, bs, cum);
+    UBS_LAUNCH_CHECK("isect_cum_kernel");
+    return UBS_OK;
+}
+
+int ubs_isect_tiles_emit(uint32_t C, uint32_t N, const float *means2d, const int32_t *radii, const float *depths,
+                         const int64_t *cum, uint32_t ts, uint32_t tw, uint32_t th, int64_t *isect_ids,
+                         int32_t *flatten_ids, void *stream) {
+    const uint64_t total = (uint64_t)C * N;
+    if (total == 0) return UBS_OK;
+    UBS_REQUIRE(total < (1ull << 31), "C*N must be < 2^31");
+    UBS_REQUIRE(means2d && radii && depths && cum && isect_ids && flatten_ids, "null pointer");
+    const uint32_t tile_bits = bit_width_u32(tw * th);
+    isect_emit_kernel<<<grid_for((int64_t)total), kThreads, 0, as_stream(stream)>>>(
+        C, N, means2d, radii, depths, cum, ts, tw, th, tile_bits, isect_ids, flatten_ids);
+    UBS_LAUNCH_CHECK("isect_emit_kernel");
+    return UBS_OK;
+}
+
+int ubs_isect_offset_encode(int64_t n, const int64_t *ids, uint32_t C, uint32_t tw, uint32_t th, int32_t *offsets,
+                            void *stream) {
+    hipStream_t st = as_stream(stream);
+    const int64_t n_tiles = (int64_t)tw * th, total_tiles = (int64_t)C * n_tiles;
+    if (total_tiles == 0) return UBS_OK;
+    UBS_REQUIRE(offsets, "null pointer");
+    if (n == 0) return zero_async(offsets, (size_t)total_tiles * sizeof(int32_t), st);
+    UBS_REQUIRE(ids, "null pointer");
+    UBS_REQUIRE(n < (1ll << 31), "n_isects must be < 2^31");
+    isect_offsets_kernel<<<grid_for(n), kThreads, 0, st>>>(n, ids, bit_width_u32((uint32_t)n_tiles), n_tiles,
+                                                           total_tiles, offsets);
+    UBS_LAUNCH_CHECK("isect_offsets_kernel");
+    return UBS_OK;
+}
+
+}  // extern "C"

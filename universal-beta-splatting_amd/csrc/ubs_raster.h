@@ -1,0 +1,88 @@
+// Shared pieces of the forward and backward tile blend kernels.
+#pragma once
+
+#include "ubs_common.h"
+
+namespace ubs {
+
+// Beta kernel weight (1 - sigma)^beta with the hardware log2/exp2
+// (v_log_f32 / v_exp_f32).  The forward and the backward call this same
+// function, so the backward recovers transmittance from bit-identical alphas.
+__device__ __forceinline__ float beta_pow(float one_minus_sigma, float beta) {
+    return __builtin_amdgcn_exp2f(beta * __builtin_amdgcn_logf(one_minus_sigma));
+}
+
+// sigma = (a dx^2 + c dy^2) + 2b dx dy, the reference's grouping
+// (rasterize_to_pixels_fwd.cu:146-148); b2 = 2 b is exact.
+__device__ __forceinline__ float beta_sigma(float a, float b2, float c, float dx, float dy) {
+    return (a * dx * dx + c * dy * dy) + b2 * dx * dy;
+}
+
+// Workgroup -> (camera, tile) with an XCD-aware order.  Workgroups are
+// dispatched round-robin over the 8 XCDs (workgroup i runs on XCD i % 8), and
+// each XCD has its own L2; giving every XCD one contiguous band of tiles keeps
+// the primitives that neighbouring tiles share inside one L2.
+constexpr int kNumXcd = 8;
+
+__host__ __device__ inline uint32_t xcd_grid(uint32_t total) {
+    const uint32_t per = (total + kNumXcd - 1) / kNumXcd;
+    return per * kNumXcd;
+}
+
+__device__ __forceinline__ bool xcd_tile(uint32_t bid, uint32_t total, uint32_t &logical) {
+    const uint32_t per = (total + kNumXcd - 1) / kNumXcd;
+    logical = (bid % kNumXcd) * per + bid / kNumXcd;
+    return logical < total;
+}
+
+// Half extents of the sigma < 1 ellipse of conic [[a, b], [b, c]]:
+// |dx| <= sqrt(c / det), |dy| <= sqrt(a / det), padded by 0.1% + 1e-3 px so
+// that every pixel outside the box has fp32 sigma >= 1 (culling never drops
+// a pair the per-pixel test would keep).
+__device__ __forceinline__ void support_box(float a, float b, float c, float &hx, float &hy) {
+    const float det = a * c - b * b;
+    if (det > 0.f) {
+        hx = sqrtf(c / det) * 1.001f + 1e-3f;
+        hy = sqrtf(a / det) * 1.001f + 1e-3f;
+    } else {
+        hx = __builtin_huge_valf();
+        hy = __builtin_huge_valf();
+    }
+}
+
+// Bit (sy * SB + sx) set when the padded support box of a primitive centred
+// at (mx, my) reaches a pixel centre of 8x8 sub-block (sx, sy) of the tile
+// whose first pixel is (x0, y0).
+template <int SB>
+__device__ __forceinline__ uint32_t subblock_mask(float mx, float my, float hx, float hy, float x0, float y0) {
+    uint32_t mx_bits = 0, my_bits = 0;
+#pragma unroll
+    for (int s = 0; s < SB; ++s) {
+        const float lo_x = x0 + (float)(s * 8) + 0.5f, hi_x = lo_x + 7.f;
+        const float lo_y = y0 + (float)(s * 8) + 0.5f, hi_y = lo_y + 7.f;
+        if (mx + hx >= lo_x && mx - hx <= hi_x) mx_bits |= 1u << s;
+        if (my + hy >= lo_y && my - hy <= hi_y) my_bits |= 1u << s;
+    }
+    uint32_t m = 0;
+#pragma unroll
+    for (int sy = 0; sy < SB; ++sy)
+#pragma unroll
+        for (int sx = 0; sx < SB; ++sx)
+            if (((my_bits >> sy) & 1u) && ((mx_bits >> sx) & 1u)) m |= 1u << (sy * SB + sx);
+    return m;
+}
+
+// Channel chunks the kernels are instantiated for; wider colour vectors are
+// blended chunk by chunk (the transmittance does not depend on colour).  The
+// widest chunk is bounded by LDS: 32 channels for 8x8 and 16x16 tiles, 4 for
+// 32x32 tiles.
+constexpr int kChunkSizes[] = {32, 16, 8, 4, 3, 2, 1};
+
+inline int pick_chunk(uint32_t remaining, uint32_t tile_size) {
+    const uint32_t cap = tile_size == 32 ? 4u : 32u;
+    for (int k : kChunkSizes)
+        if ((uint32_t)k <= remaining && (uint32_t)k <= cap) return k;
+    return 1;
+}
+
+}  // namespace ubs

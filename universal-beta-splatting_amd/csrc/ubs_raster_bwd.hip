@@ -1,0 +1,331 @@
+// Backward tile blend of Beta primitives on gfx950.
+//
+// Same workgroup/tile/sub-block layout as the forward.  Batches are walked
+// back to front; each wave compacts a batch to the primitives whose support
+// box reaches its 8x8 sub-block and whose isect index is at most the wave's
+// largest last contributor, then per primitive:
+//   * recovers each pixel's transmittance before the primitive from the
+//     exact T after the last contributor (kept by the forward) by dividing
+//     by (1 - alpha) -- the forward and backward evaluate alpha with the same
+//     function, so the divisors are bit-identical;
+//   * forms the gradients of rasterize_to_pixels_bwd.cu:200-247;
+//   * sums them over the 64 lanes with DPP butterflies, and one lane adds the
+//     totals into an LDS accumulator of the primitive (ds_add_f32), so the
+//     four waves of a tile share one set of partial sums;
+// and after the batch each thread flushes one primitive's accumulator to the
+// per-primitive gradients with hardware float atomics (one set of atomics per
+// (tile, primitive) instead of one per (wave, primitive)).
+// The T recovery can run in fp64 (template flag) where stability needs it.
+#pragma clang fp contract(off)
+
+#include <type_traits>
+
+#include "ubs_raster.h"
+
+namespace ubs {
+
+template <int TS, int CDIM, bool F64T>
+__global__ __launch_bounds__(TS *TS) void raster_bwd_kernel(
+    uint32_t C, uint32_t N, int64_t n_isects, uint32_t channels, uint32_t coff,
+    const float *__restrict__ means2d, const float *__restrict__ conics, const float *__restrict__ colors,
+    const float *__restrict__ opacities, const float *__restrict__ betas, const float *__restrict__ backgrounds,
+    const uint8_t *__restrict__ masks, uint32_t W, uint32_t H, uint32_t tw, uint32_t th,
+    const int32_t *__restrict__ tile_offsets, const int32_t *__restrict__ flatten_ids,
+    const float *__restrict__ render_alphas, const int32_t *__restrict__ last_ids, const float *__restrict__ T_final,
+    const float *__restrict__ v_render_colors, const float *__restrict__ v_render_alphas,
+    float *__restrict__ v_means2d, float *__restrict__ v_conics, float *__restrict__ v_colors,
+    float *__restrict__ v_opacities, float *__restrict__ v_betas) {
+    constexpr int BS = TS * TS;
+    constexpr int SB = TS / 8;
+    constexpr int NW = BS / 64;
+    constexpr int NV = 7 + CDIM;  // xy(2) conic(3) opacity beta colour(CDIM)
+    __shared__ float4 s_geo[BS];   // x, y, a, b
+    __shared__ float4 s_geo2[BS];  // c, opacity, beta, id (bits)
+    __shared__ float s_col[CDIM][BS];
+    __shared__ uint32_t s_mask[BS];
+    __shared__ uint16_t s_list[NW][BS];
+    __shared__ float s_acc[NV][BS];
+    __shared__ uint8_t s_hit[BS];
+    __shared__ int32_t s_wmax[NW];
+
+    uint32_t logical;
+    if (!xcd_tile(blockIdx.x, C * th * tw, logical)) return;
+    const uint32_t cam = logical / (th * tw);
+    const uint32_t tile = logical % (th * tw);
+    const uint32_t ty = tile / tw, tx = tile % tw;
+    if (masks && !masks[(size_t)cam * th * tw + tile]) return;
+
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int sx = wave % SB, sy = wave / SB;
+    const uint32_t i = ty * TS + sy * 8 + (lane >> 3);
+    const uint32_t j = tx * TS + sx * 8 + (lane & 7);
+    const bool inside = i < H && j < W;
+    const int64_t pix = (int64_t)cam * H * W + (int64_t)i * W + j;
+    const bool first_chunk = (coff == 0);
+
+    const int64_t tbase = (int64_t)cam * th * tw;
+    const int32_t range_start = tile_offsets[tbase + tile];
+    const int32_t range_end = (cam == C - 1 && tile == th * tw - 1) ? (int32_t)n_isects : tile_offsets[tbase + tile + 1];
+    const float px = (float)j + 0.5f, py = (float)i + 0.5f;
+    const float x0 = (float)(tx * TS), y0 = (float)(ty * TS);
+
+    // per-pixel state
+    const float Tf = inside ? (T_final ? T_final[pix] : 1.f - render_alphas[pix]) : 1.f;
+    using TT = typename std::conditional<F64T, double, float>::type;
+    TT T = (TT)Tf;
+    const int32_t bin_final = inside ? last_ids[pix] : 0;
+    float vrc[CDIM], buf[CDIM];
+    float bg_dot = 0.f;
+#pragma unroll
+    for (int k = 0; k < CDIM; ++k) {
+        vrc[k] = inside ? v_render_colors[pix * channels + coff + k] : 0.f;
+        buf[k] = 0.f;
+        if (backgrounds) bg_dot = __builtin_fmaf(backgrounds[cam * channels + coff + k], vrc[k], bg_dot);
+    }
+    const float vra = (inside && first_chunk) ? v_render_alphas[pix] : 0.f;
+
+    // the largest last contributor over the wave bounds the useful range
+    {
+        const int32_t wm = wave_max_i32(bin_final);
+        if (lane == 0) s_wmax[wave] = wm;
+    }
+    for (int k = tid; k < NV * BS; k += BS) (&s_acc[0][0])[k] = 0.f;
+    if (tid < BS) s_hit[tid] = 0;
+    __syncthreads();
+    const int32_t wave_bin_max = s_wmax[wave];
+
+    const int32_t nb = range_end > range_start ? (range_end - range_start + BS - 1) / BS : 0;
+    for (int32_t b = 0; b < nb; ++b) {
+        const int32_t batch_end = range_end - 1 - b * BS;  // isect index of entry t is batch_end - t
+        const int32_t cnt = min(BS, batch_end + 1 - range_start);
+        if (tid < cnt) {
+            const int32_t g = flatten_ids[batch_end - tid];
+            const float2 xy = reinterpret_cast<const float2 *>(means2d)[g];
+            const float ca = conics[(int64_t)g * 3], cb = conics[(int64_t)g * 3 + 1], cc = conics[(int64_t)g * 3 + 2];
+            s_geo[tid] = make_float4(xy.x, xy.y, ca, cb);
+            s_geo2[tid] = make_float4(cc, opacities[g], betas[g], __int_as_float(g));
+            const float *cp = colors + (int64_t)g * channels + coff;
+#pragma unroll
+            for (int k = 0; k < CDIM; ++k) s_col[k][tid] = cp[k];
+            float hx, hy;
+            support_box(ca, cb, cc, hx, hy);
+            s_mask[tid] = subblock_mask<SB>(xy.x, xy.y, hx, hy, x0, y0);
+        }
+        __syncthreads();
+        int n_mine = 0;
+        for (int c0 = 0; c0 < cnt; c0 += 64) {
+            const int e = c0 + lane;
+            const bool hit = e < cnt && ((s_mask[e] >> wave) & 1u) && (batch_end - e) <= wave_bin_max;
+            const uint64_t bal = __ballot(hit);
+            if (hit) {
+                const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+                s_list[wave][n_mine + __popcll(bal & lt)] = (uint16_t)e;
+            }
+            n_mine += __popcll(bal);
+        }
+        for (int k = 0; k < n_mine; ++k) {
+            const int t = s_list[wave][k];
+            const float4 g1 = s_geo[t];
+            const float4 g2 = s_geo2[t];
+            const float dx = g1.x - px, dy = g1.y - py;
+            const float sigma = __builtin_fmaf(2.f * g1.w * dx, dy, __builtin_fmaf(g1.z * dx, dx, (g2.x * dy) * dy));
+            const bool valid = inside && (batch_end - t) <= bin_final && sigma >= 0.f && sigma < 1.f;
+            if (__ballot(valid) == 0ull) continue;
+            float v[NV];
+#pragma unroll
+            for (int q = 0; q < NV; ++q) v[q] = 0.f;
+            if (valid) {
+                const float opac = g2.y, beta = g2.z;
+                const float oms = 1.f - sigma;
+                const float lg = __builtin_amdgcn_logf(oms);  // log2(1 - sigma)
+                const float vis = __builtin_amdgcn_exp2f(beta * lg);
+                const float alpha = fminf(0.999f, opac * vis);
+                float ra;
+                if constexpr (F64T) {
+                    const double rad = 1.0 / (1.0 - (double)alpha);
+                    T *= rad;
+                    ra = (float)rad;
+                } else {
+                    ra = __builtin_amdgcn_rcpf(1.f - alpha);
+                    T *= ra;
+                }
+                const float Tc = (float)T;
+                const float fac = alpha * Tc;
+                float v_alpha = 0.f;
+#pragma unroll
+                for (int q = 0; q < CDIM; ++q) {
+                    v[7 + q] = fac * vrc[q];
+                    v_alpha = __builtin_fmaf(s_col[q][t] * Tc - buf[q] * ra, vrc[q], v_alpha);
+                }
+                v_alpha = __builtin_fmaf(Tf * ra, vra - bg_dot, v_alpha);
+                if (opac * vis <= 0.999f) {
+                    const float pm1 = __builtin_amdgcn_exp2f((beta - 1.f) * lg);  // (1-sigma)^(beta-1)
+                    const float v_sigma = -v_alpha * opac * beta * pm1;
+                    v[2] = dx * dx * v_sigma;
+                    v[3] = 2.f * dx * dy * v_sigma;
+                    v[4] = dy * dy * v_sigma;
+                    v[0] = 2.f * v_sigma * __builtin_fmaf(g1.z, dx, g1.w * dy);
+                    v[1] = 2.f * v_sigma * __builtin_fmaf(g1.w, dx, g2.x * dy);
+                    v[5] = vis * v_alpha;
+                    v[6] = v_alpha * opac * vis * (lg * 0.69314718055994531f);  // ln(1-sigma)
+                }
+#pragma unroll
+                for (int q = 0; q < CDIM; ++q) buf[q] = __builtin_fmaf(s_col[q][t], fac, buf[q]);
+            }
+            float tot[NV];
+#pragma unroll
+            for (int q = 0; q < NV; ++q) tot[q] = wave_sum(v[q]);
+            if (lane == 0) {
+#pragma unroll
+                for (int q = 0; q < NV; ++q) atomicAdd(&s_acc[q][t], tot[q]);
+                s_hit[t] = 1;
+            }
+        }
+        __syncthreads();
+        // ---- flush this batch's accumulators to the per-primitive gradients
+        if (tid < cnt && s_hit[tid]) {
+            const int32_t g = __float_as_int(s_geo2[tid].w);
+            float a[NV];
+#pragma unroll
+            for (int q = 0; q < NV; ++q) {
+                a[q] = s_acc[q][tid];
+                s_acc[q][tid] = 0.f;
+            }
+            s_hit[tid] = 0;
+            if (a[0] != 0.f) atomic_add_f32(v_means2d + (int64_t)g * 2, a[0]);
+            if (a[1] != 0.f) atomic_add_f32(v_means2d + (int64_t)g * 2 + 1, a[1]);
+            if (a[2] != 0.f) atomic_add_f32(v_conics + (int64_t)g * 3, a[2]);
+            if (a[3] != 0.f) atomic_add_f32(v_conics + (int64_t)g * 3 + 1, a[3]);
+            if (a[4] != 0.f) atomic_add_f32(v_conics + (int64_t)g * 3 + 2, a[4]);
+            if (a[5] != 0.f) atomic_add_f32(v_opacities + g, a[5]);
+            if (a[6] != 0.f) atomic_add_f32(v_betas + g, a[6]);
+#pragma unroll
+            for (int q = 0; q < CDIM; ++q)
+                if (a[7 + q] != 0.f) atomic_add_f32(v_colors + (int64_t)g * channels + coff + q, a[7 + q]);
+        }
+        __syncthreads();
+    }
+}
+
+template <int TS, int CDIM>
+static int launch_bwd(bool f64t, uint32_t C, uint32_t N, int64_t n_isects, uint32_t channels, uint32_t coff,
+                      const float *means2d, const float *conics, const float *colors, const float *opac,
+                      const float *betas, const float *bg, const uint8_t *masks, uint32_t W, uint32_t H,
+                      uint32_t tw, uint32_t th, const int32_t *offs, const int32_t *flat, const float *ra,
+                      const int32_t *last, const float *Tf, const float *vrc, const float *vra, float *vm2,
+                      float *vcon, float *vcol, float *vop, float *vbe, hipStream_t st) {
+    const uint32_t total = C * th * tw;
+    if (f64t)
+        raster_bwd_kernel<TS, CDIM, true><<<xcd_grid(total), TS * TS, 0, st>>>(
+            C, N, n_isects, channels, coff, means2d, conics, colors, opac, betas, bg, masks, W, H, tw, th, offs, flat,
+            ra, last, Tf, vrc, vra, vm2, vcon, vcol, vop, vbe);
+    else
+        raster_bwd_kernel<TS, CDIM, false><<<xcd_grid(total), TS * TS, 0, st>>>(
+            C, N, n_isects, channels, coff, means2d, conics, colors, opac, betas, bg, masks, W, H, tw, th, offs, flat,
+            ra, last, Tf, vrc, vra, vm2, vcon, vcol, vop, vbe);
+    UBS_LAUNCH_CHECK("raster_bwd_kernel");
+    return UBS_OK;
+}
+
+template <int TS>
+static int launch_bwd_chunk(int cd, bool f64t, uint32_t C, uint32_t N, int64_t n_isects, uint32_t channels,
+                            uint32_t coff, const float *means2d, const float *conics, const float *colors,
+                            const float *opac, const float *betas, const float *bg, const uint8_t *masks, uint32_t W,
+                            uint32_t H, uint32_t tw, uint32_t th, const int32_t *offs, const int32_t *flat,
+                            const float *ra, const int32_t *last, const float *Tf, const float *vrc,
+                            const float *vra, float *vm2, float *vcon, float *vcol, float *vop, float *vbe,
+                            hipStream_t st) {
+#define UBS_BWD_ARGS                                                                                            \
+    f64t, C, N, n_isects, channels, coff, means2d, conics, colors, opac, betas, bg, masks, W, H, tw, th, offs,  \
+        flat, ra, last, Tf, vrc, vra, vm2, vcon, vcol, vop, vbe, st
+    switch (cd) {
+    case 1: return launch_bwd<TS, 1>(UBS_BWD_ARGS);
+    case 2: return launch_bwd<TS, 2>(UBS_BWD_ARGS);
+    case 3: return launch_bwd<TS, 3>(UBS_BWD_ARGS);
+    case 4: return launch_bwd<TS, 4>(UBS_BWD_ARGS);
+    case 8:
+        if constexpr (TS <= 16) return launch_bwd<TS, 8>(UBS_BWD_ARGS);
+        break;
+    case 16:
+        if constexpr (TS <= 16) return launch_bwd<TS, 16>(UBS_BWD_ARGS);
+        break;
+    case 32:
+        if constexpr (TS <= 16) return launch_bwd<TS, 32>(UBS_BWD_ARGS);
+        break;
+    default:
+        break;
+    }
+#undef UBS_BWD_ARGS
+    set_error("internal: bad channel chunk %d", cd);
+    return UBS_ERR_INVALID;
+}
+
+// fp64 transmittance recovery switch (process-wide, read at each call).
+static int g_f64_transmittance = 0;
+
+}  // namespace ubs
+
+using namespace ubs;
+
+extern "C" {
+
+// Selects fp32 (0, default) or fp64 (1) arithmetic for the backward's
+// transmittance recovery.  Not part of the reference interface.
+int ubs_set_backward_fp64_transmittance(int enable) {
+    g_f64_transmittance = enable ? 1 : 0;
+    return UBS_OK;
+}
+
+int ubs_rasterize_to_pixels_bwd(uint32_t C, uint32_t N, int64_t n_isects, uint32_t channels, const float *means2d,
+                                const float *conics, const float *colors, const float *opacities,
+                                const float *betas, const float *backgrounds, const uint8_t *masks, uint32_t W,
+                                uint32_t H, uint32_t tile_size, uint32_t tw, uint32_t th,
+                                const int32_t *tile_offsets, const int32_t *flatten_ids, const float *render_alphas,
+                                const int32_t *last_ids, const float *T_final, const float *v_render_colors,
+                                const float *v_render_alphas, float *v_means2d, float *v_conics, float *v_colors,
+                                float *v_opacities, float *v_betas, void *stream) {
+    hipStream_t st = as_stream(stream);
+    UBS_REQUIRE(tile_size == 8 || tile_size == 16 || tile_size == 32, "tile_size must be 8, 16 or 32 (got %u)",
+                tile_size);
+    UBS_REQUIRE(channels >= 1, "channels must be >= 1");
+    UBS_REQUIRE((uint64_t)tw * tile_size >= W && (uint64_t)th * tile_size >= H, "tile grid does not cover the image");
+    UBS_REQUIRE(n_isects >= 0 && n_isects < (1ll << 31), "n_isects out of range");
+    const uint64_t CN = (uint64_t)C * N;
+    UBS_REQUIRE(CN == 0 || (v_means2d && v_conics && v_colors && v_opacities && v_betas), "null gradient output");
+    int e = zero_async(v_means2d, CN * 2 * sizeof(float), st);
+    if (!e) e = zero_async(v_conics, CN * 3 * sizeof(float), st);
+    if (!e) e = zero_async(v_colors, CN * channels * sizeof(float), st);
+    if (!e) e = zero_async(v_opacities, CN * sizeof(float), st);
+    if (!e) e = zero_async(v_betas, CN * sizeof(float), st);
+    if (e) return e;
+    if (n_isects == 0 || (uint64_t)C * W * H == 0) return UBS_OK;
+    UBS_REQUIRE((uint64_t)C * th * tw < (1ull << 31), "too many tiles");
+    UBS_REQUIRE(means2d && conics && colors && opacities && betas && tile_offsets && flatten_ids && last_ids &&
+                    (render_alphas || T_final) && v_render_colors && v_render_alphas,
+                "null pointer");
+    const bool f64t = g_f64_transmittance != 0;
+    uint32_t coff = 0;
+    while (coff < channels) {
+        const int cd = pick_chunk(channels - coff, tile_size);
+        if (tile_size == 16)
+            e = launch_bwd_chunk<16>(cd, f64t, C, N, n_isects, channels, coff, means2d, conics, colors, opacities,
+                                     betas, backgrounds, masks, W, H, tw, th, tile_offsets, flatten_ids,
+                                     render_alphas, last_ids, T_final, v_render_colors, v_render_alphas, v_means2d,
+                                     v_conics, v_colors, v_opacities, v_betas, st);
+        else if (tile_size == 8)
+            e = launch_bwd_chunk<8>(cd, f64t, C, N, n_isects, channels, coff, means2d, conics, colors, opacities,
+                                    betas, backgrounds, masks, W, H, tw, th, tile_offsets, flatten_ids,
+                                    render_alphas, last_ids, T_final, v_render_colors, v_render_alphas, v_means2d,
+                                    v_conics, v_colors, v_opacities, v_betas, st);
+        else
+            e = launch_bwd_chunk<32>(cd, f64t, C, N, n_isects, channels, coff, means2d, conics, colors, opacities,
+                                     betas, backgrounds, masks, W, H, tw, th, tile_offsets, flatten_ids,
+                                     render_alphas, last_ids, T_final, v_render_colors, v_render_alphas, v_means2d,
+                                     v_conics, v_colors, v_opacities, v_betas, st);
+        if (e) return e;
+        coff += (uint32_t)cd;
+    }
+    return UBS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,242 @@
+// Forward tile blend of Beta primitives on gfx950.
+//
+// One workgroup per (camera, tile), TS x TS threads.  Each wave64 owns an 8x8
+// pixel sub-block of the tile (lane l -> pixel (l & 7, l >> 3) of it), which
+// is the unit of culling: a batch of TS*TS primitives is staged in LDS, every
+// primitive gets a bit mask of the sub-blocks its sigma < 1 support box
+// reaches, and each wave compacts the batch to the primitives that touch its
+// sub-block (ballot + popcount) before the per-pixel front-to-back loop.
+// Pixels stop at the first primitive that would take transmittance to
+// <= 1e-4 (exclusive), as rasterize_to_pixels_fwd.cu:150-172 does; a wave
+// leaves the batch as soon as all of its pixels are done and the workgroup
+// stops loading batches once every pixel is done.
+#pragma clang fp contract(off)
+
+#include "ubs_raster.h"
+
+namespace ubs {
+
+template <int TS, int CDIM>
+__global__ __launch_bounds__(TS *TS) void raster_fwd_kernel(
+    uint32_t C, uint32_t N, int64_t n_isects, uint32_t channels, uint32_t coff,
+    const float *__restrict__ means2d, const float *__restrict__ conics, const float *__restrict__ colors,
+    const float *__restrict__ opacities, const float *__restrict__ betas, const float *__restrict__ backgrounds,
+    const uint8_t *__restrict__ masks, uint32_t W, uint32_t H, uint32_t tw, uint32_t th,
+    const int32_t *__restrict__ tile_offsets, const int32_t *__restrict__ flatten_ids,
+    float *__restrict__ render_colors, float *__restrict__ render_alphas, int32_t *__restrict__ last_ids,
+    float *__restrict__ T_final) {
+    constexpr int BS = TS * TS;        // threads = batch size
+    constexpr int SB = TS / 8;         // sub-blocks per tile side
+    constexpr int NW = BS / 64;        // waves
+    __shared__ float4 s_geo[BS];       // x, y, a, 2b
+    __shared__ float4 s_geo2[BS];      // c, opacity, beta, (unused)
+    __shared__ float s_col[CDIM][BS];
+    __shared__ uint32_t s_mask[BS];
+    __shared__ uint16_t s_list[NW][BS];
+
+    uint32_t logical;
+    if (!xcd_tile(blockIdx.x, C * th * tw, logical)) return;
+    const uint32_t cam = logical / (th * tw);
+    const uint32_t tile = logical % (th * tw);
+    const uint32_t ty = tile / tw, tx = tile % tw;
+
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int sx = wave % SB, sy = wave / SB;
+    const uint32_t i = ty * TS + sy * 8 + (lane >> 3);
+    const uint32_t j = tx * TS + sx * 8 + (lane & 7);
+    const bool inside = i < H && j < W;
+    const int64_t pix = (int64_t)cam * H * W + (int64_t)i * W + j;
+    const bool write_meta = (coff == 0);
+
+    if (masks && !masks[(size_t)cam * th * tw + tile]) {
+        if (inside) {
+#pragma unroll
+            for (int k = 0; k < CDIM; ++k)
+                render_colors[pix * channels + coff + k] = backgrounds ? backgrounds[cam * channels + coff + k] : 0.f;
+            if (write_meta) {
+                render_alphas[pix] = 0.f;
+                last_ids[pix] = 0;
+                if (T_final) T_final[pix] = 1.f;
+            }
+        }
+        return;
+    }
+
+    const int64_t tbase = (int64_t)cam * th * tw;
+    const int32_t range_start = tile_offsets[tbase + tile];
+    const int32_t range_end = (cam == C - 1 && tile == th * tw - 1) ? (int32_t)n_isects : tile_offsets[tbase + tile + 1];
+    const float px = (float)j + 0.5f, py = (float)i + 0.5f;
+    const float x0 = (float)(tx * TS), y0 = (float)(ty * TS);
+
+    float T = 1.f;
+    int32_t cur = 0;
+    bool done = !inside;
+    float acc[CDIM];
+#pragma unroll
+    for (int k = 0; k < CDIM; ++k) acc[k] = 0.f;
+
+    const int32_t nb = range_end > range_start ? (range_end - range_start + BS - 1) / BS : 0;
+    for (int32_t b = 0; b < nb; ++b) {
+        if (__syncthreads_count(done) >= BS) break;
+        const int32_t batch_start = range_start + b * BS;
+        const int32_t cnt = min(BS, range_end - batch_start);
+        // ---- stage one primitive per thread
+        if (tid < cnt) {
+            const int32_t g = flatten_ids[batch_start + tid];
+            const float2 xy = reinterpret_cast<const float2 *>(means2d)[g];
+            const float ca = conics[(int64_t)g * 3], cb = conics[(int64_t)g * 3 + 1], cc = conics[(int64_t)g * 3 + 2];
+            s_geo[tid] = make_float4(xy.x, xy.y, ca, 2.f * cb);
+            s_geo2[tid] = make_float4(cc, opacities[g], betas[g], 0.f);
+            const float *cp = colors + (int64_t)g * channels + coff;
+#pragma unroll
+            for (int k = 0; k < CDIM; ++k) s_col[k][tid] = cp[k];
+            float hx, hy;
+            support_box(ca, cb, cc, hx, hy);
+            s_mask[tid] = subblock_mask<SB>(xy.x, xy.y, hx, hy, x0, y0);
+        }
+        __syncthreads();
+        // ---- per-wave compaction of the batch to this sub-block's primitives
+        int n_mine = 0;
+        for (int c0 = 0; c0 < cnt; c0 += 64) {
+            const int e = c0 + lane;
+            const bool hit = e < cnt && ((s_mask[e] >> wave) & 1u);
+            const uint64_t bal = __ballot(hit);
+            if (hit) {
+                const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+                s_list[wave][n_mine + __popcll(bal & lt)] = (uint16_t)e;
+            }
+            n_mine += __popcll(bal);
+        }
+        // (s_list[wave] is written and read by this wave only: LDS is in order)
+        // ---- front-to-back blend over the compacted list
+        for (int k = 0; k < n_mine; ++k) {
+            if (__ballot(!done) == 0ull) break;
+            const int t = s_list[wave][k];
+            const float4 g1 = s_geo[t];
+            const float4 g2 = s_geo2[t];
+            const float dx = g1.x - px, dy = g1.y - py;
+            const float sigma = __builtin_fmaf(g1.w * dx, dy, __builtin_fmaf(g1.z * dx, dx, (g2.x * dy) * dy));
+            if (!done && sigma >= 0.f && sigma < 1.f) {
+                const float alpha = fminf(0.999f, g2.y * beta_pow(1.f - sigma, g2.z));
+                const float nT = T * (1.f - alpha);
+                if (nT <= 1e-4f) {
+                    done = true;
+                } else {
+                    const float vis = alpha * T;
+#pragma unroll
+                    for (int q = 0; q < CDIM; ++q) acc[q] = __builtin_fmaf(s_col[q][t], vis, acc[q]);
+                    cur = batch_start + t;
+                    T = nT;
+                }
+            }
+        }
+    }
+
+    if (inside) {
+#pragma unroll
+        for (int k = 0; k < CDIM; ++k)
+            render_colors[pix * channels + coff + k] =
+                backgrounds ? __builtin_fmaf(T, backgrounds[cam * channels + coff + k], acc[k]) : acc[k];
+        if (write_meta) {
+            render_alphas[pix] = 1.f - T;
+            last_ids[pix] = cur;
+            if (T_final) T_final[pix] = T;
+        }
+    }
+}
+
+template <int TS, int CDIM>
+static int launch_fwd(uint32_t C, uint32_t N, int64_t n_isects, uint32_t channels, uint32_t coff,
+                      const float *means2d, const float *conics, const float *colors, const float *opac,
+                      const float *betas, const float *bg, const uint8_t *masks, uint32_t W, uint32_t H,
+                      uint32_t tw, uint32_t th, const int32_t *offs, const int32_t *flat, float *rc, float *ra,
+                      int32_t *last, float *Tf, hipStream_t st) {
+    const uint32_t total = C * th * tw;
+    raster_fwd_kernel<TS, CDIM><<<xcd_grid(total), TS * TS, 0, st>>>(
+        C, N, n_isects, channels, coff, means2d, conics, colors, opac, betas, bg, masks, W, H, tw, th, offs, flat,
+        rc, ra, last, Tf);
+    UBS_LAUNCH_CHECK("raster_fwd_kernel");
+    return UBS_OK;
+}
+
+template <int TS>
+static int launch_fwd_chunk(int cd, uint32_t C, uint32_t N, int64_t n_isects, uint32_t channels, uint32_t coff,
+                            const float *means2d, const float *conics, const float *colors, const float *opac,
+                            const float *betas, const float *bg, const uint8_t *masks, uint32_t W, uint32_t H,
+                            uint32_t tw, uint32_t th, const int32_t *offs, const int32_t *flat, float *rc,
+                            float *ra, int32_t *last, float *Tf, hipStream_t st) {
+#define UBS_FWD_CASE(K)                                                                                     \
+    case K:                                                                                                 \
+        return launch_fwd<TS, K>(C, N, n_isects, channels, coff, means2d, conics, colors, opac, betas, bg,  \
+                                 masks, W, H, tw, th, offs, flat, rc, ra, last, Tf, st);
+    switch (cd) {
+        UBS_FWD_CASE(1)
+        UBS_FWD_CASE(2)
+        UBS_FWD_CASE(3)
+        UBS_FWD_CASE(4)
+    case 8:
+        if constexpr (TS <= 16)
+            return launch_fwd<TS, 8>(C, N, n_isects, channels, coff, means2d, conics, colors, opac, betas, bg,
+                                     masks, W, H, tw, th, offs, flat, rc, ra, last, Tf, st);
+        break;
+    case 16:
+        if constexpr (TS <= 16)
+            return launch_fwd<TS, 16>(C, N, n_isects, channels, coff, means2d, conics, colors, opac, betas, bg,
+                                      masks, W, H, tw, th, offs, flat, rc, ra, last, Tf, st);
+        break;
+    case 32:
+        if constexpr (TS <= 16)
+            return launch_fwd<TS, 32>(C, N, n_isects, channels, coff, means2d, conics, colors, opac, betas, bg,
+                                      masks, W, H, tw, th, offs, flat, rc, ra, last, Tf, st);
+        break;
+    default:
+        break;
+    }
+#undef UBS_FWD_CASE
+    set_error("internal: bad channel chunk %d", cd);
+    return UBS_ERR_INVALID;
+}
+
+}  // namespace ubs
+
+using namespace ubs;
+
+extern "C" int ubs_rasterize_to_pixels_fwd(uint32_t C, uint32_t N, int64_t n_isects, uint32_t channels,
+                                           const float *means2d, const float *conics, const float *colors,
+                                           const float *opacities, const float *betas, const float *backgrounds,
+                                           const uint8_t *masks, uint32_t W, uint32_t H, uint32_t tile_size,
+                                           uint32_t tw, uint32_t th, const int32_t *tile_offsets,
+                                           const int32_t *flatten_ids, float *render_colors, float *render_alphas,
+                                           int32_t *last_ids, float *T_final, void *stream) {
+    hipStream_t st = as_stream(stream);
+    UBS_REQUIRE(tile_size == 8 || tile_size == 16 || tile_size == 32, "tile_size must be 8, 16 or 32 (got %u)",
+                tile_size);
+    UBS_REQUIRE(channels >= 1, "channels must be >= 1");
+    UBS_REQUIRE((uint64_t)tw * tile_size >= W && (uint64_t)th * tile_size >= H, "tile grid does not cover the image");
+    UBS_REQUIRE(n_isects >= 0 && n_isects < (1ll << 31), "n_isects out of range");
+    if ((uint64_t)C * W * H == 0) return UBS_OK;
+    UBS_REQUIRE((uint64_t)C * th * tw < (1ull << 31), "too many tiles");
+    UBS_REQUIRE(means2d && conics && colors && opacities && betas && tile_offsets && render_colors && render_alphas &&
+                    last_ids && (flatten_ids || n_isects == 0),
+                "null pointer");
+    uint32_t coff = 0;
+    while (coff < channels) {
+        const int cd = pick_chunk(channels - coff, tile_size);
+        int e;
+        if (tile_size == 16)
+            e = launch_fwd_chunk<16>(cd, C, N, n_isects, channels, coff, means2d, conics, colors, opacities, betas,
+                                     backgrounds, masks, W, H, tw, th, tile_offsets, flatten_ids, render_colors,
+                                     render_alphas, last_ids, T_final, st);
+        else if (tile_size == 8)
+            e = launch_fwd_chunk<8>(cd, C, N, n_isects, channels, coff, means2d, conics, colors, opacities, betas,
+                                    backgrounds, masks, W, H, tw, th, tile_offsets, flatten_ids, render_colors,
+                                    render_alphas, last_ids, T_final, st);
+        else
+            e = launch_fwd_chunk<32>(cd, C, N, n_isects, channels, coff, means2d, conics, colors, opacities, betas,
+                                     backgrounds, masks, W, H, tw, th, tile_offsets, flatten_ids, render_colors,
+                                     render_alphas, last_ids, T_final, st);
+        if (e) return e;
+        coff += (uint32_t)cd;
+    }
+    return UBS_OK;
+}

@@ -1,0 +1,306 @@
+// Onesweep LSD radix sort of (int64 key, int32 value) pairs on gfx950.
+//
+// Stable ordering on key bits [begin_bit, end_bit), the contract of
+// cub::DeviceRadixSort::SortPairs as the reference calls it
+// (isect_tiles.cu:224-278).  Design:
+//   * one histogram launch counts the 8-bit digits of every pass at once
+//     (keys read once), a tiny launch turns them into per-pass exclusive
+//     digit offsets;
+//   * one launch per pass: a workgroup takes the next tile id from an atomic
+//     counter (so every tile it waits on is already running), ranks its 4096
+//     keys with wave64 ballots (match-by-bits, stable: lane order inside an
+//     item, item order inside a wave, wave order inside the tile), publishes
+//     its digit counts, stages the tile in LDS in digit order, resolves its
+//     global prefix by decoupled look-back over earlier tiles, and writes
+//     out runs of equal digits as contiguous (coalesced) segments.
+//   * every look-back wait is bounded: past the spin limit it raises the
+//     status word and stops waiting (the result is then invalid and the
+//     caller is told through ubs_radix_sort_status).
+#include "ubs_common.h"
+
+namespace ubs {
+
+constexpr int kRadixBits = 8;
+constexpr int kRadix = 1 << kRadixBits;
+constexpr int kSortThreads = 256;
+constexpr int kSortWaves = kSortThreads / 64;
+constexpr int kItems = 16;
+constexpr int kTile = kSortThreads * kItems;  // 4096 keys per workgroup
+constexpr uint32_t kFlagAgg = 1u << 30;
+constexpr uint32_t kFlagPrefix = 2u << 30;
+constexpr uint32_t kValueMask = (1u << 30) - 1u;
+constexpr uint32_t kSpinLimit = 1u << 22;
+
+struct SortLayout {
+    size_t status_off, hist_off, counter_off, lookback_off, total;
+    int passes;
+    int64_t tiles;
+};
+
+static SortLayout sort_layout(int64_t n, int begin_bit, int end_bit) {
+    SortLayout L{};
+    L.passes = end_bit > begin_bit ? (end_bit - begin_bit + kRadixBits - 1) / kRadixBits : 0;
+    L.tiles = (n + kTile - 1) / kTile;
+    size_t off = 0;
+    L.status_off = off;
+    off += 256;
+    L.hist_off = off;
+    off += (size_t)std::max(L.passes, 1) * kRadix * sizeof(uint32_t);
+    L.counter_off = off;
+    off += 256;
+    L.lookback_off = off;
+    off += (size_t)std::max(L.passes, 1) * (size_t)std::max<int64_t>(L.tiles, 1) * kRadix * sizeof(uint32_t);
+    L.total = (off + 255) & ~(size_t)255;
+    return L;
+}
+
+__device__ __forceinline__ uint32_t digit_of(int64_t key, int shift, uint32_t mask) {
+    return (uint32_t)(((uint64_t)key >> shift) & mask);
+}
+
+// Digit counts of every pass in one read of the keys.
+__global__ void sort_hist_kernel(int64_t n, const int64_t *__restrict__ keys, int begin_bit, int end_bit,
+                                 int passes, uint32_t *__restrict__ hist) {
+    __shared__ uint32_t h[8][kRadix];
+    for (int k = threadIdx.x; k < passes * kRadix; k += blockDim.x) h[k / kRadix][k % kRadix] = 0;
+    __syncthreads();
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t key = keys[i];
+        for (int p = 0; p < passes; ++p) {
+            const int shift = begin_bit + p * kRadixBits;
+            const int bits = min(kRadixBits, end_bit - shift);
+            atomicAdd(&h[p][digit_of(key, shift, (1u << bits) - 1u)], 1u);
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < passes * kRadix; k += blockDim.x) {
+        const uint32_t v = h[k / kRadix][k % kRadix];
+        if (v) atomicAdd(&hist[k], v);
+    }
+}
+
+// Exclusive scan of each pass's 256 digit counts (one workgroup per pass).
+__global__ void sort_hist_scan_kernel(uint32_t *__restrict__ hist) {
+    __shared__ uint32_t s[kRadix];
+    uint32_t *h = hist + (size_t)blockIdx.x * kRadix;
+    const int d = threadIdx.x;
+    s[d] = h[d];
+    __syncthreads();
+    uint32_t acc = 0;
+    for (int k = 0; k < d; ++k) acc += s[k];
+    h[d] = acc;
+}
+
+__global__ __launch_bounds__(kSortThreads) void sort_pass_kernel(
+    int64_t n, const int64_t *__restrict__ keys_in, int64_t *__restrict__ keys_out,
+    const int32_t *__restrict__ vals_in, int32_t *__restrict__ vals_out, int shift, int bits,
+    const uint32_t *__restrict__ digit_base, uint32_t *__restrict__ tile_counter, uint32_t *lookback,
+    uint32_t *status) {
+    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_wave_cnt[kSortWaves][kRadix];
+    __shared__ uint32_t s_digit_start[kRadix];
+    __shared__ uint32_t s_out_base[kRadix];
+    __shared__ int64_t s_keys[kTile];
+    __shared__ int32_t s_vals[kTile];
+
+    const int tid = threadIdx.x, wave = tid >> 6, lane = (int)lane_id();
+    const uint32_t mask = (1u << bits) - 1u;
+    if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
+    for (int k = tid; k < kSortWaves * kRadix; k += kSortThreads) (&s_wave_cnt[0][0])[k] = 0;
+    __syncthreads();
+    const int64_t tile = s_tile;
+    const int64_t base = tile * kTile;
+
+    // ---- load (striped inside the wave: item i of lane l is wave_base + i*64 + l)
+    int64_t key[kItems];
+    int32_t val[kItems];
+    uint32_t dig[kItems], rank[kItems];
+    const int64_t wave_base = base + (int64_t)wave * 64 * kItems;
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+        const int64_t idx = wave_base + i * 64 + lane;
+        const bool ok = idx < n;
+        key[i] = ok ? keys_in[idx] : 0;
+        val[i] = ok ? vals_in[idx] : 0;
+        dig[i] = digit_of(key[i], shift, mask);
+    }
+
+    // ---- stable rank inside the wave via ballots on the digit bits
+    volatile uint32_t *cnt = s_wave_cnt[wave];
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+        const int64_t idx = wave_base + i * 64 + lane;
+        const bool ok = idx < n;
+        uint64_t peers = __ballot(ok);
+        for (int b = 0; b < bits; ++b) {
+            const bool bit = (dig[i] >> b) & 1u;
+            const uint64_t bal = __ballot(bit);
+            peers &= bit ? bal : ~bal;
+        }
+        uint32_t r = 0;
+        if (ok) {
+            const uint32_t before = (uint32_t)__popcll(peers & lt_mask);
+            const uint32_t c0 = cnt[dig[i]];
+            r = c0 + before;
+            if (before == 0) cnt[dig[i]] = c0 + (uint32_t)__popcll(peers);
+        }
+        rank[i] = r;
+    }
+    __syncthreads();
+
+    // ---- tile digit totals, per-wave exclusive offsets; publish aggregate
+    uint32_t tot = 0;
+    {
+        const int d = tid;  // kSortThreads == kRadix
+#pragma unroll
+        for (int w = 0; w < kSortWaves; ++w) {
+            const uint32_t c = s_wave_cnt[w][d];
+            s_wave_cnt[w][d] = tot;
+            tot += c;
+        }
+        uint32_t *slot = lookback + (size_t)tile * kRadix + d;
+        __hip_atomic_store(slot, (tile == 0 ? kFlagPrefix : kFlagAgg) | tot, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        s_digit_start[d] = tot;
+    }
+    __syncthreads();
+    // exclusive scan over digits (256 entries, one per thread)
+    {
+        const int d = tid;
+        uint32_t v = s_digit_start[d];
+        uint32_t inc = v;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t u = __shfl_up(inc, o, 64);
+            if (lane >= o) inc += u;
+        }
+        __syncthreads();
+        if (lane == 63) s_out_base[wave] = inc;  // wave totals (scratch)
+        __syncthreads();
+        uint32_t wbase = 0;
+#pragma unroll
+        for (int w = 0; w < kSortWaves; ++w)
+            if (w < wave) wbase += s_out_base[w];
+        __syncthreads();
+        s_digit_start[d] = wbase + inc - v;
+    }
+    __syncthreads();
+
+    // ---- stage the tile in LDS in digit order
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+        const int64_t idx = wave_base + i * 64 + lane;
+        if (idx < n) {
+            const uint32_t pos = s_digit_start[dig[i]] + s_wave_cnt[wave][dig[i]] + rank[i];
+            s_keys[pos] = key[i];
+            s_vals[pos] = val[i];
+        }
+    }
+
+    // ---- decoupled look-back for this tile's global prefix of digit d
+    {
+        const int d = tid;
+        uint32_t excl = 0;
+        if (tile > 0) {
+            int64_t j = tile - 1;
+            uint32_t spins = 0;
+            while (j >= 0) {
+                const uint32_t s = __hip_atomic_load(lookback + (size_t)j * kRadix + d, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+                const uint32_t flag = s & ~kValueMask;
+                if (flag == 0u) {
+                    if (++spins > kSpinLimit) {
+                        __hip_atomic_store(status, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                excl += s & kValueMask;
+                if (flag == kFlagPrefix) break;
+                --j;
+            }
+            __hip_atomic_store(lookback + (size_t)tile * kRadix + d, kFlagPrefix | ((excl + tot) & kValueMask),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        s_out_base[d] = digit_base[d] + excl - s_digit_start[d];
+    }
+    __syncthreads();
+
+    // ---- write out: consecutive LDS slots of one digit go to consecutive
+    // global slots
+    const int64_t count = min((int64_t)kTile, n - base);
+    for (int j = tid; j < count; j += kSortThreads) {
+        const int64_t k = s_keys[j];
+        const uint32_t pos = s_out_base[digit_of(k, shift, mask)] + (uint32_t)j;
+        if ((int64_t)pos < n) {  // always true unless a look-back wait gave up
+            keys_out[pos] = k;
+            vals_out[pos] = s_vals[j];
+        }
+    }
+}
+
+}  // namespace ubs
+
+using namespace ubs;
+
+extern "C" {
+
+size_t ubs_radix_sort_workspace_bytes(int64_t n, int begin_bit, int end_bit) {
+    return sort_layout(n, begin_bit, end_bit).total;
+}
+
+int ubs_radix_sort_pairs(int64_t n, int64_t *keys, int64_t *keys_alt, int32_t *vals, int32_t *vals_alt,
+                         int begin_bit, int end_bit, void *workspace, size_t ws_bytes, int *result_in_alt,
+                         void *stream) {
+    hipStream_t st = as_stream(stream);
+    UBS_REQUIRE(result_in_alt, "result_in_alt is null");
+    UBS_REQUIRE(begin_bit >= 0 && end_bit <= 64 && begin_bit <= end_bit, "bad bit range [%d, %d)", begin_bit,
+                end_bit);
+    UBS_REQUIRE(n >= 0 && n <= (int64_t)kValueMask, "n=%lld out of range (max 2^30-1)", (long long)n);
+    const SortLayout L = sort_layout(n, begin_bit, end_bit);
+    UBS_REQUIRE(L.passes <= 8, "too many passes");
+    *result_in_alt = 0;
+    if (!workspace || ws_bytes < L.total) {
+        set_error("sort workspace too small (%zu < %zu)", ws_bytes, L.total);
+        return UBS_ERR_WORKSPACE;
+    }
+    char *ws = reinterpret_cast<char *>(workspace);
+    uint32_t *status = reinterpret_cast<uint32_t *>(ws + L.status_off);
+    int e = zero_async(ws, L.total, st);
+    if (e) return e;
+    if (n == 0 || L.passes == 0) return UBS_OK;
+    UBS_REQUIRE(keys && keys_alt && vals && vals_alt, "null pointer");
+    uint32_t *hist = reinterpret_cast<uint32_t *>(ws + L.hist_off);
+    uint32_t *counters = reinterpret_cast<uint32_t *>(ws + L.counter_off);
+    uint32_t *lookback = reinterpret_cast<uint32_t *>(ws + L.lookback_off);
+    const unsigned hist_blocks = (unsigned)std::min<int64_t>((n + 4095) / 4096, 2048);
+    sort_hist_kernel<<<hist_blocks, 256, 0, st>>>(n, keys, begin_bit, end_bit, L.passes, hist);
+    UBS_LAUNCH_CHECK("sort_hist_kernel");
+    sort_hist_scan_kernel<<<L.passes, kRadix, 0, st>>>(hist);
+    UBS_LAUNCH_CHECK("sort_hist_scan_kernel");
+    int64_t *kb[2] = {keys, keys_alt};
+    int32_t *vb[2] = {vals, vals_alt};
+    for (int p = 0; p < L.passes; ++p) {
+        const int shift = begin_bit + p * kRadixBits;
+        const int bits = std::min(kRadixBits, end_bit - shift);
+        sort_pass_kernel<<<(unsigned)L.tiles, kSortThreads, 0, st>>>(
+            n, kb[p & 1], kb[(p + 1) & 1], vb[p & 1], vb[(p + 1) & 1], shift, bits, hist + (size_t)p * kRadix,
+            counters + p, lookback + (size_t)p * L.tiles * kRadix, status);
+        UBS_LAUNCH_CHECK("sort_pass_kernel");
+    }
+    *result_in_alt = L.passes & 1;
+    return UBS_OK;
+}
+
+int ubs_radix_sort_status(const void *workspace, int *status_out, void *stream) {
+    UBS_REQUIRE(workspace && status_out, "null pointer");
+    uint32_t v = 0;
+    UBS_HIP_CHECK(hipMemcpyAsync(&v, workspace, sizeof(uint32_t), hipMemcpyDeviceToHost, as_stream(stream)));
+    UBS_HIP_CHECK(hipStreamSynchronize(as_stream(stream)));
+    *status_out = (int)v;
+    return UBS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,127 @@
+"""Beta-model glue around the rasteriser: parameter activations, view/time
+query, conditioning and the rasterization() call.
+
+Mirrors BetaModel.render (scene/beta_model.py:660-722) and its getters
+(:36-52, :103-159): scale = softplus, opacity = sigmoid, beta = 4 exp(b),
+mean = cat(xyz, mean_rest), query = normalise(xyz - camera centre) (+ t for
+7-D), conditioned by cond_mean_convariance_opacity and rendered with the
+conditioned 3x3 covariances.  The query carries no gradient into the
+conditioning op, as in the reference.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+from typing import Dict, Optional
+
+import torch
+import torch.nn.functional as F
+from torch import Tensor
+
+from .cuda._wrapper import (
+    cond_mean_convariance_opacity,
+    l_triangle_to_rotmat,
+    rot_scale_l_triangle_to_covar,
+)
+from .rendering import rasterization
+
+
+def quat_to_rotmat(q: Tensor) -> Tensor:
+    """Differentiable rotation from (w, x, y, z) quaternions (normalised here)."""
+    q = F.normalize(q, dim=-1)
+    w, x, y, z = q.unbind(-1)
+    R = torch.stack([
+        1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y),
+        2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x),
+        2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y),
+    ], dim=-1)
+    return R.reshape(q.shape[:-1] + (3, 3))
+
+
+@dataclass
+class BetaParams:
+    """Leaf parameters of a D-dim Beta scene, pre-activation (BetaModel layout)."""
+    xyz: Tensor          # [N,3]
+    mean_rest: Tensor    # [N,D-3]
+    scale_raw: Tensor    # [N,D]
+    l_triangle: Tensor   # [N,D(D-1)/2]
+    opacity_raw: Tensor  # [N,1]
+    beta_raw: Tensor     # [N,D-2]
+    rgb: Tensor          # [N,3]
+    quats: Optional[Tensor] = None  # [N,4]: rotation from quaternions instead of l_triangle[:, :3]
+
+    @property
+    def D(self) -> int:
+        return self.scale_raw.shape[1]
+
+    def leaves(self) -> Dict[str, Tensor]:
+        d = dict(xyz=self.xyz, mean_rest=self.mean_rest, scale_raw=self.scale_raw,
+                 l_triangle=self.l_triangle, opacity_raw=self.opacity_raw, beta_raw=self.beta_raw,
+                 rgb=self.rgb)
+        if self.quats is not None:
+            d["quats"] = self.quats
+        return d
+
+    @staticmethod
+    def from_scene(sc, device, requires_grad: bool = True, use_quats: bool = True) -> "BetaParams":
+        def t(a):
+            return torch.tensor(a, dtype=torch.float32, device=device).requires_grad_(requires_grad)
+        return BetaParams(xyz=t(sc.xyz), mean_rest=t(sc.mean_rest), scale_raw=t(sc.scale_raw),
+                          l_triangle=t(sc.l_triangle), opacity_raw=t(sc.opacity_raw),
+                          beta_raw=t(sc.beta_raw), rgb=t(sc.rgb),
+                          quats=t(sc.quats) if use_quats else None)
+
+
+def rest_indices(D: int, device) -> tuple:
+    """rest_i, rest_j of scene/beta_model.py:69-73 as device int32 tensors."""
+    ti, tj = torch.tril_indices(D, D, offset=-1)
+    m = (ti >= 3) | (tj >= 3)
+    return ti[m].to(torch.int32).to(device), tj[m].to(torch.int32).to(device)
+
+
+def camera_center(viewmat: Tensor) -> Tensor:
+    """World position of a world->camera [4,4] row-major matrix."""
+    R, t = viewmat[:3, :3], viewmat[:3, 3]
+    return -(R.transpose(0, 1) @ t)
+
+
+def conditioned(params: BetaParams, viewmat: Tensor, timestamp: Optional[float] = None, rest=None):
+    """Activations + query + conditioning: (means [N,3], covars [N,3,3],
+    opacities [N], beta_xyz [N], scale [N,D])."""
+    D = params.D
+    scale = F.softplus(params.scale_raw)
+    opac = torch.sigmoid(params.opacity_raw)
+    beta = 4.0 * torch.exp(params.beta_raw)
+    if params.quats is not None:
+        rot = quat_to_rotmat(params.quats)
+    else:
+        rot = l_triangle_to_rotmat(params.l_triangle[:, :3])
+    ri, rj = rest if rest is not None else rest_indices(D, params.xyz.device)
+    cov = rot_scale_l_triangle_to_covar(rot, scale, params.l_triangle, ri, rj)
+    mean = torch.cat([params.xyz, params.mean_rest], dim=-1)
+    with torch.no_grad():
+        cam_pos = camera_center(viewmat)
+    view_dir = params.xyz - cam_pos[None]
+    view_dir = view_dir / view_dir.norm(dim=-1, keepdim=True)
+    if D == 6:
+        query = view_dir
+    elif D == 7:
+        ts = torch.full((view_dir.shape[0], 1), float(timestamp or 0.0), device=view_dir.device,
+                        dtype=view_dir.dtype)
+        query = torch.cat([view_dir, ts], dim=-1)
+    else:
+        raise NotImplementedError("Only implemented for 6D or 7D query")
+    m, V, o = cond_mean_convariance_opacity(mean, cov, opac, beta[:, 1:], query)
+    return m, V, o.squeeze(-1), beta[:, 0], scale
+
+
+def render(params: BetaParams, viewmat: Tensor, K: Tensor, width: int, height: int,
+           timestamp: Optional[float] = None, background: Optional[Tensor] = None, rest=None,
+           **raster_kwargs):
+    """One view through the drop-in path (BetaModel.render): returns
+    (render_colors [1,H,W,3], render_alphas [1,H,W,1], meta)."""
+    m, V, o, beta0, scale = conditioned(params, viewmat, timestamp, rest)
+    bg = background[None] if background is not None else None
+    return rasterization(
+        means=m, l_triagnles=params.l_triangle, scales=scale, opacities=o, betas=beta0,
+        colors=params.rgb, viewmats=viewmat[None], Ks=K[None], width=width, height=height,
+        backgrounds=bg, covars=V, **raster_kwargs)

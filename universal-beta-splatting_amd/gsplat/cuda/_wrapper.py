@@ -1,0 +1,459 @@
+"""Python API of the Beta-splat rasteriser ops, backed by libubs_hip.so.
+
+Same function names, arguments, shapes, autograd behaviour and error
+behaviour as the reference's submodules/gsplat/cuda/_wrapper.py (the entry
+points BetaModel and rasterization() call).  Each op allocates its outputs
+and workspaces through torch's caching allocator and hands raw pointers and
+the current HIP stream to the C ABI (include/ubs_capi.h).
+"""
+from __future__ import annotations
+
+import math
+from typing import Optional, Tuple
+
+import torch
+from torch import Tensor
+
+from ._backend import lib, ptr, require_cuda, stream_of
+
+
+def _call(name: str, *args) -> None:
+    L = lib()
+    L.check(getattr(L, name)(*args), name)
+
+
+# ----------------------------------------------------------------------------
+# Public API (reference: _wrapper.py:18-55, 185-491)
+# ----------------------------------------------------------------------------
+def cond_mean_convariance_opacity(
+    means: Tensor,  # [N, D]
+    covars: Tensor,  # [N, D, D]
+    opacities: Tensor,  # [N, 1]
+    betas: Tensor,  # [N, C]
+    query: Tensor,  # [N, C]
+) -> Tuple[Tensor, Tensor, Tensor]:
+    """Condition each D-dim primitive on its C = D-3 query dims.
+
+    Returns the conditioned mean [N,3], covariance [N,3,3] and opacity [N,1].
+    reference: _wrapper.py:18-31; the query gets no gradient (:611).
+    """
+    return _CondMeanConvarianceOpacity.apply(
+        means.contiguous(), covars.contiguous(), opacities.contiguous(), betas.contiguous(),
+        query.contiguous())
+
+
+def l_triangle_to_rotmat(l_triangle: Tensor) -> Tensor:
+    """First-order rotation R = I + A from the skew entries [N,3] (reference: _wrapper.py:34-36)."""
+    assert l_triangle.shape[1] == 3, l_triangle.shape
+    return _LTriangleToRotmat.apply(l_triangle.contiguous())
+
+
+def rot_scale_l_triangle_to_covar(
+    rot: Tensor,  # [N, 3, 3]
+    scale: Tensor,  # [N, D]
+    l_triangle: Tensor,  # [N, D*(D-1)//2]
+    rest_i: Tensor,
+    rest_j: Tensor,
+    spatial_block: bool = False,
+) -> Tensor:
+    """Sigma = L L^T (reference: _wrapper.py:39-55)."""
+    return _RotScaleLTriangleToCovar.apply(
+        rot.contiguous(), scale.contiguous(), l_triangle.contiguous(), rest_i, rest_j, spatial_block)
+
+
+def fully_fused_projection(
+    means: Tensor,  # [N, 3]
+    covars: Optional[Tensor],  # [N, 6] or None
+    quats: Optional[Tensor],  # [N, 4] or None
+    scales: Optional[Tensor],  # [N, 3] or None
+    viewmats: Tensor,  # [C, 4, 4]
+    Ks: Tensor,  # [C, 3, 3]
+    width: int,
+    height: int,
+    eps2d: float = 0.3,
+    near_plane: float = 0.01,
+    far_plane: float = 1e10,
+    radius_clip: float = 0.0,
+    sparse_grad: bool = False,
+    calc_compensations: bool = False,
+    ortho: bool = False,
+) -> Tuple[Tensor, Tensor, Tensor, Tensor, Tensor]:
+    """EWA projection to 2D: (radii [C,N] int32, means2d [C,N,2], depths [C,N],
+    conics [C,N,3], compensations [C,N] or None).  reference: _wrapper.py:185-282"""
+    C = viewmats.size(0)
+    N = means.size(0)
+    assert means.size() == (N, 3), means.size()
+    assert viewmats.size() == (C, 4, 4), viewmats.size()
+    assert Ks.size() == (C, 3, 3), Ks.size()
+    means = means.contiguous()
+    if covars is not None:
+        assert covars.size() == (N, 6), covars.size()
+        covars = covars.contiguous()
+    else:
+        assert quats is not None, "covars or quats is required"
+        assert scales is not None, "covars or scales is required"
+        assert quats.size() == (N, 4), quats.size()
+        assert scales.size() == (N, 3), scales.size()
+        quats = quats.contiguous()
+        scales = scales.contiguous()
+    viewmats = viewmats.contiguous()
+    Ks = Ks.contiguous()
+    return _FullyFusedProjection.apply(
+        means, covars, quats, scales, viewmats, Ks, width, height, eps2d, near_plane, far_plane,
+        radius_clip, calc_compensations, ortho)
+
+
+@torch.no_grad()
+def isect_tiles(
+    means2d: Tensor,  # [C, N, 2]
+    radii: Tensor,  # [C, N]
+    depths: Tensor,  # [C, N]
+    tile_size: int,
+    tile_width: int,
+    tile_height: int,
+    sort: bool = True,
+    n_cameras: Optional[int] = None,
+    camera_ids: Optional[Tensor] = None,
+    primitive_ids: Optional[Tensor] = None,
+) -> Tuple[Tensor, Tensor, Tensor]:
+    """Tiles per primitive [C,N] int32, isect ids int64 [n] (cam | tile |
+    depth bits) sorted stably on their low 32 + tile_bits + cam_bits bits, and
+    flatten ids int32 [n].  reference: _wrapper.py:285-343, isect_tiles.cu:99-285"""
+    C, N, _ = means2d.shape
+    assert means2d.shape == (C, N, 2), means2d.size()
+    assert radii.shape == (C, N), radii.size()
+    assert depths.shape == (C, N), depths.size()
+    means2d, radii, depths = means2d.contiguous(), radii.contiguous(), depths.contiguous()
+    require_cuda(means2d, radii, depths)
+    dev = means2d.device
+    L = lib()
+    st = stream_of(means2d)
+    tpg = torch.empty((C, N), dtype=torch.int32, device=dev)
+    cum = torch.empty((C * N,), dtype=torch.int64, device=dev)
+    n_dev = torch.zeros((1,), dtype=torch.int64, device=dev)
+    ws_bytes = int(L.ubs_isect_count_workspace_bytes(C, N))
+    ws = torch.empty((ws_bytes,), dtype=torch.uint8, device=dev)
+    L.check(L.ubs_isect_tiles_count(C, N, ptr(means2d), ptr(radii), tile_size, tile_width, tile_height,
+                                    ptr(tpg), ptr(cum), ptr(n_dev), ptr(ws), ws_bytes, st),
+            "isect_tiles(count)")
+    n_isects = int(n_dev.item())  # host sync, as isect_tiles.cu:181
+    isect_ids = torch.empty((n_isects,), dtype=torch.int64, device=dev)
+    flatten_ids = torch.empty((n_isects,), dtype=torch.int32, device=dev)
+    if n_isects:
+        L.check(L.ubs_isect_tiles_emit(C, N, ptr(means2d), ptr(radii), ptr(depths), ptr(cum), tile_size,
+                                       tile_width, tile_height, ptr(isect_ids), ptr(flatten_ids), st),
+                "isect_tiles(emit)")
+    if n_isects and sort:
+        end_bit = 32 + int(L.ubs_isect_prefix_bits(C, tile_width, tile_height))
+        isect_ids, flatten_ids = radix_sort_pairs(isect_ids, flatten_ids, 0, end_bit)
+    return tpg, isect_ids, flatten_ids
+
+
+def radix_sort_pairs(keys: Tensor, values: Tensor, begin_bit: int, end_bit: int,
+                     check_status: bool = False) -> Tuple[Tensor, Tensor]:
+    """Stable onesweep radix sort of (int64 key, int32 value) on key bits
+    [begin_bit, end_bit); returns new sorted tensors (double-buffered; the
+    inputs may be consumed as scratch)."""
+    require_cuda(keys, values)
+    assert keys.dtype == torch.int64 and values.dtype == torch.int32
+    n = keys.numel()
+    L = lib()
+    st = stream_of(keys)
+    keys_alt = torch.empty_like(keys)
+    vals_alt = torch.empty_like(values)
+    ws_bytes = int(L.ubs_radix_sort_workspace_bytes(n, begin_bit, end_bit))
+    ws = torch.empty((ws_bytes,), dtype=torch.uint8, device=keys.device)
+    import ctypes
+
+    sel = ctypes.c_int(0)
+    L.check(L.ubs_radix_sort_pairs(n, ptr(keys), ptr(keys_alt), ptr(values), ptr(vals_alt), begin_bit,
+                                   end_bit, ptr(ws), ws_bytes, ctypes.byref(sel), st), "radix_sort_pairs")
+    if check_status:
+        status = ctypes.c_int(0)
+        L.check(L.ubs_radix_sort_status(ptr(ws), ctypes.byref(status), st), "radix_sort_status")
+        if status.value != 0:
+            raise RuntimeError("radix sort look-back exceeded its spin limit; result invalid")
+    if sel.value:
+        return keys_alt, vals_alt
+    return keys, values
+
+
+@torch.no_grad()
+def isect_offset_encode(isect_ids: Tensor, n_cameras: int, tile_width: int, tile_height: int) -> Tensor:
+    """Per-(camera, tile) start index [C, th, tw] int32.  reference: _wrapper.py:346-363"""
+    isect_ids = isect_ids.contiguous()
+    require_cuda(isect_ids)
+    offsets = torch.empty((n_cameras, tile_height, tile_width), dtype=torch.int32, device=isect_ids.device)
+    _call("ubs_isect_offset_encode", isect_ids.numel(), ptr(isect_ids), n_cameras, tile_width, tile_height,
+          ptr(offsets), stream_of(isect_ids))
+    return offsets
+
+
+def rasterize_to_pixels(
+    means2d: Tensor,  # [C, N, 2]
+    conics: Tensor,  # [C, N, 3]
+    colors: Tensor,  # [C, N, channels]
+    opacities: Tensor,  # [C, N]
+    betas: Tensor,  # [C, N]
+    image_width: int,
+    image_height: int,
+    tile_size: int,
+    isect_offsets: Tensor,  # [C, tile_height, tile_width]
+    flatten_ids: Tensor,  # [n_isects]
+    backgrounds: Optional[Tensor] = None,  # [C, channels]
+    masks: Optional[Tensor] = None,  # [C, tile_height, tile_width]
+) -> Tuple[Tensor, Tensor]:
+    """Front-to-back Beta blending: (render_colors [C,H,W,channels],
+    render_alphas [C,H,W,1]).  reference: _wrapper.py:366-491"""
+    C = isect_offsets.size(0)
+    N = means2d.size(1)
+    assert means2d.shape == (C, N, 2), means2d.shape
+    assert conics.shape == (C, N, 3), conics.shape
+    assert colors.shape[:2] == (C, N), colors.shape
+    assert opacities.shape == (C, N), opacities.shape
+    assert betas.shape == (C, N), betas.shape
+    if backgrounds is not None:
+        assert backgrounds.shape == (C, colors.shape[-1]), backgrounds.shape
+        backgrounds = backgrounds.contiguous()
+    if masks is not None:
+        assert masks.shape == isect_offsets.shape, masks.shape
+        masks = masks.contiguous()
+    channels = colors.shape[-1]
+    if channels > 513 or channels == 0:
+        raise ValueError(f"Unsupported number of color channels: {channels}")
+    tile_height, tile_width = isect_offsets.shape[1:3]
+    assert tile_height * tile_size >= image_height, \
+        f"Assert Failed: {tile_height} * {tile_size} >= {image_height}"
+    assert tile_width * tile_size >= image_width, \
+        f"Assert Failed: {tile_width} * {tile_size} >= {image_width}"
+    render_colors, render_alphas = _RasterizeToPixels.apply(
+        means2d.contiguous(), conics.contiguous(), colors.contiguous(), opacities.contiguous(),
+        betas.contiguous(), backgrounds, masks, image_width, image_height, tile_size,
+        isect_offsets.contiguous(), flatten_ids.contiguous())
+    return render_colors, render_alphas
+
+
+def set_backward_fp64_transmittance(enable: bool) -> None:
+    """Recover transmittance in fp64 in the blend backward (default fp32)."""
+    lib().ubs_set_backward_fp64_transmittance(1 if enable else 0)
+
+
+# ----------------------------------------------------------------------------
+# Raw (non-autograd) launchers, used by the autograd classes and benches
+# ----------------------------------------------------------------------------
+def _mask_ptr(masks: Optional[Tensor]):
+    """bool/uint8 tile mask -> pointer to one byte per tile (nonzero = render)."""
+    if masks is None:
+        return None
+    if masks.dtype not in (torch.bool, torch.uint8):
+        raise TypeError("masks must be a bool or uint8 tensor")
+    return ptr(masks), masks
+
+
+def raster_fwd_raw(means2d, conics, colors, opacities, betas, backgrounds, masks, width, height, tile_size,
+                   isect_offsets, flatten_ids):
+    require_cuda(means2d, conics, colors, opacities, betas, backgrounds, isect_offsets, flatten_ids)
+    C, N = opacities.shape
+    channels = colors.shape[-1]
+    th, tw = isect_offsets.shape[1:3]
+    dev = means2d.device
+    render_colors = torch.empty((C, height, width, channels), dtype=torch.float32, device=dev)
+    render_alphas = torch.empty((C, height, width, 1), dtype=torch.float32, device=dev)
+    last_ids = torch.empty((C, height, width), dtype=torch.int32, device=dev)
+    T_final = torch.empty((C, height, width), dtype=torch.float32, device=dev)
+    mp = _mask_ptr(masks)
+    _call("ubs_rasterize_to_pixels_fwd", C, N, flatten_ids.numel(), channels, ptr(means2d), ptr(conics),
+          ptr(colors), ptr(opacities), ptr(betas), ptr(backgrounds), mp[0] if mp else None, width, height,
+          tile_size, tw, th, ptr(isect_offsets), ptr(flatten_ids), ptr(render_colors), ptr(render_alphas),
+          ptr(last_ids), ptr(T_final), stream_of(means2d))
+    return render_colors, render_alphas, last_ids, T_final
+
+
+def raster_bwd_raw(means2d, conics, colors, opacities, betas, backgrounds, masks, width, height, tile_size,
+                   isect_offsets, flatten_ids, render_alphas, last_ids, T_final, v_render_colors,
+                   v_render_alphas):
+    require_cuda(means2d, conics, colors, opacities, betas, backgrounds, isect_offsets, flatten_ids,
+                 render_alphas, last_ids, T_final, v_render_colors, v_render_alphas)
+    C, N = opacities.shape
+    channels = colors.shape[-1]
+    th, tw = isect_offsets.shape[1:3]
+    v_means2d = torch.empty_like(means2d)
+    v_conics = torch.empty_like(conics)
+    v_colors = torch.empty_like(colors)
+    v_opacities = torch.empty_like(opacities)
+    v_betas = torch.empty_like(betas)
+    mp = _mask_ptr(masks)
+    _call("ubs_rasterize_to_pixels_bwd", C, N, flatten_ids.numel(), channels, ptr(means2d), ptr(conics),
+          ptr(colors), ptr(opacities), ptr(betas), ptr(backgrounds), mp[0] if mp else None, width, height,
+          tile_size, tw, th, ptr(isect_offsets), ptr(flatten_ids), ptr(render_alphas), ptr(last_ids),
+          ptr(T_final), ptr(v_render_colors), ptr(v_render_alphas), ptr(v_means2d), ptr(v_conics),
+          ptr(v_colors), ptr(v_opacities), ptr(v_betas), stream_of(means2d))
+    return v_means2d, v_conics, v_colors, v_opacities, v_betas
+
+
+# ----------------------------------------------------------------------------
+# autograd Functions (reference: _wrapper.py:573-684, 807-1050)
+# ----------------------------------------------------------------------------
+class _CondMeanConvarianceOpacity(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, means, covars, opacities, betas, query):
+        require_cuda(means, covars, opacities, betas, query)
+        assert means.dim() == 2, "means must be [N,D]"
+        assert covars.dim() == 3, "covars must be [N,D,D]"
+        assert opacities.dim() == 2 and opacities.size(1) == 1, "opacities must be [N,1]"
+        assert betas.dim() == 2 and query.dim() == 2, "betas/query must be [N,C]"
+        N, D = means.shape
+        assert covars.shape == (N, D, D), "covars must match [N,D,D]"
+        assert query.size(0) == N and betas.size(0) == N, "batch sizes must match"
+        assert D >= 4, "D must be >= 4"
+        assert D - 3 == betas.size(1) == query.size(1), "C must be D-3"
+        om = torch.empty((N, 3), dtype=means.dtype, device=means.device)
+        oV = torch.empty((N, 3, 3), dtype=means.dtype, device=means.device)
+        oo = torch.empty((N, 1), dtype=means.dtype, device=means.device)
+        _call("ubs_cond_mean_convariance_opacity_fwd", N, D, ptr(means), ptr(covars), ptr(opacities),
+              ptr(betas), ptr(query), ptr(om), ptr(oV), ptr(oo), stream_of(means))
+        ctx.save_for_backward(means, covars, opacities, betas, query)
+        return om, oV, oo
+
+    @staticmethod
+    def backward(ctx, v_m, v_V, v_o):
+        means, covars, opacities, betas, query = ctx.saved_tensors
+        N, D = means.shape
+        v_m, v_V, v_o = v_m.contiguous(), v_V.contiguous(), v_o.contiguous()
+        gm = torch.empty_like(means)
+        gV = torch.empty_like(covars)
+        go = torch.empty_like(opacities)
+        gb = torch.empty_like(betas)
+        _call("ubs_cond_mean_convariance_opacity_bwd", N, D, ptr(means), ptr(covars), ptr(opacities),
+              ptr(betas), ptr(query), ptr(v_m), ptr(v_V), ptr(v_o), ptr(gm), ptr(gV), ptr(go), ptr(gb),
+              stream_of(means))
+        return gm, gV, go, gb, None
+
+
+class _LTriangleToRotmat(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, l_triangle):
+        require_cuda(l_triangle)
+        N = l_triangle.size(0)
+        R = torch.empty((N, 3, 3), dtype=l_triangle.dtype, device=l_triangle.device)
+        _call("ubs_l_triangle_to_rotmat_fwd", N, ptr(l_triangle), ptr(R), stream_of(l_triangle))
+        ctx.save_for_backward(l_triangle)
+        return R
+
+    @staticmethod
+    def backward(ctx, v_R):
+        (l_triangle,) = ctx.saved_tensors
+        v_R = v_R.contiguous()
+        out = torch.empty_like(l_triangle)
+        _call("ubs_l_triangle_to_rotmat_bwd", l_triangle.size(0), ptr(v_R), ptr(out), stream_of(v_R))
+        return out
+
+
+class _RotScaleLTriangleToCovar(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, rot, scale, l_triangle, rest_i, rest_j, spatial_block):
+        require_cuda(rot, scale, l_triangle, rest_i, rest_j)
+        N, D = scale.shape
+        rest_i = rest_i.to(torch.int32).contiguous()
+        rest_j = rest_j.to(torch.int32).contiguous()
+        dim = 3 if (spatial_block or D == 3) else D
+        covar = torch.empty((N, dim, dim), dtype=rot.dtype, device=rot.device)
+        _call("ubs_rot_scale_l_triangle_to_covar_fwd", N, D, ptr(rot), ptr(scale), ptr(l_triangle),
+              ptr(rest_i), ptr(rest_j), rest_i.numel(), int(spatial_block), ptr(covar), stream_of(rot))
+        ctx.save_for_backward(rot, scale, l_triangle, rest_i, rest_j)
+        ctx.spatial_block = spatial_block
+        return covar
+
+    @staticmethod
+    def backward(ctx, v_covar):
+        rot, scale, l_triangle, rest_i, rest_j = ctx.saved_tensors
+        if not any(ctx.needs_input_grad[:3]):
+            return (None, None, None, None, None, None)
+        N, D = scale.shape
+        v_covar = v_covar.contiguous()
+        v_rot = torch.empty_like(rot)
+        v_scale = torch.empty_like(scale)
+        v_l = torch.empty_like(l_triangle)
+        _call("ubs_rot_scale_l_triangle_to_covar_bwd", N, D, ptr(rot), ptr(scale), ptr(l_triangle),
+              ptr(rest_i), ptr(rest_j), rest_i.numel(), int(ctx.spatial_block), ptr(v_covar), ptr(v_rot),
+              ptr(v_scale), ptr(v_l), stream_of(rot))
+        return v_rot, v_scale, v_l, None, None, None
+
+
+class _FullyFusedProjection(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, means, covars, quats, scales, viewmats, Ks, width, height, eps2d, near_plane,
+                far_plane, radius_clip, calc_compensations, ortho):
+        require_cuda(means, covars, quats, scales, viewmats, Ks)
+        C, N = viewmats.size(0), means.size(0)
+        dev = means.device
+        radii = torch.empty((C, N), dtype=torch.int32, device=dev)
+        means2d = torch.empty((C, N, 2), dtype=means.dtype, device=dev)
+        depths = torch.empty((C, N), dtype=means.dtype, device=dev)
+        conics = torch.empty((C, N, 3), dtype=means.dtype, device=dev)
+        comp = torch.zeros((C, N), dtype=means.dtype, device=dev) if calc_compensations else None
+        _call("ubs_fully_fused_projection_fwd", C, N, ptr(means), ptr(covars), ptr(quats), ptr(scales),
+              ptr(viewmats), ptr(Ks), width, height, eps2d, near_plane, far_plane, radius_clip, int(ortho),
+              ptr(radii), ptr(means2d), ptr(depths), ptr(conics), ptr(comp), stream_of(means))
+        ctx.save_for_backward(means, covars, quats, scales, viewmats, Ks, radii, conics, comp)
+        ctx.width, ctx.height, ctx.eps2d, ctx.ortho = width, height, eps2d, ortho
+        return radii, means2d, depths, conics, comp
+
+    @staticmethod
+    def backward(ctx, v_radii, v_means2d, v_depths, v_conics, v_compensations):
+        means, covars, quats, scales, viewmats, Ks, radii, conics, comp = ctx.saved_tensors
+        C, N = viewmats.size(0), means.size(0)
+        v_means = torch.empty_like(means)
+        v_covars = torch.empty_like(covars) if covars is not None else None
+        v_quats = torch.empty_like(quats) if covars is None else None
+        v_scales = torch.empty_like(scales) if covars is None else None
+        want_vm = ctx.needs_input_grad[4]
+        v_viewmats = torch.empty_like(viewmats) if want_vm else None
+        v_means2d = v_means2d.contiguous()
+        v_depths = v_depths.contiguous()
+        v_conics = v_conics.contiguous()
+        if v_compensations is not None:
+            v_compensations = v_compensations.contiguous()
+        _call("ubs_fully_fused_projection_bwd", C, N, ptr(means), ptr(covars), ptr(quats), ptr(scales),
+              ptr(viewmats), ptr(Ks), ctx.width, ctx.height, ctx.eps2d, int(ctx.ortho), ptr(radii),
+              ptr(conics), ptr(comp), ptr(v_means2d), ptr(v_depths), ptr(v_conics),
+              ptr(v_compensations) if comp is not None else None, ptr(v_means), ptr(v_covars),
+              ptr(v_quats), ptr(v_scales), ptr(v_viewmats), stream_of(means))
+        if not ctx.needs_input_grad[0]:
+            v_means = None
+        if not ctx.needs_input_grad[1]:
+            v_covars = None
+        if not ctx.needs_input_grad[2]:
+            v_quats = None
+        if not ctx.needs_input_grad[3]:
+            v_scales = None
+        return (v_means, v_covars, v_quats, v_scales, v_viewmats,
+                None, None, None, None, None, None, None, None, None)
+
+
+class _RasterizeToPixels(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, means2d, conics, colors, opacities, betas, backgrounds, masks, width, height, tile_size,
+                isect_offsets, flatten_ids):
+        render_colors, render_alphas, last_ids, T_final = raster_fwd_raw(
+            means2d, conics, colors, opacities, betas, backgrounds, masks, width, height, tile_size,
+            isect_offsets, flatten_ids)
+        ctx.save_for_backward(means2d, conics, colors, opacities, betas, backgrounds, masks, isect_offsets,
+                              flatten_ids, render_alphas, last_ids, T_final)
+        ctx.width, ctx.height, ctx.tile_size = width, height, tile_size
+        return render_colors, render_alphas
+
+    @staticmethod
+    def backward(ctx, v_render_colors, v_render_alphas):
+        (means2d, conics, colors, opacities, betas, backgrounds, masks, isect_offsets, flatten_ids,
+         render_alphas, last_ids, T_final) = ctx.saved_tensors
+        v_means2d, v_conics, v_colors, v_opacities, v_betas = raster_bwd_raw(
+            means2d, conics, colors, opacities, betas, backgrounds, masks, ctx.width, ctx.height,
+            ctx.tile_size, isect_offsets, flatten_ids, render_alphas, last_ids, T_final,
+            v_render_colors.contiguous(), v_render_alphas.contiguous())
+        if ctx.needs_input_grad[5]:
+            v_backgrounds = (v_render_colors * (1.0 - render_alphas).float()).sum(dim=(1, 2))
+        else:
+            v_backgrounds = None
+        return (v_means2d, v_conics, v_colors, v_opacities, v_betas, v_backgrounds,
+                None, None, None, None, None, None)
