@@ -1,0 +1,224 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X Beta-splat rasteriser: fwd+bwd frames/s.
+
+One step = one training step of the BetaModel render path on one view:
+parameter activations, view-direction query, covariance construction,
+conditioning, projection, tile intersection + sort + ranges, blend forward,
+L1 loss against a target image, and the backward of all of it down to the
+leaf parameters.  Inputs are synthetic (seeded) and resident in HBM before the
+timed region.  Prints one JSON line (see DESIGN.md for the field meanings).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c3|c1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "universal-beta-splatting_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+METRIC = "frames/s fwd+bwd & batched-serve MPix/s at 1080p, 1M prims; p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+
+CONFIGS = {
+    "c2": dict(N=1_000_000, D=6, W=1920, H=1080,
+               workload="1M-primitive static 6-D Beta scene, one 1920x1080 view, FP32 fwd+bwd"),
+    "c3": dict(N=1_000_000, D=7, W=1352, H=1014,
+               workload="1M-primitive dynamic 7-D Beta scene, one 1352x1014 view, FP32 fwd+bwd"),
+    "c1": dict(N=10_000, D=6, W=256, H=256,
+               workload="10k-primitive static 6-D Beta scene, one 256x256 view, FP32 fwd+bwd"),
+}
+
+
+def algorithmic_bytes(name: str, c: dict, n_isect: int, C: int = 1) -> float:
+    """Minimal HBM bytes one launch of `name` must move (per-unit figures in DESIGN.md)."""
+    N, D, W, H = c["N"], c["D"], c["W"], c["H"]
+    tiles = math.ceil(W / 16) * math.ceil(H / 16)
+    pix = C * W * H
+    CD = D - 3
+    if name == "ubs_rasterize_to_pixels_fwd":
+        return 44.0 * n_isect + 24.0 * pix + 4.0 * C * tiles
+    if name == "ubs_rasterize_to_pixels_bwd":
+        return 44.0 * n_isect + 24.0 * pix + 40.0 * n_isect
+    if name == "ubs_radix_sort_pairs":
+        end_bit = 32 + int(tiles).bit_length() + int(C).bit_length()
+        passes = math.ceil(end_bit / 8)
+        return n_isect * (8.0 + 24.0 * passes)
+    if name == "ubs_isect_offset_encode":
+        return 8.0 * n_isect + 4.0 * C * tiles
+    if name == "ubs_isect_tiles_count":
+        return 28.0 * C * N
+    if name == "ubs_isect_tiles_emit":
+        return 24.0 * C * N + 12.0 * n_isect
+    if name == "ubs_fully_fused_projection_fwd":
+        return 36.0 * N + 28.0 * C * N
+    if name == "ubs_fully_fused_projection_bwd":
+        return 36.0 * N + 40.0 * C * N + 36.0 * N
+    if name == "ubs_cond_mean_convariance_opacity_fwd":
+        return N * (4.0 * (D + D * D + 1 + 2 * CD) + 52.0)
+    if name == "ubs_cond_mean_convariance_opacity_bwd":
+        return N * (4.0 * (D + D * D + 1 + 2 * CD) + 52.0 + 4.0 * (D + D * D + 1 + CD))
+    if name == "ubs_rot_scale_l_triangle_to_covar_fwd":
+        return N * 4.0 * (9 + D + D * (D - 1) // 2 + D * D)
+    if name == "ubs_rot_scale_l_triangle_to_covar_bwd":
+        return N * 4.0 * (2 * (9 + D + D * (D - 1) // 2) + D * D)
+    return float("nan")
+
+
+def cpu_baseline(cfg: dict, seed: int = 0):
+    """The oracle's full step (float32, OpenMP on the host cores) on the same
+    workload, one frame."""
+    import numpy as np
+
+    from gsplat import scenes
+    from oracle import ubs_oracle as O
+
+    O.build()
+    sc = scenes.make_scene(cfg["N"], cfg["D"], seed=seed)
+    vm, K, ts = scenes.make_cameras(1, cfg["W"], cfg["H"], seed=seed)
+    leaves = dict(xyz=sc.xyz, mean_rest=sc.mean_rest, scale_raw=sc.scale_raw, l_triangle=sc.l_triangle,
+                  opacity_raw=sc.opacity_raw, beta_raw=sc.beta_raw, rgb=sc.rgb, quats=sc.quats)
+    tgt = np.random.Generator(np.random.PCG64(seed + 5)).uniform(0, 1, (cfg["H"], cfg["W"], 3))
+    t0 = time.perf_counter()
+    O.beta_step(leaves, vm[0], K[0], cfg["W"], cfg["H"], tgt, timestamp=float(ts[0]), dtype=np.float32)
+    dt = time.perf_counter() - t0
+    return {"value": 1.0 / dt, "unit": "frames/s", "cores": O.num_threads(), "kind": "port",
+            "sample": f"one full fwd+bwd frame of the same workload through the oracle (float32, "
+                      f"{O.num_threads()} OpenMP threads): {dt:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--fp64-transmittance", action="store_true",
+                    help="recover transmittance in fp64 in the blend backward")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
+              "Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from gsplat import beta, scenes
+    from gsplat.cuda import _wrapper as W
+
+    cfg = CONFIGS[args.config]
+    dev = torch.device("cuda:0")
+    torch.manual_seed(0)
+    sc = scenes.make_scene(cfg["N"], cfg["D"], seed=0)
+    vms, Ks, ts = scenes.make_cameras(1, cfg["W"], cfg["H"], seed=0)
+    params = beta.BetaParams.from_scene(sc, dev)
+    vm = torch.tensor(vms[0], device=dev)
+    K = torch.tensor(Ks[0], device=dev)
+    t_view = float(ts[0])
+    tgt = torch.tensor(np.random.Generator(np.random.PCG64(5)).uniform(0, 1, (cfg["H"], cfg["W"], 3)),
+                       dtype=torch.float32, device=dev)
+    rest = beta.rest_indices(cfg["D"], dev)
+    leaves = list(params.leaves().values())
+    W.set_backward_fp64_transmittance(args.fp64_transmittance)
+
+    def step():
+        for p in leaves:
+            p.grad = None
+        img, alpha, meta = beta.render(params, vm, K, cfg["W"], cfg["H"], timestamp=t_view, rest=rest)
+        loss = (img[0] - tgt).abs().mean()
+        loss.backward()
+        return meta
+
+    def fwd_only():
+        with torch.no_grad():
+            return beta.render(params, vm, K, cfg["W"], cfg["H"], timestamp=t_view, rest=rest)
+
+    for _ in range(args.warmup):
+        meta = step()
+    torch.cuda.synchronize()
+    n_isect = int(meta["isect_ids"].numel())
+    n_vis = int((meta["radii"] > 0).sum().item())
+
+    # ---- timed region: K steps; per-launch HIP events on the launch stream
+    timer = W.LaunchTimer()
+    W.set_launch_timer(timer)
+    step_ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step_ev[k][0].record()
+        step()
+        step_ev[k][1].record()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    W.set_launch_timer(None)
+    launches = timer.durations_ms()
+    step_ms = sorted(s.elapsed_time(e) for s, e in step_ev)
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+
+    # ---- forward-only rate (not the headline value)
+    for _ in range(3):
+        fwd_only()
+    torch.cuda.synchronize()
+    f0 = time.perf_counter()
+    for _ in range(args.steps):
+        fwd_only()
+    torch.cuda.synchronize()
+    fwd_ms = (time.perf_counter() - f0) * 1e3 / args.steps
+
+    # ---- roofline of the dominant kernel launch
+    per_call = {k: (sum(v) / len(v), len(v) / args.steps) for k, v in launches.items()}
+    dominant = max(per_call, key=lambda k: per_call[k][0] * per_call[k][1])
+    avg_ms = per_call[dominant][0]
+    alg = algorithmic_bytes(dominant, cfg, n_isect)
+    achieved = alg / (avg_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": alg}
+    stages = {k.replace("ubs_", ""): {"avg_ms": round(v[0], 4), "per_step": v[1],
+                                      "GB_per_s": round(algorithmic_bytes(k, cfg, n_isect) / (v[0] * 1e-3) / 1e9, 1)}
+              for k, v in sorted(per_call.items(), key=lambda kv: -kv[1][0] * kv[1][1])}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(cfg)
+
+    line = {
+        "metric": METRIC,
+        "value": round(1e3 / ms_per_step, 3),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded scene and camera generator, gsplat/scenes.py)",
+        "config": {"workload": cfg["workload"], "primitives": cfg["N"], "dims": cfg["D"],
+                   "width": cfg["W"], "height": cfg["H"], "views": 1, "tile_size": 16,
+                   "n_isects": n_isect, "visible_primitives": n_vis},
+        "fwd_frames_per_s": round(1e3 / fwd_ms, 3),
+        "fwd_bwd_mpix_per_s": round(cfg["W"] * cfg["H"] * 1e3 / ms_per_step / 1e6, 2),
+        "latency_ms": {"p50": round(step_ms[len(step_ms) // 2], 3),
+                       "p99": round(step_ms[min(len(step_ms) - 1, int(math.ceil(0.99 * len(step_ms))) - 1)], 3)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "stages": stages,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -103,25 +103,31 @@ int ubs_cond_mean_convariance_opacity_bwd(int64_t N, int64_t D, const float *mea
  * EWA projection.
  * ------------------------------------------------------------------------ */
 
-/* means [N,3]; either covars [N,6] (upper triangle) or quats [N,4] + scales
+/* Layout of the `covars` argument of the projection entry points. */
+#define UBS_COVAR_TRIU6 0  /* [N,6]: c00 c01 c02 c11 c12 c22 (the reference layout) */
+#define UBS_COVAR_FULL33 1 /* [N,3,3] row-major; only the upper triangle is read, and the
+                              backward writes the upper-triangle-only gradient (what the
+                              gather at rendering.py:54-56 back-propagates) */
+
+/* means [N,3]; either covars (layout covar_layout) or quats [N,4] + scales
  * [N,3]; viewmats [C,4,4] world->camera; Ks [C,3,3].
  * Outputs radii int32 [C,N], means2d [C,N,2], depths [C,N], conics [C,N,3],
  * compensations [C,N] (may be NULL).  Culled entries get radii = 0 and leave
  * the other outputs untouched.
  * Replaces fully_fused_projection_fwd_tensor, fully_fused_projection_fwd.cu:179-256. */
 int ubs_fully_fused_projection_fwd(uint32_t C, uint32_t N, const float *means, const float *covars,
-                                   const float *quats, const float *scales, const float *viewmats,
+                                   int covar_layout, const float *quats, const float *scales, const float *viewmats,
                                    const float *Ks, int32_t width, int32_t height, float eps2d,
                                    float near_plane, float far_plane, float radius_clip, int ortho,
                                    int32_t *radii, float *means2d, float *depths, float *conics,
                                    float *compensations, void *stream);
 
 /* Backward.  v_compensations and compensations may be NULL.  Outputs (zeroed
- * here): v_means [N,3]; v_covars [N,6] when covars given, else v_quats [N,4]
- * and v_scales [N,3]; v_viewmats [C,4,4] or NULL.
+ * here): v_means [N,3]; v_covars (same layout as covars) when covars given,
+ * else v_quats [N,4] and v_scales [N,3]; v_viewmats [C,4,4] or NULL.
  * Replaces fully_fused_projection_bwd_tensor, fully_fused_projection_bwd.cu:242-348. */
 int ubs_fully_fused_projection_bwd(uint32_t C, uint32_t N, const float *means, const float *covars,
-                                   const float *quats, const float *scales, const float *viewmats,
+                                   int covar_layout, const float *quats, const float *scales, const float *viewmats,
                                    const float *Ks, int32_t width, int32_t height, float eps2d,
                                    int ortho, const int32_t *radii, const float *conics,
                                    const float *compensations, const float *v_means2d,
@@ -212,6 +218,20 @@ int ubs_rasterize_to_pixels_bwd(uint32_t C, uint32_t N, int64_t n_isects, uint32
                                 const float *v_render_colors, const float *v_render_alphas,
                                 float *v_means2d, float *v_conics, float *v_colors,
                                 float *v_opacities, float *v_betas, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Process-wide numerical switches (no counterpart in the reference).
+ * ------------------------------------------------------------------------ */
+
+/* Blend backward: recover transmittance in fp64 (1) or fp32 (0, default). */
+int ubs_set_backward_fp64_transmittance(int enable);
+
+/* Conditioning backward: 0 (default) reproduces the reference's gradient,
+ * whose Cholesky backward (cond_mean_convariance_opacity_bwd.cu:508-518)
+ * solves with L where L^T is needed, so the gradient of the conditioned
+ * opacity w.r.t. the query-query covariance block is not the true one;
+ * 1 computes the exact gradient. */
+int ubs_set_cond_backward_exact(int enable);
 
 #ifdef __cplusplus
 }

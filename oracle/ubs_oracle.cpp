@@ -365,7 +365,7 @@ void cond_fwd(int64_t N, int64_t D, const T *means, const T *covars, const T *op
 template <typename T>
 void cond_bwd(int64_t N, int64_t D, const T *means, const T *covars, const T *opac,
               const T *betas, const T *query, const T *vm, const T *vV, const T *vo, T *gm,
-              T *gV, T *go, T *gb) {
+              T *gV, T *go, T *gb, bool exact) {
     const int C = int(D - 3);
     for (int64_t i = 0; i < N; ++i) {
         CondState<T> st;
@@ -466,14 +466,30 @@ void cond_bwd(int64_t N, int64_t D, const T *means, const T *covars, const T *op
                 if (den == T(0)) den = T(1e-20f);
                 U[k][col] = sum / den;
             }
-        for (int col = 0; col < C; ++col)
-            for (int k = 0; k < C; ++k) {
-                T sum = U[col][k];
-                for (int j = 0; j < k; ++j) sum -= st.Lc[k][j] * Gm[j][col];
-                T den = st.Lc[k][k];
-                if (den == T(0)) den = T(1e-20f);
-                Gm[k][col] = sum / den;
-            }
+        if (exact) {
+            // G = U L^-1: each row w of G solves w L = u, i.e. L^T w^T = u^T
+            // (back substitution with the upper-triangular L^T)
+            for (int row = 0; row < C; ++row)
+                for (int k = C - 1; k >= 0; --k) {
+                    T sum = U[row][k];
+                    for (int j = k + 1; j < C; ++j) sum -= st.Lc[j][k] * Gm[row][j];
+                    T den = st.Lc[k][k];
+                    if (den == T(0)) den = T(1e-20f);
+                    Gm[row][k] = sum / den;
+                }
+        } else {
+            // the reference's solve (cond_mean_convariance_opacity_bwd.cu:508-518):
+            // a forward substitution with L, which yields L^-1 U^T instead of
+            // U L^-1
+            for (int col = 0; col < C; ++col)
+                for (int k = 0; k < C; ++k) {
+                    T sum = U[col][k];
+                    for (int j = 0; j < k; ++j) sum -= st.Lc[k][j] * Gm[j][col];
+                    T den = st.Lc[k][k];
+                    if (den == T(0)) den = T(1e-20f);
+                    Gm[k][col] = sum / den;
+                }
+        }
         for (int r = 0; r < C; ++r)
             for (int c = 0; c < C; ++c)
                 gVi[(3 + r) * D + 3 + c] += T(0.5f) * (Gm[r][c] + Gm[c][r]);
@@ -1035,15 +1051,16 @@ void ora_cond_fwd_f64(int64_t N, int64_t D, const double *m, const double *V, co
                       const double *b, const double *q, double *om, double *oV, double *oo) {
     cond_fwd<double>(N, D, m, V, o, b, q, om, oV, oo);
 }
+// exact = 0: the reference's Cholesky-backward solve; 1: the exact gradient.
 void ora_cond_bwd_f32(int64_t N, int64_t D, const float *m, const float *V, const float *o,
                       const float *b, const float *q, const float *vm, const float *vV,
-                      const float *vo, float *gm, float *gV, float *go, float *gb) {
-    cond_bwd<float>(N, D, m, V, o, b, q, vm, vV, vo, gm, gV, go, gb);
+                      const float *vo, float *gm, float *gV, float *go, float *gb, int exact) {
+    cond_bwd<float>(N, D, m, V, o, b, q, vm, vV, vo, gm, gV, go, gb, exact != 0);
 }
 void ora_cond_bwd_f64(int64_t N, int64_t D, const double *m, const double *V, const double *o,
                       const double *b, const double *q, const double *vm, const double *vV,
-                      const double *vo, double *gm, double *gV, double *go, double *gb) {
-    cond_bwd<double>(N, D, m, V, o, b, q, vm, vV, vo, gm, gV, go, gb);
+                      const double *vo, double *gm, double *gV, double *go, double *gb, int exact) {
+    cond_bwd<double>(N, D, m, V, o, b, q, vm, vV, vo, gm, gV, go, gb, exact != 0);
 }
 
 void ora_proj_fwd_f32(int64_t C, int64_t N, const float *means, const float *covars,

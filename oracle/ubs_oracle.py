@@ -131,7 +131,9 @@ def cond_fwd(means, covars, opacities, betas, query):
     return om, oV, oo
 
 
-def cond_bwd(means, covars, opacities, betas, query, v_m, v_V, v_o):
+def cond_bwd(means, covars, opacities, betas, query, v_m, v_V, v_o, exact=False):
+    """exact=False: the reference's gradient (its Cholesky backward solves with
+    L where L^T is needed); exact=True: the true gradient."""
     dt = means.dtype
     arrs = [_c(a, dt) for a in (means, covars, opacities, betas, query, v_m, v_V, v_o)]
     N, D = arrs[0].shape
@@ -141,7 +143,7 @@ def cond_bwd(means, covars, opacities, betas, query, v_m, v_V, v_o):
     go = np.empty((N, 1), dt)
     gb = np.empty((N, C), dt)
     _fn("ora_cond_bwd", dt)(ctypes.c_int64(N), ctypes.c_int64(D), *[_ptr(a) for a in arrs],
-                            _ptr(gm), _ptr(gV), _ptr(go), _ptr(gb))
+                            _ptr(gm), _ptr(gV), _ptr(go), _ptr(gb), ctypes.c_int(int(exact)))
     return gm, gV, go, gb
 
 
@@ -338,3 +340,105 @@ def rasterization(means, covars33, opacities, betas, colors, viewmats, Ks, width
                 isect_ids=ids, flatten_ids=flat, isect_offsets=offs, last_ids=last,
                 covars6=covars6, width=width, height=height, tile_size=tile_size, n_cameras=C)
     return out_c, out_a, meta
+
+
+# ----------------------------------------------------------------------------
+# Full Beta step (BetaModel.render + L1 loss + backward to the leaf
+# parameters), scene/beta_model.py:36-52,103-159,660-722 and
+# rendering.py:17-232, composed from the functions above.
+# ----------------------------------------------------------------------------
+def _softplus(x):
+    return np.logaddexp(0.0, x)
+
+
+def _sigmoid(x):
+    return 1.0 / (1.0 + np.exp(-x))
+
+
+def _quat_R(q):
+    qn = q / np.linalg.norm(q, axis=-1, keepdims=True)
+    w, x, y, z = qn[:, 0], qn[:, 1], qn[:, 2], qn[:, 3]
+    R = np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y),
+                  2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x),
+                  2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)], -1)
+    return R.reshape(-1, 3, 3), qn
+
+
+def _quat_R_vjp(q, qn, G):
+    """d loss / d q for R(q / |q|) given G = d loss / d R."""
+    w, x, y, z = qn[:, 0], qn[:, 1], qn[:, 2], qn[:, 3]
+    g = G.reshape(-1, 9)
+    zero = np.zeros_like(w)
+    dw = np.stack([zero, -2 * z, 2 * y, 2 * z, zero, -2 * x, -2 * y, 2 * x, zero], -1)
+    dx = np.stack([zero, 2 * y, 2 * z, 2 * y, -4 * x, -2 * w, 2 * z, 2 * w, -4 * x], -1)
+    dy = np.stack([-4 * y, 2 * x, 2 * w, 2 * x, zero, 2 * z, -2 * w, 2 * z, -4 * y], -1)
+    dz = np.stack([-4 * z, -2 * w, 2 * x, 2 * w, -4 * z, 2 * y, 2 * x, 2 * y, zero], -1)
+    gqn = np.stack([(g * dw).sum(-1), (g * dx).sum(-1), (g * dy).sum(-1), (g * dz).sum(-1)], -1)
+    nrm = np.linalg.norm(q, axis=-1, keepdims=True)
+    return (gqn - (gqn * qn).sum(-1, keepdims=True) * qn) / nrm
+
+
+def beta_step(leaves: dict, viewmat, K, width, height, target, timestamp=0.0, dtype=np.float64,
+              tile_size=16, structure=None, want_grads=True, exact_cond=False, fixed_query=None):
+    """One forward+backward of the Beta rasteriser path on the CPU.
+
+    leaves: xyz, mean_rest, scale_raw, l_triangle, opacity_raw, beta_raw, rgb,
+    quats (numpy).  Loss = mean |image - target|.  Returns (image [H,W,3],
+    grads dict, meta).  ``structure`` (from a float32 run's meta) makes an
+    FP64 evaluation blend in the float32 tile lists.
+    """
+    dt = dtype
+    L = {k: np.asarray(v, dtype=np.float64) for k, v in leaves.items()}
+    D = L["scale_raw"].shape[1]
+    scale = _softplus(L["scale_raw"])
+    opac = _sigmoid(L["opacity_raw"])
+    beta = 4.0 * np.exp(L["beta_raw"])
+    rot, qn = _quat_R(L["quats"])
+    ri, rj = [], []
+    for r in range(D):
+        for c in range(r):
+            if r >= 3 or c >= 3:
+                ri.append(r)
+                rj.append(c)
+    ri = np.asarray(ri, np.int32)
+    rj = np.asarray(rj, np.int32)
+    c_ = lambda a: np.ascontiguousarray(a, dtype=dt)
+    cov = covar_fwd(c_(rot), c_(scale), c_(L["l_triangle"]), ri, rj)
+    mean = np.concatenate([L["xyz"], L["mean_rest"]], 1)
+    vm = np.asarray(viewmat, np.float64)
+    cc = -vm[:3, :3].T @ vm[:3, 3]
+    vd = L["xyz"] - cc
+    vd = vd / np.linalg.norm(vd, axis=1, keepdims=True)
+    q = vd if D == 6 else np.concatenate([vd, np.full((vd.shape[0], 1), timestamp)], 1)
+    if fixed_query is not None:  # (the query carries no gradient; FD checks hold it fixed)
+        q = np.asarray(fixed_query, np.float64)
+    bcond = np.ascontiguousarray(beta[:, 1:])
+    m, V, o = cond_fwd(c_(mean), cov, c_(opac), c_(bcond), c_(q))
+    img, alpha, meta = rasterization(m, V, c_(o[:, 0]), c_(beta[:, 0]), c_(L["rgb"]), c_(vm[None]),
+                                     c_(np.asarray(K)[None]), width, height, tile_size=tile_size,
+                                     structure=structure)
+    meta["query"] = q
+    if not want_grads:
+        return img[0], None, meta
+    H, W_ = height, width
+    v_img = np.sign(img[0].astype(np.float64) - target) / (H * W_ * 3)
+    v_img = c_(v_img[None])
+    T_final = c_(1.0 - alpha[..., 0])
+    vm2, vcn, vcol, vop, vbe = raster_bwd(meta["means2d"], meta["conics"], meta["colors"], meta["opacities"],
+                                          meta["betas"], width, height, tile_size, meta["isect_offsets"],
+                                          meta["flatten_ids"], alpha, meta["last_ids"], v_img,
+                                          np.zeros_like(alpha), T_final=T_final)
+    v_means, v_cov6, _, _, _ = proj_bwd(m, meta["covars6"], None, None, c_(vm[None]), c_(np.asarray(K)[None]),
+                                        width, height, 0.3, meta["radii"], meta["conics"], None, vm2,
+                                        np.zeros_like(meta["depths"]), vcn)
+    v_V = np.zeros((m.shape[0], 3, 3), dt)
+    v_V[:, TRIU[0], TRIU[1]] = v_cov6
+    gm, gV, go, gb = cond_bwd(c_(mean), cov, c_(opac), c_(bcond), c_(q), v_means, v_V, c_(vop[0][:, None]),
+                              exact=exact_cond)
+    v_rot, v_scale, v_lt = covar_bwd(c_(rot), c_(scale), c_(L["l_triangle"]), ri, rj, False, gV)
+    grads = dict(
+        xyz=gm[:, :3], mean_rest=gm[:, 3:], scale_raw=v_scale * _sigmoid(L["scale_raw"]),
+        l_triangle=v_lt, opacity_raw=go * opac * (1.0 - opac),
+        beta_raw=np.concatenate([vbe[0][:, None], gb], 1) * beta, rgb=vcol[0],
+        quats=_quat_R_vjp(L["quats"], qn, v_rot.astype(np.float64)))
+    return img[0], grads, meta

@@ -1,0 +1,90 @@
+"""CPU checks of the C ABI and the host-side logic (no kernel launches):
+the library loads and exports every symbol include/ubs_capi.h declares,
+argument validation fails with an error code and message before any HIP
+call, and the Python layer raises the reference's errors."""
+import ctypes
+import os
+import re
+import subprocess
+import sys
+
+import pytest
+import torch
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "ubs_capi.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ubs_[a-z0-9_]+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def L():
+    from gsplat.cuda._backend import lib
+    return lib()
+
+
+def test_library_exports_every_declared_symbol(L):
+    names = declared_symbols()
+    assert len(names) >= 20, names
+    for n in names:
+        assert hasattr(L._dll, n), f"{n} not exported by {L.path}"
+    assert L.ubs_version() >= 100
+
+
+def test_every_declared_symbol_has_a_ctypes_signature():
+    from gsplat.cuda._backend import _SIGNATURES
+    missing = [n for n in declared_symbols() if n not in _SIGNATURES]
+    assert not missing, missing
+
+
+def test_argument_validation_without_gpu(L):
+    sel = ctypes.c_int(0)
+    rc = L.ubs_radix_sort_pairs(10, None, None, None, None, 10, 5, None, 0, ctypes.byref(sel), None)
+    assert rc == 1 and "bit range" in L.last_error()
+    rc = L.ubs_radix_sort_pairs(1 << 31, None, None, None, None, 0, 32, None, 0, ctypes.byref(sel), None)
+    assert rc == 1 and "out of range" in L.last_error()
+    rc = L.ubs_radix_sort_pairs(100, None, None, None, None, 0, 32, None, 0, ctypes.byref(sel), None)
+    assert rc == 3 and "workspace" in L.last_error()
+    assert L.ubs_radix_sort_workspace_bytes(100000, 0, 46) > 100000 // 4096 * 256 * 4 * 6
+    rc = L.ubs_rot_scale_l_triangle_to_covar_fwd(10, 13, None, None, None, None, None, 0, 0, None, None)
+    assert rc == 1 and "D must be in" in L.last_error()
+    rc = L.ubs_cond_mean_convariance_opacity_fwd(10, 3, *([None] * 8), None)
+    assert rc == 1
+    rc = L.ubs_rasterize_to_pixels_fwd(1, 10, 0, 3, *([None] * 7), 64, 64, 12, 4, 4, *([None] * 6), None)
+    assert rc == 1 and "tile_size" in L.last_error()
+    rc = L.ubs_rasterize_to_pixels_fwd(1, 10, 0, 3, *([None] * 7), 64, 64, 16, 2, 4, *([None] * 6), None)
+    assert rc == 1 and "cover" in L.last_error()
+    rc = L.ubs_fully_fused_projection_fwd(1, 10, None, None, 7, *([None] * 4), 64, 64, 0.3, 0.01, 1e10, 0.0, 0,
+                                          *([None] * 5), None)
+    assert rc == 1
+
+
+def test_prefix_bits_match_reference_formula(L):
+    import math
+    for C, tw, th in ((1, 120, 68), (1, 16, 16), (8, 80, 45), (3, 1, 1), (1, 64, 64)):
+        want = (math.floor(math.log2(tw * th)) + 1) + (math.floor(math.log2(C)) + 1)
+        assert L.ubs_isect_prefix_bits(C, tw, th) == want
+
+
+def test_python_api_errors_match_reference():
+    from gsplat.cuda import _wrapper as W
+    C, N = 1, 5
+    args = [torch.zeros(C, N, 2), torch.zeros(C, N, 3), torch.zeros(C, N, 514), torch.zeros(C, N), torch.zeros(C, N),
+            16, 16, 16, torch.zeros(C, 1, 1, dtype=torch.int32), torch.zeros(0, dtype=torch.int32)]
+    with pytest.raises(ValueError):
+        W.rasterize_to_pixels(*args)
+    with pytest.raises(AssertionError):
+        W.rasterize_to_pixels(torch.zeros(C, N, 3), *args[1:])
+    with pytest.raises(RuntimeError, match="HIP/CUDA device"):
+        W.rasterize_to_pixels(*args[:2], torch.zeros(C, N, 3), *args[3:])
+    with pytest.raises(AssertionError):
+        W.fully_fused_projection(torch.zeros(N, 3), None, None, None, torch.zeros(1, 4, 4), torch.zeros(1, 3, 3), 8, 8)
+
+
+def test_bench_refuses_multi_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True)
+    assert r.returncode == 2 and "one GPU" in r.stderr

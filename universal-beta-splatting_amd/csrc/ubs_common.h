@@ -107,6 +107,47 @@ __device__ __forceinline__ float wave_sum(float v) {
     return (r0 + r1) + (r2 + r3);
 }
 
+__device__ __forceinline__ float dpp_row_sum(float v) {
+    v += __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0xB1, 0xF, 0xF, true));
+    v += __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x4E, 0xF, 0xF, true));
+    v += __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x124, 0xF, 0xF, true));
+    v += __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x128, 0xF, 0xF, true));
+    return v;
+}
+
+// Pairwise reduce-scatter of two per-lane values across the wave halves
+// (v_permlane32_swap): lanes 0-31 get a[l] + a[l+32], lanes 32-63 get
+// b[l-32] + b[l].
+__device__ __forceinline__ float swap32_sum(float a, float b) {
+    const auto s = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(unsigned, a), __builtin_bit_cast(unsigned, b),
+                                                    false, false);
+    return __builtin_bit_cast(float, s[0]) + __builtin_bit_cast(float, s[1]);
+}
+
+// Same inside each 32-lane half (v_permlane16_swap): quarter 0 gets
+// a[l] + a[l+16], quarter 1 b[l-16] + b[l], quarters 2/3 likewise.
+__device__ __forceinline__ float swap16_sum(float a, float b) {
+    const auto s = __builtin_amdgcn_permlane16_swap(__builtin_bit_cast(unsigned, a), __builtin_bit_cast(unsigned, b),
+                                                    false, false);
+    return __builtin_bit_cast(float, s[0]) + __builtin_bit_cast(float, s[1]);
+}
+
+// Wave-wide sums of 4R per-lane values at ~2.5 VALU ops per value instead of
+// ~11: two reduce-scatter levels (permlane swaps) then DPP row sums.
+// Afterwards out[r] in every lane of quarter Q (lanes 16Q..16Q+15) is the
+// wave total of value index 4r + kQuarterValue[Q], kQuarterValue = {0,2,1,3}.
+// Must be called with all 64 lanes active.
+__device__ __forceinline__ int quarter_value(int q) { return (q == 0) ? 0 : (q == 1) ? 2 : (q == 2) ? 1 : 3; }
+
+template <int R>
+__device__ __forceinline__ void wave_reduce_scatter4(const float (&v)[4 * R], float (&out)[R]) {
+    float u[2 * R];
+#pragma unroll
+    for (int p = 0; p < 2 * R; ++p) u[p] = swap32_sum(v[2 * p], v[2 * p + 1]);
+#pragma unroll
+    for (int r = 0; r < R; ++r) out[r] = dpp_row_sum(swap16_sum(u[2 * r], u[2 * r + 1]));
+}
+
 __device__ __forceinline__ int wave_max_i32(int v) {
     // butterfly through ds_swizzle-free shuffles (used once per tile)
     for (int o = 32; o > 0; o >>= 1) {

@@ -218,7 +218,7 @@ __global__ __launch_bounds__(kThreads) void cond_bwd_kernel(int64_t N, const flo
                                 const float *__restrict__ query, const float *__restrict__ vm,
                                 const float *__restrict__ vV, const float *__restrict__ vo,
                                 float *__restrict__ gm, float *__restrict__ gV, float *__restrict__ go,
-                                float *__restrict__ gb) {
+                                float *__restrict__ gb, int exact) {
     constexpr int D = C + 3;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
@@ -355,17 +355,34 @@ __global__ __launch_bounds__(kThreads) void cond_bwd_kernel(int64_t N, const flo
             if (den == 0.f) den = 1e-20f;
             U[k][col] = sum / den;
         }
+    if (exact) {
+        // G = U L^-1 (back substitution with L^T, row by row)
 #pragma unroll
-    for (int col = 0; col < C; ++col)
+        for (int row = 0; row < C; ++row)
 #pragma unroll
-        for (int k = 0; k < C; ++k) {
-            float sum = U[col][k];
+            for (int k = C - 1; k >= 0; --k) {
+                float sum = U[row][k];
 #pragma unroll
-            for (int j = 0; j < k; ++j) sum -= st.Lc[k][j] * Gm[j][col];
-            float den = st.Lc[k][k];
-            if (den == 0.f) den = 1e-20f;
-            Gm[k][col] = sum / den;
-        }
+                for (int j = k + 1; j < C; ++j) sum -= st.Lc[j][k] * Gm[row][j];
+                float den = st.Lc[k][k];
+                if (den == 0.f) den = 1e-20f;
+                Gm[row][k] = sum / den;
+            }
+    } else {
+        // the reference's solve (cond_mean_convariance_opacity_bwd.cu:508-518),
+        // a forward substitution with L that yields L^-1 U^T
+#pragma unroll
+        for (int col = 0; col < C; ++col)
+#pragma unroll
+            for (int k = 0; k < C; ++k) {
+                float sum = U[col][k];
+#pragma unroll
+                for (int j = 0; j < k; ++j) sum -= st.Lc[k][j] * Gm[j][col];
+                float den = st.Lc[k][k];
+                if (den == 0.f) den = 1e-20f;
+                Gm[k][col] = sum / den;
+            }
+    }
 #pragma unroll
     for (int r = 0; r < C; ++r)
 #pragma unroll
@@ -412,9 +429,12 @@ static void launch_cond_fwd(int64_t N, const float *m, const float *V, const flo
 template <int C>
 static void launch_cond_bwd(int64_t N, const float *m, const float *V, const float *o, const float *b,
                             const float *q, const float *vm, const float *vV, const float *vo,
-                            float *gm, float *gV, float *go, float *gb, hipStream_t st) {
-    cond_bwd_kernel<C><<<grid_for(N), kThreads, 0, st>>>(N, m, V, o, b, q, vm, vV, vo, gm, gV, go, gb);
+                            float *gm, float *gV, float *go, float *gb, int exact, hipStream_t st) {
+    cond_bwd_kernel<C><<<grid_for(N), kThreads, 0, st>>>(N, m, V, o, b, q, vm, vV, vo, gm, gV, go, gb, exact);
 }
+
+// 0: the reference's gradient (default, drop-in parity); 1: the exact one.
+static int g_exact_cholesky_backward = 0;
 
 }  // namespace ubs
 
@@ -456,8 +476,17 @@ int ubs_cond_mean_convariance_opacity_bwd(int64_t N, int64_t D, const float *m, 
     if (N == 0) return UBS_OK;
     UBS_REQUIRE(m && V && o && b && q && vm && vV && vo && gm && gV && go && gb, "null pointer");
     UBS_DISPATCH_C((int)(D - 3), launch_cond_bwd, N, m, V, o, b, q, vm, vV, vo, gm, gV, go, gb,
-                   as_stream(stream));
+                   g_exact_cholesky_backward, as_stream(stream));
     UBS_LAUNCH_CHECK("cond_bwd_kernel");
+    return UBS_OK;
+}
+
+// Chooses the opacity-path Cholesky backward of the conditioning op:
+// 0 = the reference's formula (it solves with L where L^T is needed, so its
+// gradient w.r.t. the conditioning covariance block is not the true one),
+// 1 = the exact gradient.  Process-wide; not part of the reference interface.
+int ubs_set_cond_backward_exact(int enable) {
+    g_exact_cholesky_backward = enable ? 1 : 0;
     return UBS_OK;
 }
 

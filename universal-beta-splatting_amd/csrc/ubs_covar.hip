@@ -39,9 +39,28 @@ struct RestLds {
     int8_t c[kMaxRest];
 };
 
+// True (for the whole block) when the rest pattern is the canonical one of
+// scene/beta_model.py:69-73: rows 3..D-1 list columns 0..r-1 in order.
+template <int D>
+__device__ __forceinline__ bool rest_is_canonical(const RestLds &rl, int M) {
+    bool ok = (M == D * (D - 1) / 2 - 3);
+    if (ok && (int)threadIdx.x < M) {
+        int m = 0, r_want = 3, c_want = 0;
+        // position -> (row, col) of the canonical enumeration
+        for (int r = 3; r < D; ++r)
+            for (int c = 0; c < r; ++c, ++m)
+                if (m == (int)threadIdx.x) {
+                    r_want = r;
+                    c_want = c;
+                }
+        ok = rl.r[threadIdx.x] == r_want && rl.c[threadIdx.x] == c_want;
+    }
+    return __syncthreads_and(ok) != 0;
+}
+
 template <int D>
 __device__ __forceinline__ void load_L(const float *__restrict__ R, const float *__restrict__ s,
-                                       const float *__restrict__ x_rest, const RestLds &rl, int M,
+                                       const float *__restrict__ x_rest, const RestLds &rl, int M, bool canon,
                                        float (&L)[D][D], uint32_t (&present)[D]) {
 #pragma unroll
     for (int r = 0; r < D; ++r) {
@@ -60,6 +79,18 @@ __device__ __forceinline__ void load_L(const float *__restrict__ R, const float 
     for (int r = 3; r < D; ++r) {
         L[r][r] = s[r];
         present[r] |= 1u << r;
+    }
+    if (canon) {
+        // rows 3..D-1 hold all their strictly-lower entries, in order
+        int m = 0;
+#pragma unroll
+        for (int r = 3; r < D; ++r)
+#pragma unroll
+            for (int c = 0; c < r; ++c, ++m) {
+                L[r][c] = x_rest[m];
+                present[r] |= 1u << c;
+            }
+        return;
     }
     for (int m = 0; m < M; ++m) {
         const int r = rl.r[m], c = rl.c[m];
@@ -93,12 +124,13 @@ __global__ __launch_bounds__(kThreads) void covar_fwd_kernel(int64_t N, const fl
         rl.c[threadIdx.x] = ok ? (int8_t)rest_j[threadIdx.x] : (int8_t)-1;
     }
     __syncthreads();
+    const bool canon = rest_is_canonical<D>(rl, M);
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     constexpr int FULL_M = D * (D - 1) / 2;
     float L[D][D];
     uint32_t P[D];
-    load_L<D>(rot + i * 9, scale + i * D, ltri + i * FULL_M + 3, rl, M, L, P);
+    load_L<D>(rot + i * 9, scale + i * D, ltri + i * FULL_M + 3, rl, M, canon, L, P);
     if (spatial || D == 3) {
         float *o = out + i * 9;
 #pragma unroll
@@ -168,6 +200,7 @@ __global__ __launch_bounds__(kThreads) void covar_bwd_kernel(int64_t N, const fl
         rl.c[threadIdx.x] = ok ? (int8_t)rest_j[threadIdx.x] : (int8_t)-1;
     }
     __syncthreads();
+    const bool canon = rest_is_canonical<D>(rl, M);
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     constexpr int FULL_M = D * (D - 1) / 2;
@@ -177,7 +210,7 @@ __global__ __launch_bounds__(kThreads) void covar_bwd_kernel(int64_t N, const fl
     const float *s = scale + i * D;
     float L[D][D];
     uint32_t P[D];
-    load_L<D>(R, s, ltri + i * FULL_M + 3, rl, M, L, P);
+    load_L<D>(R, s, ltri + i * FULL_M + 3, rl, M, canon, L, P);
     const float *G = vcov + i * dim * dim;
     auto S = [&](int a, int b) { return G[a * dim + b] + G[b * dim + a]; };
     float *dR = vrot + i * 9, *dS = vscale + i * D, *dL = vltri + i * FULL_M;
@@ -211,7 +244,30 @@ __global__ __launch_bounds__(kThreads) void covar_bwd_kernel(int64_t N, const fl
     dL[0] = 0.f;
     dL[1] = 0.f;
     dL[2] = 0.f;
-    for (int m = 0; m < M; ++m) {
+    if (canon) {
+        // same sums as the generic loop below, with compile-time indices
+        int m = 0;
+#pragma unroll
+        for (int r = 3; r < D; ++r)
+#pragma unroll
+            for (int c = 0; c < r; ++c, ++m) {
+                float g = 0.f;
+                if (c < 3) {
+                    g += S(r, 0) * L[0][c];
+                    g += S(r, 1) * L[1][c];
+                    g += S(r, 2) * L[2][c];
+#pragma unroll
+                    for (int j = 3; j < D; ++j)
+                        if (j > c) g += S(r, j) * L[j][c];
+                } else {
+                    g += S(r, c) * L[c][c];
+#pragma unroll
+                    for (int j = c + 1; j < D; ++j) g += S(r, j) * L[j][c];
+                }
+                dL[3 + m] = g;
+            }
+    }
+    for (int m = 0; m < (canon ? 0 : M); ++m) {
         const int r = rl.r[m], c = rl.c[m];
         float g = 0.f;
         if (r >= 3 && r < D && c < r) {

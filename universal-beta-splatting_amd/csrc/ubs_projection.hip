@@ -37,10 +37,22 @@ __device__ __forceinline__ void quat_to_R(const float *q, float (&R)[3][3]) {
     R[0][2] = 2.f * (xz + wy); R[1][2] = 2.f * (yz - wx); R[2][2] = 1.f - 2.f * (x2 + y2);
 }
 
+// covars layout: UBS_COVAR_TRIU6 -> [N,6] (c00 c01 c02 c11 c12 c22);
+// UBS_COVAR_FULL33 -> [N,3,3] row-major of which only the upper triangle is
+// read (the gather rendering.py:54-56 does before projecting).
 __device__ __forceinline__ void load_cov(const float *__restrict__ covars, const float *__restrict__ quats,
-                                         const float *__restrict__ scales, uint32_t g, float (&S)[3][3]) {
+                                         const float *__restrict__ scales, uint32_t g, float (&S)[3][3],
+                                         int cov33) {
     if (covars) {
-        const float *c = covars + (size_t)g * 6;
+        float c[6];
+        if (cov33) {
+            const float *m = covars + (size_t)g * 9;
+            c[0] = m[0]; c[1] = m[1]; c[2] = m[2]; c[3] = m[4]; c[4] = m[5]; c[5] = m[8];
+        } else {
+            const float *m = covars + (size_t)g * 6;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) c[k] = m[k];
+        }
         S[0][0] = c[0]; S[0][1] = c[1]; S[0][2] = c[2];
         S[1][0] = c[1]; S[1][1] = c[3]; S[1][2] = c[4];
         S[2][0] = c[2]; S[2][1] = c[4]; S[2][2] = c[5];
@@ -63,7 +75,7 @@ __device__ __forceinline__ void load_cov(const float *__restrict__ covars, const
 __device__ __forceinline__ void geom(const float *__restrict__ means, const float *__restrict__ covars,
                                      const float *__restrict__ quats, const float *__restrict__ scales,
                                      const float *__restrict__ vm, const float *__restrict__ K, uint32_t g,
-                                     int32_t W, int32_t H, bool ortho, Geom &G) {
+                                     int32_t W, int32_t H, bool ortho, int cov33, Geom &G) {
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
 #pragma unroll
@@ -73,7 +85,7 @@ __device__ __forceinline__ void geom(const float *__restrict__ means, const floa
     const float p0 = means[(size_t)g * 3], p1 = means[(size_t)g * 3 + 1], p2 = means[(size_t)g * 3 + 2];
 #pragma unroll
     for (int r = 0; r < 3; ++r) G.pc[r] = ((G.R[r][0] * p0 + G.R[r][1] * p1) + G.R[r][2] * p2) + G.t[r];
-    load_cov(covars, quats, scales, g, G.Sw);
+    load_cov(covars, quats, scales, g, G.Sw, cov33);
     float M[3][3];
 #pragma unroll
     for (int r = 0; r < 3; ++r)
@@ -115,7 +127,7 @@ __device__ __forceinline__ void geom(const float *__restrict__ means, const floa
 
 // reference: fully_fused_projection_fwd.cu:20-177
 __global__ __launch_bounds__(kThreads) void proj_fwd_kernel(uint32_t C, uint32_t N, const float *__restrict__ means,
-                                const float *__restrict__ covars, const float *__restrict__ quats,
+                                const float *__restrict__ covars, int cov33, const float *__restrict__ quats,
                                 const float *__restrict__ scales, const float *__restrict__ viewmats,
                                 const float *__restrict__ Ks, int32_t W, int32_t H, float eps2d,
                                 float near_plane, float far_plane, float radius_clip, int ortho,
@@ -133,7 +145,7 @@ __global__ __launch_bounds__(kThreads) void proj_fwd_kernel(uint32_t C, uint32_t
         return;
     }
     Geom G;
-    geom(means, covars, quats, scales, vm, K, g, W, H, ortho != 0, G);
+    geom(means, covars, quats, scales, vm, K, g, W, H, ortho != 0, cov33, G);
     float m2x, m2y;
     if (ortho) {
         m2x = G.fx * G.pc[0] + G.cx;
@@ -178,7 +190,7 @@ __global__ __launch_bounds__(kThreads) void proj_fwd_kernel(uint32_t C, uint32_t
 // reference: fully_fused_projection_bwd.cu:50-239 with utils.cuh:38-62,
 // 98-136, 214-249, 294-372, 385-435, 451-502.
 __global__ __launch_bounds__(kThreads) void proj_bwd_kernel(uint32_t C, uint32_t N, const float *__restrict__ means,
-                                const float *__restrict__ covars, const float *__restrict__ quats,
+                                const float *__restrict__ covars, int cov33, const float *__restrict__ quats,
                                 const float *__restrict__ scales, const float *__restrict__ viewmats,
                                 const float *__restrict__ Ks, int32_t W, int32_t H, float eps2d, int ortho,
                                 const int32_t *__restrict__ radii, const float *__restrict__ conics,
@@ -226,7 +238,7 @@ __global__ __launch_bounds__(kThreads) void proj_bwd_kernel(uint32_t C, uint32_t
             vcov2[1][1] += vsq * (om * P[1][1] - eps2d * detc);
         }
         Geom G;
-        geom(means, covars, quats, scales, vm, K, g, W, H, ortho != 0, G);
+        geom(means, covars, quats, scales, vm, K, g, W, H, ortho != 0, cov33, G);
         float vSc[3][3], vpc[3] = {0.f, 0.f, 0.f};
         {
             float tmp[3][2];
@@ -320,11 +332,27 @@ __global__ __launch_bounds__(kThreads) void proj_bwd_kernel(uint32_t C, uint32_t
         if (covars) {
             const float o6[6] = {vS[0][0], vS[0][1] + vS[1][0], vS[0][2] + vS[2][0], vS[1][1],
                                  vS[1][2] + vS[2][1], vS[2][2]};
-            float *oc = v_covars + (size_t)g * 6;
+            if (cov33) {
+                // gradient of the upper-triangle gather: upper entries get o6, lower stay 0
+                constexpr int kSlot[6] = {0, 1, 2, 4, 5, 8};
+                float *oc = v_covars + (size_t)g * 9;
+                if (direct) {
+                    oc[3] = 0.f;
+                    oc[6] = 0.f;
+                    oc[7] = 0.f;
+                }
 #pragma unroll
-            for (int k = 0; k < 6; ++k) {
-                if (direct) oc[k] = o6[k];
-                else atomic_add_f32(oc + k, o6[k]);
+                for (int k = 0; k < 6; ++k) {
+                    if (direct) oc[kSlot[k]] = o6[k];
+                    else atomic_add_f32(oc + kSlot[k], o6[k]);
+                }
+            } else {
+                float *oc = v_covars + (size_t)g * 6;
+#pragma unroll
+                for (int k = 0; k < 6; ++k) {
+                    if (direct) oc[k] = o6[k];
+                    else atomic_add_f32(oc + k, o6[k]);
+                }
             }
         } else {
             float vq[4] = {0.f, 0.f, 0.f, 0.f}, vs3[3] = {0.f, 0.f, 0.f};
@@ -417,7 +445,7 @@ using namespace ubs;
 extern "C" {
 
 int ubs_fully_fused_projection_fwd(uint32_t C, uint32_t N, const float *means, const float *covars,
-                                   const float *quats, const float *scales, const float *viewmats,
+                                   int covar_layout, const float *quats, const float *scales, const float *viewmats,
                                    const float *Ks, int32_t W, int32_t H, float eps2d, float near_plane,
                                    float far_plane, float radius_clip, int ortho, int32_t *radii,
                                    float *means2d, float *depths, float *conics, float *comp, void *stream) {
@@ -426,15 +454,17 @@ int ubs_fully_fused_projection_fwd(uint32_t C, uint32_t N, const float *means, c
     UBS_REQUIRE(total < (1ull << 31), "C*N must be < 2^31");
     UBS_REQUIRE(means && viewmats && Ks && radii && means2d && depths && conics, "null pointer");
     UBS_REQUIRE(covars || (quats && scales), "covars or (quats, scales) required");
+    UBS_REQUIRE(covar_layout == UBS_COVAR_TRIU6 || covar_layout == UBS_COVAR_FULL33, "bad covar_layout %d",
+                covar_layout);
     proj_fwd_kernel<<<grid_for((int64_t)total), kThreads, 0, as_stream(stream)>>>(
-        C, N, means, covars, covars ? nullptr : quats, covars ? nullptr : scales, viewmats, Ks, W, H, eps2d,
+        C, N, means, covars, covar_layout == UBS_COVAR_FULL33, covars ? nullptr : quats, covars ? nullptr : scales, viewmats, Ks, W, H, eps2d,
         near_plane, far_plane, radius_clip, ortho, radii, means2d, depths, conics, comp);
     UBS_LAUNCH_CHECK("proj_fwd_kernel");
     return UBS_OK;
 }
 
 int ubs_fully_fused_projection_bwd(uint32_t C, uint32_t N, const float *means, const float *covars,
-                                   const float *quats, const float *scales, const float *viewmats,
+                                   int covar_layout, const float *quats, const float *scales, const float *viewmats,
                                    const float *Ks, int32_t W, int32_t H, float eps2d, int ortho,
                                    const int32_t *radii, const float *conics, const float *comp,
                                    const float *v_means2d, const float *v_depths, const float *v_conics,
@@ -453,15 +483,18 @@ int ubs_fully_fused_projection_bwd(uint32_t C, uint32_t N, const float *means, c
     UBS_REQUIRE(v_means && (covars ? v_covars != nullptr : (v_quats && v_scales)), "null gradient output");
     if (C != 1) {
         int e = zero_async(v_means, (size_t)N * 3 * sizeof(float), st);
-        if (!e && covars) e = zero_async(v_covars, (size_t)N * 6 * sizeof(float), st);
+        if (!e && covars)
+            e = zero_async(v_covars, (size_t)N * (covar_layout == UBS_COVAR_FULL33 ? 9 : 6) * sizeof(float), st);
         if (!e && !covars) e = zero_async(v_quats, (size_t)N * 4 * sizeof(float), st);
         if (!e && !covars) e = zero_async(v_scales, (size_t)N * 3 * sizeof(float), st);
         if (e) return e;
     }
     if (total == 0) return UBS_OK;
     UBS_REQUIRE(means && viewmats && Ks && radii && conics && v_means2d && v_depths && v_conics, "null pointer");
+    UBS_REQUIRE(covar_layout == UBS_COVAR_TRIU6 || covar_layout == UBS_COVAR_FULL33, "bad covar_layout %d",
+                covar_layout);
     proj_bwd_kernel<<<grid_for((int64_t)total), kThreads, 0, st>>>(
-        C, N, means, covars, covars ? nullptr : quats, covars ? nullptr : scales, viewmats, Ks, W, H, eps2d,
+        C, N, means, covars, covar_layout == UBS_COVAR_FULL33, covars ? nullptr : quats, covars ? nullptr : scales, viewmats, Ks, W, H, eps2d,
         ortho, radii, conics, comp, v_means2d, v_depths, v_conics, v_comp, v_means, covars ? v_covars : nullptr,
         covars ? nullptr : v_quats, covars ? nullptr : v_scales, v_viewmats);
     UBS_LAUNCH_CHECK("proj_bwd_kernel");

@@ -39,12 +39,13 @@ __global__ __launch_bounds__(TS *TS) void raster_bwd_kernel(
     constexpr int SB = TS / 8;
     constexpr int NW = BS / 64;
     constexpr int NV = 7 + CDIM;  // xy(2) conic(3) opacity beta colour(CDIM)
+    constexpr int NR = (NV + 3) / 4;
     __shared__ float4 s_geo[BS];   // x, y, a, b
     __shared__ float4 s_geo2[BS];  // c, opacity, beta, id (bits)
     __shared__ float s_col[CDIM][BS];
     __shared__ uint32_t s_mask[BS];
     __shared__ uint16_t s_list[NW][BS];
-    __shared__ float s_acc[NV][BS];
+    __shared__ float s_acc[NV][BS + 1];  // +1: the four quarter lanes hit distinct banks
     __shared__ uint8_t s_hit[BS];
     __shared__ int32_t s_wmax[NW];
 
@@ -89,7 +90,7 @@ __global__ __launch_bounds__(TS *TS) void raster_bwd_kernel(
         const int32_t wm = wave_max_i32(bin_final);
         if (lane == 0) s_wmax[wave] = wm;
     }
-    for (int k = tid; k < NV * BS; k += BS) (&s_acc[0][0])[k] = 0.f;
+    for (int k = tid; k < NV * (BS + 1); k += BS) (&s_acc[0][0])[k] = 0.f;
     if (tid < BS) s_hit[tid] = 0;
     __syncthreads();
     const int32_t wave_bin_max = s_wmax[wave];
@@ -123,17 +124,26 @@ __global__ __launch_bounds__(TS *TS) void raster_bwd_kernel(
             }
             n_mine += __popcll(bal);
         }
+        // software-pipelined: the next entry's LDS reads are in flight while
+        // the current one is processed
+        int t_nx = n_mine > 0 ? s_list[wave][0] : 0;
+        float4 g1_nx = s_geo[t_nx], g2_nx = s_geo2[t_nx];
         for (int k = 0; k < n_mine; ++k) {
-            const int t = s_list[wave][k];
-            const float4 g1 = s_geo[t];
-            const float4 g2 = s_geo2[t];
+            const int t = t_nx;
+            const float4 g1 = g1_nx;
+            const float4 g2 = g2_nx;
+            if (k + 1 < n_mine) {
+                t_nx = s_list[wave][k + 1];
+                g1_nx = s_geo[t_nx];
+                g2_nx = s_geo2[t_nx];
+            }
             const float dx = g1.x - px, dy = g1.y - py;
             const float sigma = __builtin_fmaf(2.f * g1.w * dx, dy, __builtin_fmaf(g1.z * dx, dx, (g2.x * dy) * dy));
             const bool valid = inside && (batch_end - t) <= bin_final && sigma >= 0.f && sigma < 1.f;
             if (__ballot(valid) == 0ull) continue;
-            float v[NV];
+            float v[4 * NR];
 #pragma unroll
-            for (int q = 0; q < NV; ++q) v[q] = 0.f;
+            for (int q = 0; q < 4 * NR; ++q) v[q] = 0.f;
             if (valid) {
                 const float opac = g2.y, beta = g2.z;
                 const float oms = 1.f - sigma;
@@ -172,13 +182,18 @@ __global__ __launch_bounds__(TS *TS) void raster_bwd_kernel(
 #pragma unroll
                 for (int q = 0; q < CDIM; ++q) buf[q] = __builtin_fmaf(s_col[q][t], fac, buf[q]);
             }
-            float tot[NV];
+            // wave totals, scattered over the four 16-lane quarters; one lane
+            // per quarter adds its NR totals into the primitive's accumulator
+            float tot[NR];
+            wave_reduce_scatter4<NR>(v, tot);
+            if ((lane & 15) == 0) {
+                const int qv = quarter_value(lane >> 4);
 #pragma unroll
-            for (int q = 0; q < NV; ++q) tot[q] = wave_sum(v[q]);
-            if (lane == 0) {
-#pragma unroll
-                for (int q = 0; q < NV; ++q) atomicAdd(&s_acc[q][t], tot[q]);
-                s_hit[t] = 1;
+                for (int r = 0; r < NR; ++r) {
+                    const int idx = 4 * r + qv;
+                    if (idx < NV) atomicAdd(&s_acc[idx][t], tot[r]);
+                }
+                if (lane == 0) s_hit[t] = 1;
             }
         }
         __syncthreads();

@@ -109,11 +109,20 @@ __global__ __launch_bounds__(TS *TS) void raster_fwd_kernel(
         }
         // (s_list[wave] is written and read by this wave only: LDS is in order)
         // ---- front-to-back blend over the compacted list
+        // software-pipelined: the next entry's LDS reads are issued before the
+        // current entry is blended
+        int t_nx = n_mine > 0 ? s_list[wave][0] : 0;
+        float4 g1_nx = s_geo[t_nx], g2_nx = s_geo2[t_nx];
         for (int k = 0; k < n_mine; ++k) {
             if (__ballot(!done) == 0ull) break;
-            const int t = s_list[wave][k];
-            const float4 g1 = s_geo[t];
-            const float4 g2 = s_geo2[t];
+            const int t = t_nx;
+            const float4 g1 = g1_nx;
+            const float4 g2 = g2_nx;
+            if (k + 1 < n_mine) {
+                t_nx = s_list[wave][k + 1];
+                g1_nx = s_geo[t_nx];
+                g2_nx = s_geo2[t_nx];
+            }
             const float dx = g1.x - px, dy = g1.y - py;
             const float sigma = __builtin_fmaf(g1.w * dx, dy, __builtin_fmaf(g1.z * dx, dx, (g2.x * dy) * dy));
             if (!done && sigma >= 0.f && sigma < 1.f) {

@@ -40,10 +40,10 @@ _SIGNATURES = {
     "ubs_cond_mean_convariance_opacity_bwd": (
         _c_int, [_c_i64, _c_i64] + [_vp] * 12 + [_vp]),
     "ubs_fully_fused_projection_fwd": (
-        _c_int, [_c_u32, _c_u32, _vp, _vp, _vp, _vp, _vp, _vp, _c_i32, _c_i32, _c_f32, _c_f32, _c_f32,
+        _c_int, [_c_u32, _c_u32, _vp, _vp, _c_int, _vp, _vp, _vp, _vp, _c_i32, _c_i32, _c_f32, _c_f32, _c_f32,
                  _c_f32, _c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
     "ubs_fully_fused_projection_bwd": (
-        _c_int, [_c_u32, _c_u32, _vp, _vp, _vp, _vp, _vp, _vp, _c_i32, _c_i32, _c_f32, _c_int, _vp, _vp,
+        _c_int, [_c_u32, _c_u32, _vp, _vp, _c_int, _vp, _vp, _vp, _vp, _c_i32, _c_i32, _c_f32, _c_int, _vp, _vp,
                  _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "ubs_isect_count_workspace_bytes": (_c_size, [_c_u32, _c_u32]),
     "ubs_isect_tiles_count": (
@@ -63,6 +63,7 @@ _SIGNATURES = {
         _c_int, [_c_u32, _c_u32, _c_i64, _c_u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_u32, _c_u32, _c_u32,
                  _c_u32, _c_u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "ubs_set_backward_fp64_transmittance": (_c_int, [_c_int]),
+    "ubs_set_cond_backward_exact": (_c_int, [_c_int]),
 }
 
 

@@ -17,9 +17,47 @@ from torch import Tensor
 from ._backend import lib, ptr, require_cuda, stream_of
 
 
+class LaunchTimer:
+    """Records a pair of HIP events (torch.cuda.Event, current stream -- the
+    stream every op launches on) around each C-ABI call whose name is in
+    ``names`` (all when None).  Install with ``set_launch_timer``."""
+
+    def __init__(self, names=None):
+        self.names = None if names is None else set(names)
+        self.events = []  # (name, start, end)
+
+    def wants(self, name: str) -> bool:
+        return self.names is None or name in self.names
+
+    def durations_ms(self):
+        torch.cuda.synchronize()
+        out = {}
+        for name, s, e in self.events:
+            out.setdefault(name, []).append(s.elapsed_time(e))
+        return out
+
+
+_timer: Optional[LaunchTimer] = None
+
+
+def set_launch_timer(timer: Optional[LaunchTimer]) -> None:
+    global _timer
+    _timer = timer
+
+
 def _call(name: str, *args) -> None:
     L = lib()
-    L.check(getattr(L, name)(*args), name)
+    t = _timer
+    if t is not None and t.wants(name):
+        s = torch.cuda.Event(enable_timing=True)
+        e = torch.cuda.Event(enable_timing=True)
+        s.record()
+        rc = getattr(L, name)(*args)
+        e.record()
+        t.events.append((name, s, e))
+    else:
+        rc = getattr(L, name)(*args)
+    L.check(rc, name)
 
 
 # ----------------------------------------------------------------------------
@@ -100,7 +138,33 @@ def fully_fused_projection(
     Ks = Ks.contiguous()
     return _FullyFusedProjection.apply(
         means, covars, quats, scales, viewmats, Ks, width, height, eps2d, near_plane, far_plane,
-        radius_clip, calc_compensations, ortho)
+        radius_clip, calc_compensations, ortho, 0)
+
+
+def fully_fused_projection_covars33(
+    means: Tensor,  # [N, 3]
+    covars: Tensor,  # [N, 3, 3], upper triangle read
+    viewmats: Tensor,
+    Ks: Tensor,
+    width: int,
+    height: int,
+    eps2d: float = 0.3,
+    near_plane: float = 0.01,
+    far_plane: float = 1e10,
+    radius_clip: float = 0.0,
+    calc_compensations: bool = False,
+) -> Tuple[Tensor, Tensor, Tensor, Tensor, Tensor]:
+    """fully_fused_projection on the upper triangle of [N,3,3] covariances.
+
+    Same results as gathering covars[..., [0,0,0,1,1,2], [0,1,2,1,2,2]] and
+    projecting (rendering.py:54-56,81-97), with the gather fused into the
+    kernel; the gradient w.r.t. covars is upper-triangle-only, as the gather's.
+    """
+    N = means.size(0)
+    assert covars.shape == (N, 3, 3), covars.shape
+    return _FullyFusedProjection.apply(
+        means.contiguous(), covars.contiguous(), None, None, viewmats.contiguous(), Ks.contiguous(), width,
+        height, eps2d, near_plane, far_plane, radius_clip, calc_compensations, False, 1)
 
 
 @torch.no_grad()
@@ -133,16 +197,14 @@ def isect_tiles(
     n_dev = torch.zeros((1,), dtype=torch.int64, device=dev)
     ws_bytes = int(L.ubs_isect_count_workspace_bytes(C, N))
     ws = torch.empty((ws_bytes,), dtype=torch.uint8, device=dev)
-    L.check(L.ubs_isect_tiles_count(C, N, ptr(means2d), ptr(radii), tile_size, tile_width, tile_height,
-                                    ptr(tpg), ptr(cum), ptr(n_dev), ptr(ws), ws_bytes, st),
-            "isect_tiles(count)")
+    _call("ubs_isect_tiles_count", C, N, ptr(means2d), ptr(radii), tile_size, tile_width, tile_height,
+          ptr(tpg), ptr(cum), ptr(n_dev), ptr(ws), ws_bytes, st)
     n_isects = int(n_dev.item())  # host sync, as isect_tiles.cu:181
     isect_ids = torch.empty((n_isects,), dtype=torch.int64, device=dev)
     flatten_ids = torch.empty((n_isects,), dtype=torch.int32, device=dev)
     if n_isects:
-        L.check(L.ubs_isect_tiles_emit(C, N, ptr(means2d), ptr(radii), ptr(depths), ptr(cum), tile_size,
-                                       tile_width, tile_height, ptr(isect_ids), ptr(flatten_ids), st),
-                "isect_tiles(emit)")
+        _call("ubs_isect_tiles_emit", C, N, ptr(means2d), ptr(radii), ptr(depths), ptr(cum), tile_size,
+              tile_width, tile_height, ptr(isect_ids), ptr(flatten_ids), st)
     if n_isects and sort:
         end_bit = 32 + int(L.ubs_isect_prefix_bits(C, tile_width, tile_height))
         isect_ids, flatten_ids = radix_sort_pairs(isect_ids, flatten_ids, 0, end_bit)
@@ -166,8 +228,8 @@ def radix_sort_pairs(keys: Tensor, values: Tensor, begin_bit: int, end_bit: int,
     import ctypes
 
     sel = ctypes.c_int(0)
-    L.check(L.ubs_radix_sort_pairs(n, ptr(keys), ptr(keys_alt), ptr(values), ptr(vals_alt), begin_bit,
-                                   end_bit, ptr(ws), ws_bytes, ctypes.byref(sel), st), "radix_sort_pairs")
+    _call("ubs_radix_sort_pairs", n, ptr(keys), ptr(keys_alt), ptr(values), ptr(vals_alt), begin_bit, end_bit,
+          ptr(ws), ws_bytes, ctypes.byref(sel), st)
     if check_status:
         status = ctypes.c_int(0)
         L.check(L.ubs_radix_sort_status(ptr(ws), ctypes.byref(status), st), "radix_sort_status")
@@ -236,6 +298,14 @@ def rasterize_to_pixels(
 def set_backward_fp64_transmittance(enable: bool) -> None:
     """Recover transmittance in fp64 in the blend backward (default fp32)."""
     lib().ubs_set_backward_fp64_transmittance(1 if enable else 0)
+
+
+def set_cond_backward_exact(enable: bool) -> None:
+    """Conditioning backward: False (default) reproduces the reference's
+    gradient, whose Cholesky backward solves with L where L^T is needed
+    (cond_mean_convariance_opacity_bwd.cu:508-518); True gives the exact
+    gradient of the conditioned opacity w.r.t. the query covariance block."""
+    lib().ubs_set_cond_backward_exact(1 if enable else 0)
 
 
 # ----------------------------------------------------------------------------
@@ -383,7 +453,7 @@ class _RotScaleLTriangleToCovar(torch.autograd.Function):
 class _FullyFusedProjection(torch.autograd.Function):
     @staticmethod
     def forward(ctx, means, covars, quats, scales, viewmats, Ks, width, height, eps2d, near_plane,
-                far_plane, radius_clip, calc_compensations, ortho):
+                far_plane, radius_clip, calc_compensations, ortho, covar_layout):
         require_cuda(means, covars, quats, scales, viewmats, Ks)
         C, N = viewmats.size(0), means.size(0)
         dev = means.device
@@ -392,11 +462,12 @@ class _FullyFusedProjection(torch.autograd.Function):
         depths = torch.empty((C, N), dtype=means.dtype, device=dev)
         conics = torch.empty((C, N, 3), dtype=means.dtype, device=dev)
         comp = torch.zeros((C, N), dtype=means.dtype, device=dev) if calc_compensations else None
-        _call("ubs_fully_fused_projection_fwd", C, N, ptr(means), ptr(covars), ptr(quats), ptr(scales),
-              ptr(viewmats), ptr(Ks), width, height, eps2d, near_plane, far_plane, radius_clip, int(ortho),
-              ptr(radii), ptr(means2d), ptr(depths), ptr(conics), ptr(comp), stream_of(means))
+        _call("ubs_fully_fused_projection_fwd", C, N, ptr(means), ptr(covars), covar_layout, ptr(quats),
+              ptr(scales), ptr(viewmats), ptr(Ks), width, height, eps2d, near_plane, far_plane, radius_clip,
+              int(ortho), ptr(radii), ptr(means2d), ptr(depths), ptr(conics), ptr(comp), stream_of(means))
         ctx.save_for_backward(means, covars, quats, scales, viewmats, Ks, radii, conics, comp)
         ctx.width, ctx.height, ctx.eps2d, ctx.ortho = width, height, eps2d, ortho
+        ctx.covar_layout = covar_layout
         return radii, means2d, depths, conics, comp
 
     @staticmethod
@@ -414,8 +485,8 @@ class _FullyFusedProjection(torch.autograd.Function):
         v_conics = v_conics.contiguous()
         if v_compensations is not None:
             v_compensations = v_compensations.contiguous()
-        _call("ubs_fully_fused_projection_bwd", C, N, ptr(means), ptr(covars), ptr(quats), ptr(scales),
-              ptr(viewmats), ptr(Ks), ctx.width, ctx.height, ctx.eps2d, int(ctx.ortho), ptr(radii),
+        _call("ubs_fully_fused_projection_bwd", C, N, ptr(means), ptr(covars), ctx.covar_layout, ptr(quats),
+              ptr(scales), ptr(viewmats), ptr(Ks), ctx.width, ctx.height, ctx.eps2d, int(ctx.ortho), ptr(radii),
               ptr(conics), ptr(comp), ptr(v_means2d), ptr(v_depths), ptr(v_conics),
               ptr(v_compensations) if comp is not None else None, ptr(v_means), ptr(v_covars),
               ptr(v_quats), ptr(v_scales), ptr(v_viewmats), stream_of(means))
@@ -428,7 +499,7 @@ class _FullyFusedProjection(torch.autograd.Function):
         if not ctx.needs_input_grad[3]:
             v_scales = None
         return (v_means, v_covars, v_quats, v_scales, v_viewmats,
-                None, None, None, None, None, None, None, None, None)
+                None, None, None, None, None, None, None, None, None, None)
 
 
 class _RasterizeToPixels(torch.autograd.Function):

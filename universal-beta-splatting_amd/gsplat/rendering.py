@@ -14,13 +14,12 @@ from typing_extensions import Literal
 
 from .cuda._wrapper import (
     fully_fused_projection,
+    fully_fused_projection_covars33,
     isect_offset_encode,
     isect_tiles,
     rasterize_to_pixels,
 )
 from .utils import depth_to_normal
-
-_TRIU = ([0, 0, 0, 1, 1, 2], [0, 1, 2, 1, 2, 2])
 
 
 def rasterization(
@@ -62,10 +61,10 @@ def rasterization(
         assert l_triagnles.shape == (N, 4), l_triagnles.shape
         assert scales.shape == (N, 3), scales.shape
     else:
+        # only the upper triangle of the (possibly non-symmetric) 3x3 is used;
+        # the projection kernel reads it in place (no [N,6] gather)
         assert covars.shape == (N, 3, 3), covars.shape
         l_triagnles, scales = None, None
-        # upper triangle of the (possibly non-symmetric) 3x3 -> [N, 6]
-        covars = covars[..., _TRIU[0], _TRIU[1]]
     assert opacities.shape == (N,), opacities.shape
     assert betas.shape == (N,), betas.shape
     assert viewmats.shape == (C, 4, 4), viewmats.shape
@@ -77,10 +76,16 @@ def rasterization(
         colors.dim() == 3 and colors.shape[:2] == (C, N)
     ), colors.shape
 
-    proj_results = fully_fused_projection(
-        means, covars, l_triagnles, scales, viewmats, Ks, width, height, eps2d=eps2d,
-        near_plane=near_plane, far_plane=far_plane, radius_clip=radius_clip, sparse_grad=False,
-        calc_compensations=(rasterize_mode == "antialiased"), ortho=False)
+    if covars is not None:
+        proj_results = fully_fused_projection_covars33(
+            means, covars, viewmats, Ks, width, height, eps2d=eps2d, near_plane=near_plane,
+            far_plane=far_plane, radius_clip=radius_clip,
+            calc_compensations=(rasterize_mode == "antialiased"))
+    else:
+        proj_results = fully_fused_projection(
+            means, covars, l_triagnles, scales, viewmats, Ks, width, height, eps2d=eps2d,
+            near_plane=near_plane, far_plane=far_plane, radius_clip=radius_clip, sparse_grad=False,
+            calc_compensations=(rasterize_mode == "antialiased"), ortho=False)
     radii, means2d, depths, conics, compensations = proj_results
     opacities = opacities.repeat(C, 1)  # [C, N]
     betas = betas.repeat(C, 1)  # [C, N]
