@@ -83,6 +83,30 @@ def test_cond_fwd_exact_bwd_close(device, D):
         assert rel_l2(host(t.grad), w) < 2e-4, (n, rel_l2(host(t.grad), w))
 
 
+@pytest.mark.parametrize("exact", [False, True])
+def test_cond_bwd_modes(device, exact):
+    """Both conditioning-backward modes against the oracle's same mode; the
+    two modes differ only in the covariance gradient."""
+    c = conditioned_scene(3000, 6, seed=8)
+    a = c["act"]
+    ins = [a["mean"], c["cov"], a["opac"], np.ascontiguousarray(a["beta"][:, 1:]), c["query"]]
+    g = np.random.Generator(np.random.PCG64(12))
+    vm, vV, vo = (g.normal(0, 1, s).astype(np.float32) for s in ((3000, 3), (3000, 3, 3), (3000, 1)))
+    W.set_cond_backward_exact(exact)
+    try:
+        tins = [cu(x).requires_grad_(i < 4) for i, x in enumerate(ins)]
+        outs = W.cond_mean_convariance_opacity(*tins)
+        torch.autograd.backward(list(outs), [cu(vm), cu(vV), cu(vo)])
+    finally:
+        W.set_cond_backward_exact(False)
+    d = np.float64
+    want = O.cond_bwd(*[x.astype(d) for x in ins], vm.astype(d), vV.astype(d), vo.astype(d), exact=exact)
+    other = O.cond_bwd(*[x.astype(d) for x in ins], vm.astype(d), vV.astype(d), vo.astype(d), exact=not exact)
+    for t, w, n in zip(tins[:4], want, ("g_means", "g_covars", "g_opac", "g_betas")):
+        assert rel_l2(host(t.grad), w) < 2e-4, (n, rel_l2(host(t.grad), w))
+    assert rel_l2(host(tins[1].grad), other[1]) > 10 * rel_l2(host(tins[1].grad), want[1])
+
+
 # ------------------------------------------------------------------ projection
 @pytest.mark.parametrize("mode,C", [("covars", 1), ("covars", 3), ("quats", 1), ("quats", 2)])
 def test_projection_fwd_exact(device, mode, C):
@@ -139,6 +163,37 @@ def test_projection_bwd_close(device, mode, C):
         assert rel_l2(host(t), w) < 1e-4, (n, rel_l2(host(t), w))
 
 
+@pytest.mark.parametrize("C", [1, 2])
+def test_projection_covars33_matches_gathered(device, C):
+    # the fused [N,3,3] gather (rendering.py:54-56) against the [N,6] path
+    s = opaque_scene(4000, W=320, H=200, n_cams=C, seed=30 + C)
+    g = np.random.Generator(np.random.PCG64(31))
+    c33 = s["covars33"] + np.tril(g.normal(0, 1, (4000, 3, 3)), -1).astype(np.float32)  # lower part is ignored
+    tm6, tm33 = cu(s["means"]).requires_grad_(True), cu(s["means"]).requires_grad_(True)
+    tc6, tc33 = cu(s["covars6"]).requires_grad_(True), cu(c33).requires_grad_(True)
+    a = W.fully_fused_projection(tm6, tc6, None, None, cu(s["viewmats"]), cu(s["Ks"]), 320, 200,
+                                 calc_compensations=True)
+    b = W.fully_fused_projection_covars33(tm33, tc33, cu(s["viewmats"]), cu(s["Ks"]), 320, 200,
+                                          calc_compensations=True)
+    assert_same(host(a[0]), host(b[0]), "radii")
+    vis = host(a[0]) > 0  # outputs of culled primitives are left unwritten, as in the reference
+    for x, y, n in zip(a[1:], b[1:], ("means2d", "depths", "conics", "comp")):
+        assert_same(host(x)[vis], host(y)[vis], n)
+    outs = []
+    for o in (a, b):
+        gg = np.random.Generator(np.random.PCG64(32))
+        grads = [cu((gg.normal(0, 1, host(t).shape) * (vis if t.dim() == 2 else vis[..., None])).astype(np.float32))
+                 for t in o[1:]]
+        torch.autograd.backward(list(o[1:]), grads)
+    np.testing.assert_allclose(host(tm33.grad), host(tm6.grad), rtol=1e-6, atol=1e-6)
+    G33 = host(tc33.grad)
+    want = np.zeros_like(G33)
+    tri = ([0, 0, 0, 1, 1, 2], [0, 1, 2, 1, 2, 2])
+    want[:, tri[0], tri[1]] = host(tc6.grad)
+    np.testing.assert_allclose(G33, want, rtol=1e-6, atol=1e-6)
+    assert (G33[:, [1, 2, 2], [0, 0, 1]] == 0).all()
+
+
 # ------------------------------------------------------- isect / sort / offsets
 @pytest.mark.parametrize("C", [1, 2])
 def test_isect_and_offsets_exact(device, C):
@@ -184,16 +239,18 @@ def test_radix_sort_stable(device, n, end_bit):
 
 
 # --------------------------------------------------------------------- raster
-def _raster_inputs(scene_fn, W_, H_, C=1, **kw):
+def _raster_inputs(scene_fn, W_, H_, C=1, ts=16, channels=3, **kw):
     s = scene_fn(W=W_, H=H_, n_cams=C, **kw)
     r, m2, dp, cn, _ = O.proj_fwd(s["means"], s["covars6"], None, None, s["viewmats"], s["Ks"], W_, H_)
-    tw, th = (W_ + 15) // 16, (H_ + 15) // 16
-    _, ids, flat = O.isect_tiles(m2, r, dp, 16, tw, th)
+    tw, th = (W_ + ts - 1) // ts, (H_ + ts - 1) // ts
+    _, ids, flat = O.isect_tiles(m2, r, dp, ts, tw, th)
     off = O.isect_offset_encode(ids, C, tw, th)
     op = np.ascontiguousarray(np.broadcast_to(s["opac"], (C, s["opac"].size)))
     be = np.ascontiguousarray(np.broadcast_to(s["beta"], (C, s["beta"].size)))
-    col = np.ascontiguousarray(np.broadcast_to(s["rgb"], (C,) + s["rgb"].shape))
-    return dict(m2=m2, cn=cn, col=col, op=op, be=be, off=off, flat=flat, tw=tw, th=th, C=C)
+    rgb = s["rgb"] if channels == 3 else \
+        np.random.Generator(np.random.PCG64(channels)).uniform(0, 1, (s["rgb"].shape[0], channels)).astype(np.float32)
+    col = np.ascontiguousarray(np.broadcast_to(rgb, (C,) + rgb.shape))
+    return dict(m2=m2, cn=cn, col=col, op=op, be=be, off=off, flat=flat, tw=tw, th=th, C=C, ts=ts)
 
 
 @pytest.mark.parametrize("bg", [False, True])
@@ -232,4 +289,46 @@ def test_raster_bwd_close(device):
     want = O.raster_bwd(*(x[k].astype(d) for k in ("m2", "cn", "col", "op", "be")), W_, H_, 16, x["off"],
                         x["flat"], want_a, want_last, vrc.astype(d), vra.astype(d), T_final=1.0 - want_a[..., 0])
     for a, b, n in zip(got, want, ("v_means2d", "v_conics", "v_colors", "v_opac", "v_betas")):
+        assert rel_l2(host(a), b) < 2e-3, (n, rel_l2(host(a), b))
+
+
+@pytest.mark.parametrize("C,channels,ts,bg,mask", [
+    (1, 1, 16, False, False), (2, 3, 8, True, False), (1, 4, 32, False, False), (1, 7, 16, True, True),
+    (2, 32, 16, False, False), (1, 35, 8, False, True), (1, 3, 32, True, False)])
+def test_raster_bwd_variants(device, C, channels, ts, bg, mask):
+    """Blend backward over channel counts (chunked register tiles), tile
+    sizes, cameras, backgrounds and tile masks.  Upstream gradients are
+    zeroed on pixels near a forward branch point (T cutoff, sigma = 0), so
+    every remaining term is smooth and the FP64 oracle is the exact target."""
+    W_, H_ = 136, 104
+    x = _raster_inputs(lambda **k: opaque_scene(1200, opacity=(0.2, 0.95), **k), W_, H_, C=C, ts=ts,
+                       channels=channels)
+    g = np.random.Generator(np.random.PCG64(100 + channels))
+    bgs = g.uniform(0, 1, (C, channels)).astype(np.float32) if bg else None
+    masks = (g.uniform(0, 1, (C, x["th"], x["tw"])) > 0.25) if mask else None
+    d = np.float64
+    want_c, want_a, want_last, mT, mS = O.raster_fwd(*(x[k].astype(d) for k in ("m2", "cn", "col", "op", "be")),
+                                                    W_, H_, ts, x["off"], x["flat"], backgrounds=bgs, masks=masks,
+                                                    margins=True)
+    tmask = None if masks is None else cu(masks.astype(np.bool_))
+    rc, ra, last, Tf = W.raster_fwd_raw(cu(x["m2"]), cu(x["cn"]), cu(x["col"]), cu(x["op"]), cu(x["be"]),
+                                        None if bgs is None else cu(bgs), tmask, W_, H_, ts, cu(x["off"]),
+                                        cu(x["flat"]))
+    safe = (mT > 1e-3) & (mS > 1e-6)
+    if masks is not None:  # masked tiles blend nothing: no branch points, gradients must stay zero
+        off_px = ~np.repeat(np.repeat(masks, ts, axis=1), ts, axis=2)[:, :H_, :W_]
+        assert off_px.any() and safe[~off_px].mean() > 0.9
+        safe |= off_px
+    assert safe.mean() > 0.9
+    np.testing.assert_allclose(host(rc)[safe], want_c[safe], atol=3e-5)
+    vrc = (g.normal(0, 1, host(rc).shape) * safe[..., None]).astype(np.float32)
+    vra = (g.normal(0, 1, host(ra).shape) * safe[..., None]).astype(np.float32)
+    got = W.raster_bwd_raw(cu(x["m2"]), cu(x["cn"]), cu(x["col"]), cu(x["op"]), cu(x["be"]),
+                           None if bgs is None else cu(bgs), tmask, W_, H_, ts, cu(x["off"]), cu(x["flat"]), ra,
+                           last, Tf, cu(vrc), cu(vra))
+    want = O.raster_bwd(*(x[k].astype(d) for k in ("m2", "cn", "col", "op", "be")), W_, H_, ts, x["off"],
+                        x["flat"], want_a, want_last, vrc.astype(d), vra.astype(d), backgrounds=bgs, masks=masks,
+                        T_final=1.0 - want_a[..., 0])
+    for a, b, n in zip(got, want, ("v_means2d", "v_conics", "v_colors", "v_opac", "v_betas")):
+        assert np.abs(b).max() > 0, n
         assert rel_l2(host(a), b) < 2e-3, (n, rel_l2(host(a), b))

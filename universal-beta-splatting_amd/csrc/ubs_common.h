@@ -119,17 +119,20 @@ __device__ __forceinline__ float dpp_row_sum(float v) {
 // (v_permlane32_swap): lanes 0-31 get a[l] + a[l+32], lanes 32-63 get
 // b[l-32] + b[l].
 __device__ __forceinline__ float swap32_sum(float a, float b) {
-    const auto s = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(unsigned, a), __builtin_bit_cast(unsigned, b),
-                                                    false, false);
-    return __builtin_bit_cast(float, s[0]) + __builtin_bit_cast(float, s[1]);
+    const auto s = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    // copy the two results out of the vector before reinterpreting them:
+    // hipcc lowers __builtin_bit_cast of a vector element s[1] as a load of
+    // s[0], so the bit cast must never see the subscript.
+    const unsigned lo = s[0], hi = s[1];
+    return __uint_as_float(lo) + __uint_as_float(hi);
 }
 
 // Same inside each 32-lane half (v_permlane16_swap): quarter 0 gets
 // a[l] + a[l+16], quarter 1 b[l-16] + b[l], quarters 2/3 likewise.
 __device__ __forceinline__ float swap16_sum(float a, float b) {
-    const auto s = __builtin_amdgcn_permlane16_swap(__builtin_bit_cast(unsigned, a), __builtin_bit_cast(unsigned, b),
-                                                    false, false);
-    return __builtin_bit_cast(float, s[0]) + __builtin_bit_cast(float, s[1]);
+    const auto s = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    const unsigned lo = s[0], hi = s[1];
+    return __uint_as_float(lo) + __uint_as_float(hi);
 }
 
 // Wave-wide sums of 4R per-lane values at ~2.5 VALU ops per value instead of
