@@ -6,7 +6,8 @@ parameter activations, view-direction query, covariance construction,
 conditioning, projection, tile intersection + sort + ranges, blend forward,
 L1 loss against a target image, and the backward of all of it down to the
 leaf parameters.  Inputs are synthetic (seeded) and resident in HBM before the
-timed region.  Prints one JSON line (see DESIGN.md for the field meanings).
+timed region.  Prints one JSON line: throughput, latency percentiles, the
+roofline of the dominant kernel and a CPU baseline from the oracle.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c3|c1]
 """
@@ -38,7 +39,12 @@ CONFIGS = {
 
 
 def algorithmic_bytes(name: str, c: dict, n_isect: int, C: int = 1) -> float:
-    """Minimal HBM bytes one launch of `name` must move (per-unit figures in DESIGN.md)."""
+    """Minimal HBM bytes one launch of `name` must move: per-unit bytes x units.
+
+    Blend: 44 B per intersection (id, xy, conic, opacity, beta, 3 colours),
+    24 B per pixel (colour, alpha, last id, T), and in the backward 40 B of
+    gradient per intersection.  Sort: 8 B key read for the histograms, then
+    per 8-bit pass a read and a write of a 12 B (key, value) pair."""
     N, D, W, H = c["N"], c["D"], c["W"], c["H"]
     tiles = math.ceil(W / 16) * math.ceil(H / 16)
     pix = C * W * H
@@ -94,6 +100,24 @@ def cpu_baseline(cfg: dict, seed: int = 0):
                       f"{O.num_threads()} OpenMP threads): {dt:.2f} s"}
 
 
+def copy_calibration(mib: int, reps: int = 5) -> None:
+    """Plain copies of a known size: each reads and writes mib MiB of HBM."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.arange(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    assert torch.equal(dst[-16:], src[-16:])
+    print(json.dumps({"copy_bytes": n * 4, "reps": reps, "ms_per_copy": round(dt * 1e3, 4),
+                      "GB_per_s_read_plus_write": round(2 * n * 4 / dt / 1e9, 1)}))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,11 +127,17 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fp64-transmittance", action="store_true",
                     help="recover transmittance in fp64 in the blend backward")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run 5 device-to-device copies of MIB MiB (the known-size reference for "
+                         "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
               "Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
 
     import numpy as np
     import torch

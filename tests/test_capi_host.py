@@ -88,3 +88,30 @@ def test_python_api_errors_match_reference():
 def test_bench_refuses_multi_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_c_shim_loads_under_another_package_name():
+    """The `_C` stand-in loads the way a maintainer binds it into the
+    reference's _backend.py (a file-location spec under a foreign package
+    name) and provides every extension function the rasteriser path calls."""
+    import importlib.util
+
+    cuda_dir = os.path.join(ROOT, "universal-beta-splatting_amd", "gsplat", "cuda")
+    spec = importlib.util.spec_from_file_location("ubs_shim_probe_pkg", os.path.join(cuda_dir, "__init__.py"),
+                                                  submodule_search_locations=[cuda_dir])
+    pkg = importlib.util.module_from_spec(spec)
+    sys.modules["ubs_shim_probe_pkg"] = pkg
+    try:
+        spec.loader.exec_module(pkg)
+        shim = importlib.import_module("ubs_shim_probe_pkg._c_shim")
+        for name in ("cond_mean_convariance_opacity_fwd", "cond_mean_convariance_opacity_bwd",
+                     "rot_scale_l_triangle_to_covar_fwd", "rot_scale_l_triangle_to_covar_bwd",
+                     "l_triangle_to_rotmat_fwd", "l_triangle_to_rotmat_bwd", "fully_fused_projection_fwd",
+                     "fully_fused_projection_bwd", "isect_tiles", "isect_offset_encode",
+                     "rasterize_to_pixels_fwd", "rasterize_to_pixels_bwd"):
+            assert callable(getattr(shim, name)), name
+        with pytest.raises(RuntimeError, match="HIP/CUDA device"):
+            shim.isect_offset_encode(torch.zeros(3, dtype=torch.int64), 1, 2, 2)
+    finally:
+        for k in [k for k in sys.modules if k.startswith("ubs_shim_probe_pkg")]:
+            del sys.modules[k]

@@ -135,20 +135,51 @@ __device__ __forceinline__ float swap16_sum(float a, float b) {
     return __uint_as_float(lo) + __uint_as_float(hi);
 }
 
-// Wave-wide sums of 4R per-lane values at ~2.5 VALU ops per value instead of
-// ~11: two reduce-scatter levels (permlane swaps) then DPP row sums.
-// Afterwards out[r] in every lane of quarter Q (lanes 16Q..16Q+15) is the
-// wave total of value index 4r + kQuarterValue[Q], kQuarterValue = {0,2,1,3}.
-// Must be called with all 64 lanes active.
-__device__ __forceinline__ int quarter_value(int q) { return (q == 0) ? 0 : (q == 1) ? 2 : (q == 2) ? 1 : 3; }
+// Wave totals of NV per-lane values at ~2.5 VALU ops per value instead of
+// ~11 for a butterfly per value: two reduce-scatter levels (permlane swaps)
+// leave each 16-lane quarter with partial sums of a different value, and DPP
+// row sums finish them.  Values go in groups of four; after the call every
+// lane of quarter Q (lanes 16Q..16Q+15) holds in out[r] the wave total of
+// value wave_total_index<NV>(r, Q), or of nothing when that returns -1.
+// A trailing group of one or two values costs one group-level instead of a
+// padded four.  Must be called with all 64 lanes active.
+template <int NV>
+struct WaveTotals {
+    static constexpr int kFull = NV / 4;
+    static constexpr int kTail = NV % 4;
+    static constexpr int kGroups = kFull + (kTail ? 1 : 0);
+};
 
-template <int R>
-__device__ __forceinline__ void wave_reduce_scatter4(const float (&v)[4 * R], float (&out)[R]) {
-    float u[2 * R];
+// Quarter Q of a full group holds value 4r + {0, 2, 1, 3}[Q] (bit-reversed Q).
+__device__ __forceinline__ int quarter_value(int q) { return ((q & 1) << 1) | (q >> 1); }
+
+template <int NV>
+__host__ __device__ constexpr int wave_total_index(int r, int q) {
+    constexpr int F = WaveTotals<NV>::kFull, T = WaveTotals<NV>::kTail;
+    if (r < F || T == 3) {
+        const int idx = 4 * r + (((q & 1) << 1) | (q >> 1));
+        return idx < NV ? idx : -1;
+    }
+    if (T == 1) return q == 0 ? 4 * r : -1;
+    return q == 0 ? 4 * r : (q == 2 ? 4 * r + 1 : -1);  // T == 2
+}
+
+template <int NV>
+__device__ __forceinline__ void wave_totals(const float (&v)[NV], float (&out)[WaveTotals<NV>::kGroups]) {
+    constexpr int F = WaveTotals<NV>::kFull, T = WaveTotals<NV>::kTail;
 #pragma unroll
-    for (int p = 0; p < 2 * R; ++p) u[p] = swap32_sum(v[2 * p], v[2 * p + 1]);
-#pragma unroll
-    for (int r = 0; r < R; ++r) out[r] = dpp_row_sum(swap16_sum(u[2 * r], u[2 * r + 1]));
+    for (int r = 0; r < F; ++r)
+        out[r] = dpp_row_sum(swap16_sum(swap32_sum(v[4 * r], v[4 * r + 1]), swap32_sum(v[4 * r + 2], v[4 * r + 3])));
+    if constexpr (T == 3) {
+        out[F] = dpp_row_sum(swap16_sum(swap32_sum(v[4 * F], v[4 * F + 1]), swap32_sum(v[4 * F + 2], 0.f)));
+    } else if constexpr (T == 2) {
+        // halves hold the two values; the self-swap sums each half's rows
+        const float u = swap32_sum(v[4 * F], v[4 * F + 1]);
+        out[F] = dpp_row_sum(swap16_sum(u, u));
+    } else if constexpr (T == 1) {
+        const float u = swap32_sum(v[4 * F], v[4 * F]);
+        out[F] = dpp_row_sum(swap16_sum(u, u));
+    }
 }
 
 __device__ __forceinline__ int wave_max_i32(int v) {

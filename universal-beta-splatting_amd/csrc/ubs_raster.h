@@ -72,6 +72,59 @@ __device__ __forceinline__ uint32_t subblock_mask(float mx, float my, float hx, 
     return m;
 }
 
+// Whether the sigma < 1 ellipse of conic [[a, b], [b, c]] centred at (mx, my)
+// reaches the rectangle [lo_x, hi_x] x [lo_y, hi_y] of pixel centres.  The
+// quadratic form is convex, so when the centre lies outside the rectangle
+// its minimum over the rectangle lies on one of the four edges, where it is
+// a 1-D quadratic minimised in closed form and clamped to the edge.  The
+// threshold carries a margin of 1e-3 plus 1e-5 of the largest absolute term
+// sum over the rectangle, far above the rounding of the per-pixel fp32
+// sigma, so a pixel whose sigma the blend would accept is never culled.
+__host__ __device__ inline bool ellipse_meets_rect(float mx, float my, float a, float b, float c, float lo_x,
+                                                   float hi_x, float lo_y, float hi_y) {
+    const float dx0 = mx - hi_x, dx1 = mx - lo_x;  // offsets mean - pixel over the rectangle
+    const float dy0 = my - hi_y, dy1 = my - lo_y;
+    if (dx0 <= 0.f && dx1 >= 0.f && dy0 <= 0.f && dy1 >= 0.f) return true;
+    if (!(a > 0.f && c > 0.f)) return true;
+    const float ia = 1.f / a, ic = 1.f / c;
+    float best = __builtin_huge_valf();
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+        const float dx = e ? dx1 : dx0;
+        const float dy = fminf(fmaxf(-b * dx * ic, dy0), dy1);
+        best = fminf(best, (a * dx * dx + c * dy * dy) + 2.f * b * dx * dy);
+    }
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+        const float dy = e ? dy1 : dy0;
+        const float dx = fminf(fmaxf(-b * dy * ia, dx0), dx1);
+        best = fminf(best, (a * dx * dx + c * dy * dy) + 2.f * b * dx * dy);
+    }
+    const float mxx = fmaxf(dx0 * dx0, dx1 * dx1), myy = fmaxf(dy0 * dy0, dy1 * dy1);
+    const float sabs = (a * mxx + c * myy) + 2.f * fabsf(b) * sqrtf(mxx * myy);
+    return best < 1.001f + 1e-5f * sabs;
+}
+
+// subblock_mask refined by the exact ellipse test for every sub-block the
+// support box reaches.
+template <int SB>
+__device__ __forceinline__ uint32_t subblock_mask_exact(float mx, float my, float a, float b, float c, float hx,
+                                                        float hy, float x0, float y0) {
+    uint32_t m = subblock_mask<SB>(mx, my, hx, hy, x0, y0);
+    if (!(hx < __builtin_huge_valf())) return m;
+    uint32_t out = m;
+#pragma unroll
+    for (int sy = 0; sy < SB; ++sy)
+#pragma unroll
+        for (int sx = 0; sx < SB; ++sx) {
+            const uint32_t bit = 1u << (sy * SB + sx);
+            if (!(m & bit)) continue;
+            const float lo_x = x0 + (float)(sx * 8) + 0.5f, lo_y = y0 + (float)(sy * 8) + 0.5f;
+            if (!ellipse_meets_rect(mx, my, a, b, c, lo_x, lo_x + 7.f, lo_y, lo_y + 7.f)) out &= ~bit;
+        }
+    return out;
+}
+
 // Channel chunks the kernels are instantiated for; wider colour vectors are
 // blended chunk by chunk (the transmittance does not depend on colour).  The
 // widest chunk is bounded by LDS: 32 channels for 8x8 and 16x16 tiles, 4 for

@@ -39,7 +39,7 @@ __global__ __launch_bounds__(TS *TS) void raster_bwd_kernel(
     constexpr int SB = TS / 8;
     constexpr int NW = BS / 64;
     constexpr int NV = 7 + CDIM;  // xy(2) conic(3) opacity beta colour(CDIM)
-    constexpr int NR = (NV + 3) / 4;
+    constexpr int NG = WaveTotals<NV>::kGroups;
     __shared__ float4 s_geo[BS];   // x, y, a, b
     __shared__ float4 s_geo2[BS];  // c, opacity, beta, id (bits)
     __shared__ float s_col[CDIM][BS];
@@ -94,6 +94,11 @@ __global__ __launch_bounds__(TS *TS) void raster_bwd_kernel(
     if (tid < BS) s_hit[tid] = 0;
     __syncthreads();
     const int32_t wave_bin_max = s_wmax[wave];
+    // the value each quarter leader (lanes 0, 16, 32, 48) adds per group
+    const bool q_leader = (lane & 15) == 0;
+    int q_idx[NG];
+#pragma unroll
+    for (int r = 0; r < NG; ++r) q_idx[r] = wave_total_index<NV>(r, lane >> 4);
 
     const int32_t nb = range_end > range_start ? (range_end - range_start + BS - 1) / BS : 0;
     for (int32_t b = 0; b < nb; ++b) {
@@ -110,7 +115,7 @@ __global__ __launch_bounds__(TS *TS) void raster_bwd_kernel(
             for (int k = 0; k < CDIM; ++k) s_col[k][tid] = cp[k];
             float hx, hy;
             support_box(ca, cb, cc, hx, hy);
-            s_mask[tid] = subblock_mask<SB>(xy.x, xy.y, hx, hy, x0, y0);
+            s_mask[tid] = subblock_mask_exact<SB>(xy.x, xy.y, ca, cb, cc, hx, hy, x0, y0);
         }
         __syncthreads();
         int n_mine = 0;
@@ -141,9 +146,9 @@ __global__ __launch_bounds__(TS *TS) void raster_bwd_kernel(
             const float sigma = __builtin_fmaf(2.f * g1.w * dx, dy, __builtin_fmaf(g1.z * dx, dx, (g2.x * dy) * dy));
             const bool valid = inside && (batch_end - t) <= bin_final && sigma >= 0.f && sigma < 1.f;
             if (__ballot(valid) == 0ull) continue;
-            float v[4 * NR];
+            float v[NV];
 #pragma unroll
-            for (int q = 0; q < 4 * NR; ++q) v[q] = 0.f;
+            for (int q = 0; q < NV; ++q) v[q] = 0.f;
             if (valid) {
                 const float opac = g2.y, beta = g2.z;
                 const float oms = 1.f - sigma;
@@ -183,16 +188,13 @@ __global__ __launch_bounds__(TS *TS) void raster_bwd_kernel(
                 for (int q = 0; q < CDIM; ++q) buf[q] = __builtin_fmaf(s_col[q][t], fac, buf[q]);
             }
             // wave totals, scattered over the four 16-lane quarters; one lane
-            // per quarter adds its NR totals into the primitive's accumulator
-            float tot[NR];
-            wave_reduce_scatter4<NR>(v, tot);
-            if ((lane & 15) == 0) {
-                const int qv = quarter_value(lane >> 4);
+            // per quarter adds its totals into the primitive's accumulator
+            float tot[NG];
+            wave_totals<NV>(v, tot);
+            if (q_leader) {
 #pragma unroll
-                for (int r = 0; r < NR; ++r) {
-                    const int idx = 4 * r + qv;
-                    if (idx < NV) atomicAdd(&s_acc[idx][t], tot[r]);
-                }
+                for (int r = 0; r < NG; ++r)
+                    if (q_idx[r] >= 0) atomicAdd(&s_acc[q_idx[r]][t], tot[r]);
                 if (lane == 0) s_hit[t] = 1;
             }
         }

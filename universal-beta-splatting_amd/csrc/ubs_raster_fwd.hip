@@ -92,7 +92,7 @@ __global__ __launch_bounds__(TS *TS) void raster_fwd_kernel(
             for (int k = 0; k < CDIM; ++k) s_col[k][tid] = cp[k];
             float hx, hy;
             support_box(ca, cb, cc, hx, hy);
-            s_mask[tid] = subblock_mask<SB>(xy.x, xy.y, hx, hy, x0, y0);
+            s_mask[tid] = subblock_mask_exact<SB>(xy.x, xy.y, ca, cb, cc, hx, hy, x0, y0);
         }
         __syncthreads();
         // ---- per-wave compaction of the batch to this sub-block's primitives

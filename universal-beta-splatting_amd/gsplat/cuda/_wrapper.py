@@ -361,75 +361,164 @@ def raster_bwd_raw(means2d, conics, colors, opacities, betas, backgrounds, masks
     return v_means2d, v_conics, v_colors, v_opacities, v_betas
 
 
+def cond_fwd_raw(means, covars, opacities, betas, query):
+    """cond_mean_convariance_opacity_fwd: (means [N,3], covars [N,3,3], opacities [N,1])."""
+    require_cuda(means, covars, opacities, betas, query)
+    assert means.dim() == 2, "means must be [N,D]"
+    assert covars.dim() == 3, "covars must be [N,D,D]"
+    assert opacities.dim() == 2 and opacities.size(1) == 1, "opacities must be [N,1]"
+    assert betas.dim() == 2 and query.dim() == 2, "betas/query must be [N,C]"
+    N, D = means.shape
+    assert covars.shape == (N, D, D), "covars must match [N,D,D]"
+    assert query.size(0) == N and betas.size(0) == N, "batch sizes must match"
+    assert D >= 4, "D must be >= 4"
+    assert D - 3 == betas.size(1) == query.size(1), "C must be D-3"
+    om = torch.empty((N, 3), dtype=means.dtype, device=means.device)
+    oV = torch.empty((N, 3, 3), dtype=means.dtype, device=means.device)
+    oo = torch.empty((N, 1), dtype=means.dtype, device=means.device)
+    _call("ubs_cond_mean_convariance_opacity_fwd", N, D, ptr(means), ptr(covars), ptr(opacities),
+          ptr(betas), ptr(query), ptr(om), ptr(oV), ptr(oo), stream_of(means))
+    return om, oV, oo
+
+
+def cond_bwd_raw(means, covars, opacities, betas, query, v_m, v_V, v_o):
+    """cond_mean_convariance_opacity_bwd: (v_means [N,D], v_covars [N,D,D], v_opacities [N,1], v_betas [N,C])."""
+    require_cuda(means, covars, opacities, betas, query, v_m, v_V, v_o)
+    N, D = means.shape
+    v_m, v_V, v_o = v_m.contiguous(), v_V.contiguous(), v_o.contiguous()
+    gm = torch.empty_like(means)
+    gV = torch.empty_like(covars)
+    go = torch.empty_like(opacities)
+    gb = torch.empty_like(betas)
+    _call("ubs_cond_mean_convariance_opacity_bwd", N, D, ptr(means), ptr(covars), ptr(opacities),
+          ptr(betas), ptr(query), ptr(v_m), ptr(v_V), ptr(v_o), ptr(gm), ptr(gV), ptr(go), ptr(gb),
+          stream_of(means))
+    return gm, gV, go, gb
+
+
+def rotmat_fwd_raw(l_triangle):
+    """l_triangle_to_rotmat_fwd: [N,3] -> [N,3,3]."""
+    require_cuda(l_triangle)
+    N = l_triangle.size(0)
+    R = torch.empty((N, 3, 3), dtype=l_triangle.dtype, device=l_triangle.device)
+    _call("ubs_l_triangle_to_rotmat_fwd", N, ptr(l_triangle), ptr(R), stream_of(l_triangle))
+    return R
+
+
+def rotmat_bwd_raw(l_triangle, v_R):
+    """l_triangle_to_rotmat_bwd: v_R [N,3,3] -> v_l_triangle [N,3]."""
+    require_cuda(l_triangle, v_R)
+    v_R = v_R.contiguous()
+    out = torch.empty_like(l_triangle)
+    _call("ubs_l_triangle_to_rotmat_bwd", l_triangle.size(0), ptr(v_R), ptr(out), stream_of(v_R))
+    return out
+
+
+def covar_fwd_raw(rot, scale, l_triangle, rest_i, rest_j, spatial_block):
+    """rot_scale_l_triangle_to_covar_fwd: [N,D,D] (or [N,3,3] for the spatial block)."""
+    require_cuda(rot, scale, l_triangle, rest_i, rest_j)
+    N, D = scale.shape
+    rest_i = rest_i.to(torch.int32).contiguous()
+    rest_j = rest_j.to(torch.int32).contiguous()
+    dim = 3 if (spatial_block or D == 3) else D
+    covar = torch.empty((N, dim, dim), dtype=rot.dtype, device=rot.device)
+    _call("ubs_rot_scale_l_triangle_to_covar_fwd", N, D, ptr(rot), ptr(scale), ptr(l_triangle),
+          ptr(rest_i), ptr(rest_j), rest_i.numel(), int(spatial_block), ptr(covar), stream_of(rot))
+    return covar
+
+
+def covar_bwd_raw(rot, scale, l_triangle, rest_i, rest_j, spatial_block, v_covar):
+    """rot_scale_l_triangle_to_covar_bwd: (v_rot [N,3,3], v_scale [N,D], v_l_triangle [N,D(D-1)/2])."""
+    require_cuda(rot, scale, l_triangle, rest_i, rest_j, v_covar)
+    N, D = scale.shape
+    rest_i = rest_i.to(torch.int32).contiguous()
+    rest_j = rest_j.to(torch.int32).contiguous()
+    v_covar = v_covar.contiguous()
+    v_rot = torch.empty_like(rot)
+    v_scale = torch.empty_like(scale)
+    v_l = torch.empty_like(l_triangle)
+    _call("ubs_rot_scale_l_triangle_to_covar_bwd", N, D, ptr(rot), ptr(scale), ptr(l_triangle),
+          ptr(rest_i), ptr(rest_j), rest_i.numel(), int(spatial_block), ptr(v_covar), ptr(v_rot),
+          ptr(v_scale), ptr(v_l), stream_of(rot))
+    return v_rot, v_scale, v_l
+
+
+def projection_fwd_raw(means, covars, quats, scales, viewmats, Ks, width, height, eps2d, near_plane, far_plane,
+                       radius_clip, calc_compensations, ortho, covar_layout=0):
+    """fully_fused_projection_fwd: (radii, means2d, depths, conics, compensations or None)."""
+    require_cuda(means, covars, quats, scales, viewmats, Ks)
+    C, N = viewmats.size(0), means.size(0)
+    dev = means.device
+    radii = torch.empty((C, N), dtype=torch.int32, device=dev)
+    means2d = torch.empty((C, N, 2), dtype=means.dtype, device=dev)
+    depths = torch.empty((C, N), dtype=means.dtype, device=dev)
+    conics = torch.empty((C, N, 3), dtype=means.dtype, device=dev)
+    comp = torch.zeros((C, N), dtype=means.dtype, device=dev) if calc_compensations else None
+    _call("ubs_fully_fused_projection_fwd", C, N, ptr(means), ptr(covars), covar_layout, ptr(quats),
+          ptr(scales), ptr(viewmats), ptr(Ks), width, height, eps2d, near_plane, far_plane, radius_clip,
+          int(ortho), ptr(radii), ptr(means2d), ptr(depths), ptr(conics), ptr(comp), stream_of(means))
+    return radii, means2d, depths, conics, comp
+
+
+def projection_bwd_raw(means, covars, quats, scales, viewmats, Ks, width, height, eps2d, ortho, radii, conics,
+                       compensations, v_means2d, v_depths, v_conics, v_compensations, viewmats_requires_grad,
+                       covar_layout=0):
+    """fully_fused_projection_bwd: (v_means, v_covars, v_quats, v_scales, v_viewmats); the
+    entries of the absent covariance parametrisation and v_viewmats when not required are None."""
+    require_cuda(means, covars, quats, scales, viewmats, Ks, radii, conics, v_means2d, v_depths, v_conics)
+    C, N = viewmats.size(0), means.size(0)
+    v_means = torch.empty_like(means)
+    v_covars = torch.empty_like(covars) if covars is not None else None
+    v_quats = torch.empty_like(quats) if covars is None else None
+    v_scales = torch.empty_like(scales) if covars is None else None
+    v_viewmats = torch.empty_like(viewmats) if viewmats_requires_grad else None
+    v_means2d = v_means2d.contiguous()
+    v_depths = v_depths.contiguous()
+    v_conics = v_conics.contiguous()
+    if v_compensations is not None:
+        v_compensations = v_compensations.contiguous()
+    _call("ubs_fully_fused_projection_bwd", C, N, ptr(means), ptr(covars), covar_layout, ptr(quats),
+          ptr(scales), ptr(viewmats), ptr(Ks), width, height, eps2d, int(ortho), ptr(radii),
+          ptr(conics), ptr(compensations), ptr(v_means2d), ptr(v_depths), ptr(v_conics),
+          ptr(v_compensations) if compensations is not None else None, ptr(v_means), ptr(v_covars),
+          ptr(v_quats), ptr(v_scales), ptr(v_viewmats), stream_of(means))
+    return v_means, v_covars, v_quats, v_scales, v_viewmats
+
+
 # ----------------------------------------------------------------------------
 # autograd Functions (reference: _wrapper.py:573-684, 807-1050)
 # ----------------------------------------------------------------------------
 class _CondMeanConvarianceOpacity(torch.autograd.Function):
     @staticmethod
     def forward(ctx, means, covars, opacities, betas, query):
-        require_cuda(means, covars, opacities, betas, query)
-        assert means.dim() == 2, "means must be [N,D]"
-        assert covars.dim() == 3, "covars must be [N,D,D]"
-        assert opacities.dim() == 2 and opacities.size(1) == 1, "opacities must be [N,1]"
-        assert betas.dim() == 2 and query.dim() == 2, "betas/query must be [N,C]"
-        N, D = means.shape
-        assert covars.shape == (N, D, D), "covars must match [N,D,D]"
-        assert query.size(0) == N and betas.size(0) == N, "batch sizes must match"
-        assert D >= 4, "D must be >= 4"
-        assert D - 3 == betas.size(1) == query.size(1), "C must be D-3"
-        om = torch.empty((N, 3), dtype=means.dtype, device=means.device)
-        oV = torch.empty((N, 3, 3), dtype=means.dtype, device=means.device)
-        oo = torch.empty((N, 1), dtype=means.dtype, device=means.device)
-        _call("ubs_cond_mean_convariance_opacity_fwd", N, D, ptr(means), ptr(covars), ptr(opacities),
-              ptr(betas), ptr(query), ptr(om), ptr(oV), ptr(oo), stream_of(means))
+        om, oV, oo = cond_fwd_raw(means, covars, opacities, betas, query)
         ctx.save_for_backward(means, covars, opacities, betas, query)
         return om, oV, oo
 
     @staticmethod
     def backward(ctx, v_m, v_V, v_o):
         means, covars, opacities, betas, query = ctx.saved_tensors
-        N, D = means.shape
-        v_m, v_V, v_o = v_m.contiguous(), v_V.contiguous(), v_o.contiguous()
-        gm = torch.empty_like(means)
-        gV = torch.empty_like(covars)
-        go = torch.empty_like(opacities)
-        gb = torch.empty_like(betas)
-        _call("ubs_cond_mean_convariance_opacity_bwd", N, D, ptr(means), ptr(covars), ptr(opacities),
-              ptr(betas), ptr(query), ptr(v_m), ptr(v_V), ptr(v_o), ptr(gm), ptr(gV), ptr(go), ptr(gb),
-              stream_of(means))
+        gm, gV, go, gb = cond_bwd_raw(means, covars, opacities, betas, query, v_m, v_V, v_o)
         return gm, gV, go, gb, None
 
 
 class _LTriangleToRotmat(torch.autograd.Function):
     @staticmethod
     def forward(ctx, l_triangle):
-        require_cuda(l_triangle)
-        N = l_triangle.size(0)
-        R = torch.empty((N, 3, 3), dtype=l_triangle.dtype, device=l_triangle.device)
-        _call("ubs_l_triangle_to_rotmat_fwd", N, ptr(l_triangle), ptr(R), stream_of(l_triangle))
+        R = rotmat_fwd_raw(l_triangle)
         ctx.save_for_backward(l_triangle)
         return R
 
     @staticmethod
     def backward(ctx, v_R):
         (l_triangle,) = ctx.saved_tensors
-        v_R = v_R.contiguous()
-        out = torch.empty_like(l_triangle)
-        _call("ubs_l_triangle_to_rotmat_bwd", l_triangle.size(0), ptr(v_R), ptr(out), stream_of(v_R))
-        return out
+        return rotmat_bwd_raw(l_triangle, v_R)
 
 
 class _RotScaleLTriangleToCovar(torch.autograd.Function):
     @staticmethod
     def forward(ctx, rot, scale, l_triangle, rest_i, rest_j, spatial_block):
-        require_cuda(rot, scale, l_triangle, rest_i, rest_j)
-        N, D = scale.shape
-        rest_i = rest_i.to(torch.int32).contiguous()
-        rest_j = rest_j.to(torch.int32).contiguous()
-        dim = 3 if (spatial_block or D == 3) else D
-        covar = torch.empty((N, dim, dim), dtype=rot.dtype, device=rot.device)
-        _call("ubs_rot_scale_l_triangle_to_covar_fwd", N, D, ptr(rot), ptr(scale), ptr(l_triangle),
-              ptr(rest_i), ptr(rest_j), rest_i.numel(), int(spatial_block), ptr(covar), stream_of(rot))
+        covar = covar_fwd_raw(rot, scale, l_triangle, rest_i, rest_j, spatial_block)
         ctx.save_for_backward(rot, scale, l_triangle, rest_i, rest_j)
         ctx.spatial_block = spatial_block
         return covar
@@ -439,14 +528,7 @@ class _RotScaleLTriangleToCovar(torch.autograd.Function):
         rot, scale, l_triangle, rest_i, rest_j = ctx.saved_tensors
         if not any(ctx.needs_input_grad[:3]):
             return (None, None, None, None, None, None)
-        N, D = scale.shape
-        v_covar = v_covar.contiguous()
-        v_rot = torch.empty_like(rot)
-        v_scale = torch.empty_like(scale)
-        v_l = torch.empty_like(l_triangle)
-        _call("ubs_rot_scale_l_triangle_to_covar_bwd", N, D, ptr(rot), ptr(scale), ptr(l_triangle),
-              ptr(rest_i), ptr(rest_j), rest_i.numel(), int(ctx.spatial_block), ptr(v_covar), ptr(v_rot),
-              ptr(v_scale), ptr(v_l), stream_of(rot))
+        v_rot, v_scale, v_l = covar_bwd_raw(rot, scale, l_triangle, rest_i, rest_j, ctx.spatial_block, v_covar)
         return v_rot, v_scale, v_l, None, None, None
 
 
@@ -454,17 +536,9 @@ class _FullyFusedProjection(torch.autograd.Function):
     @staticmethod
     def forward(ctx, means, covars, quats, scales, viewmats, Ks, width, height, eps2d, near_plane,
                 far_plane, radius_clip, calc_compensations, ortho, covar_layout):
-        require_cuda(means, covars, quats, scales, viewmats, Ks)
-        C, N = viewmats.size(0), means.size(0)
-        dev = means.device
-        radii = torch.empty((C, N), dtype=torch.int32, device=dev)
-        means2d = torch.empty((C, N, 2), dtype=means.dtype, device=dev)
-        depths = torch.empty((C, N), dtype=means.dtype, device=dev)
-        conics = torch.empty((C, N, 3), dtype=means.dtype, device=dev)
-        comp = torch.zeros((C, N), dtype=means.dtype, device=dev) if calc_compensations else None
-        _call("ubs_fully_fused_projection_fwd", C, N, ptr(means), ptr(covars), covar_layout, ptr(quats),
-              ptr(scales), ptr(viewmats), ptr(Ks), width, height, eps2d, near_plane, far_plane, radius_clip,
-              int(ortho), ptr(radii), ptr(means2d), ptr(depths), ptr(conics), ptr(comp), stream_of(means))
+        radii, means2d, depths, conics, comp = projection_fwd_raw(
+            means, covars, quats, scales, viewmats, Ks, width, height, eps2d, near_plane, far_plane, radius_clip,
+            calc_compensations, ortho, covar_layout)
         ctx.save_for_backward(means, covars, quats, scales, viewmats, Ks, radii, conics, comp)
         ctx.width, ctx.height, ctx.eps2d, ctx.ortho = width, height, eps2d, ortho
         ctx.covar_layout = covar_layout
@@ -473,23 +547,9 @@ class _FullyFusedProjection(torch.autograd.Function):
     @staticmethod
     def backward(ctx, v_radii, v_means2d, v_depths, v_conics, v_compensations):
         means, covars, quats, scales, viewmats, Ks, radii, conics, comp = ctx.saved_tensors
-        C, N = viewmats.size(0), means.size(0)
-        v_means = torch.empty_like(means)
-        v_covars = torch.empty_like(covars) if covars is not None else None
-        v_quats = torch.empty_like(quats) if covars is None else None
-        v_scales = torch.empty_like(scales) if covars is None else None
-        want_vm = ctx.needs_input_grad[4]
-        v_viewmats = torch.empty_like(viewmats) if want_vm else None
-        v_means2d = v_means2d.contiguous()
-        v_depths = v_depths.contiguous()
-        v_conics = v_conics.contiguous()
-        if v_compensations is not None:
-            v_compensations = v_compensations.contiguous()
-        _call("ubs_fully_fused_projection_bwd", C, N, ptr(means), ptr(covars), ctx.covar_layout, ptr(quats),
-              ptr(scales), ptr(viewmats), ptr(Ks), ctx.width, ctx.height, ctx.eps2d, int(ctx.ortho), ptr(radii),
-              ptr(conics), ptr(comp), ptr(v_means2d), ptr(v_depths), ptr(v_conics),
-              ptr(v_compensations) if comp is not None else None, ptr(v_means), ptr(v_covars),
-              ptr(v_quats), ptr(v_scales), ptr(v_viewmats), stream_of(means))
+        v_means, v_covars, v_quats, v_scales, v_viewmats = projection_bwd_raw(
+            means, covars, quats, scales, viewmats, Ks, ctx.width, ctx.height, ctx.eps2d, ctx.ortho, radii, conics,
+            comp, v_means2d, v_depths, v_conics, v_compensations, ctx.needs_input_grad[4], ctx.covar_layout)
         if not ctx.needs_input_grad[0]:
             v_means = None
         if not ctx.needs_input_grad[1]:
