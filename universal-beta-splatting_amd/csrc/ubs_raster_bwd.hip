@@ -8,10 +8,12 @@
 //     exact T after the last contributor (kept by the forward) by dividing
 //     by (1 - alpha) -- the forward and backward evaluate alpha with the same
 //     function, so the divisors are bit-identical;
-//   * forms the gradients of rasterize_to_pixels_bwd.cu:200-247;
-//   * sums them over the 64 lanes with DPP butterflies, and one lane adds the
-//     totals into an LDS accumulator of the primitive (ds_add_f32), so the
-//     four waves of a tile share one set of partial sums;
+//   * forms the gradients of rasterize_to_pixels_bwd.cu:200-247, branch-free
+//     (lanes without a contribution compute exact zeros);
+//   * sums them over the 64 lanes with a permlane reduce-scatter that leaves
+//     each 16-lane quarter with different totals (wave_totals), and one lane
+//     per quarter adds them into the primitive's LDS accumulator
+//     (ds_add_f32), so the four waves of a tile share one set of partial sums;
 // and after the batch each thread flushes one primitive's accumulator to the
 // per-primitive gradients with hardware float atomics (one set of atomics per
 // (tile, primitive) instead of one per (wave, primitive)).
@@ -44,7 +46,6 @@ __global__ __launch_bounds__(TS *TS) void raster_bwd_kernel(
     __shared__ float4 s_geo2[BS];  // c, opacity, beta, id (bits)
     __shared__ float s_col[CDIM][BS];
     __shared__ uint32_t s_mask[BS];
-    __shared__ uint16_t s_list[NW][BS];
     __shared__ float s_acc[NV][BS + 1];  // +1: the four quarter lanes hit distinct banks
     __shared__ uint8_t s_hit[BS];
     __shared__ int32_t s_wmax[NW];
@@ -75,15 +76,16 @@ __global__ __launch_bounds__(TS *TS) void raster_bwd_kernel(
     using TT = typename std::conditional<F64T, double, float>::type;
     TT T = (TT)Tf;
     const int32_t bin_final = inside ? last_ids[pix] : 0;
-    float vrc[CDIM], buf[CDIM];
+    float vrc[CDIM];
     float bg_dot = 0.f;
 #pragma unroll
     for (int k = 0; k < CDIM; ++k) {
         vrc[k] = inside ? v_render_colors[pix * channels + coff + k] : 0.f;
-        buf[k] = 0.f;
         if (backgrounds) bg_dot = __builtin_fmaf(backgrounds[cam * channels + coff + k], vrc[k], bg_dot);
     }
     const float vra = (inside && first_chunk) ? v_render_alphas[pix] : 0.f;
+    const float vra_bg = vra - bg_dot;  // d/d(T_final) of the pixel's outputs
+    float bufv = 0.f;                   // sum over later primitives of c . v_render_color * alpha T
 
     // the largest last contributor over the wave bounds the useful range
     {
@@ -118,84 +120,75 @@ __global__ __launch_bounds__(TS *TS) void raster_bwd_kernel(
             s_mask[tid] = subblock_mask_exact<SB>(xy.x, xy.y, ca, cb, cc, hx, hy, x0, y0);
         }
         __syncthreads();
-        int n_mine = 0;
+        // ---- back-to-front over this sub-block's primitives of the batch
+        // whose isect index is at most the wave's largest last contributor:
+        // one ballot per 64 entries, then its set bits in order (the entry
+        // index is wave-uniform, so its LDS reads are broadcasts)
         for (int c0 = 0; c0 < cnt; c0 += 64) {
             const int e = c0 + lane;
-            const bool hit = e < cnt && ((s_mask[e] >> wave) & 1u) && (batch_end - e) <= wave_bin_max;
-            const uint64_t bal = __ballot(hit);
-            if (hit) {
-                const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-                s_list[wave][n_mine + __popcll(bal & lt)] = (uint16_t)e;
-            }
-            n_mine += __popcll(bal);
-        }
-        // software-pipelined: the next entry's LDS reads are in flight while
-        // the current one is processed
-        int t_nx = n_mine > 0 ? s_list[wave][0] : 0;
-        float4 g1_nx = s_geo[t_nx], g2_nx = s_geo2[t_nx];
-        for (int k = 0; k < n_mine; ++k) {
-            const int t = t_nx;
-            const float4 g1 = g1_nx;
-            const float4 g2 = g2_nx;
-            if (k + 1 < n_mine) {
-                t_nx = s_list[wave][k + 1];
-                g1_nx = s_geo[t_nx];
-                g2_nx = s_geo2[t_nx];
-            }
-            const float dx = g1.x - px, dy = g1.y - py;
-            const float sigma = __builtin_fmaf(2.f * g1.w * dx, dy, __builtin_fmaf(g1.z * dx, dx, (g2.x * dy) * dy));
-            const bool valid = inside && (batch_end - t) <= bin_final && sigma >= 0.f && sigma < 1.f;
-            if (__ballot(valid) == 0ull) continue;
-            float v[NV];
-#pragma unroll
-            for (int q = 0; q < NV; ++q) v[q] = 0.f;
-            if (valid) {
+            uint64_t todo = __ballot(e < cnt && ((s_mask[e] >> wave) & 1u) && (batch_end - e) <= wave_bin_max);
+            while (todo != 0ull) {
+                const int t = c0 + __builtin_ctzll(todo);
+                todo &= todo - 1ull;
+                const float4 g1 = s_geo[t];
+                const float4 g2 = s_geo2[t];
+                const float dx = g1.x - px, dy = g1.y - py;
+                const float sigma = __builtin_fmaf(2.f * g1.w * dx, dy, __builtin_fmaf(g1.z * dx, dx, (g2.x * dy) * dy));
+                const bool valid = inside && (batch_end - t) <= bin_final && sigma >= 0.f && sigma < 1.f;
+                if (__ballot(valid) == 0ull) continue;
+                // Branch-free: lanes without a contribution run the same
+                // instructions with alpha = 0, (1 - sigma) = 1 and a zero v_alpha,
+                // so they add exact zeros and leave T and the colour sum unchanged.
+                float v[NV];
                 const float opac = g2.y, beta = g2.z;
-                const float oms = 1.f - sigma;
+                const float oms = valid ? 1.f - sigma : 1.f;
                 const float lg = __builtin_amdgcn_logf(oms);  // log2(1 - sigma)
                 const float vis = __builtin_amdgcn_exp2f(beta * lg);
-                const float alpha = fminf(0.999f, opac * vis);
+                const float ov = opac * vis;
+                const float alpha = valid ? fminf(0.999f, ov) : 0.f;
                 float ra;
                 if constexpr (F64T) {
                     const double rad = 1.0 / (1.0 - (double)alpha);
-                    T *= rad;
+                    T = valid ? T * rad : T;
                     ra = (float)rad;
                 } else {
                     ra = __builtin_amdgcn_rcpf(1.f - alpha);
-                    T *= ra;
+                    T = valid ? T * ra : T;
                 }
                 const float Tc = (float)T;
                 const float fac = alpha * Tc;
-                float v_alpha = 0.f;
+                // sum_k (c_k T - buf_k ra) v_k = T (c . v) - ra (buf . v), with
+                // bufv = buf . v carried as one running sum
+                float cdot = 0.f;
 #pragma unroll
                 for (int q = 0; q < CDIM; ++q) {
                     v[7 + q] = fac * vrc[q];
-                    v_alpha = __builtin_fmaf(s_col[q][t] * Tc - buf[q] * ra, vrc[q], v_alpha);
+                    cdot = __builtin_fmaf(s_col[q][t], vrc[q], cdot);
                 }
-                v_alpha = __builtin_fmaf(Tf * ra, vra - bg_dot, v_alpha);
-                if (opac * vis <= 0.999f) {
-                    const float pm1 = __builtin_amdgcn_exp2f((beta - 1.f) * lg);  // (1-sigma)^(beta-1)
-                    const float v_sigma = -v_alpha * opac * beta * pm1;
-                    v[2] = dx * dx * v_sigma;
-                    v[3] = 2.f * dx * dy * v_sigma;
-                    v[4] = dy * dy * v_sigma;
-                    v[0] = 2.f * v_sigma * __builtin_fmaf(g1.z, dx, g1.w * dy);
-                    v[1] = 2.f * v_sigma * __builtin_fmaf(g1.w, dx, g2.x * dy);
-                    v[5] = vis * v_alpha;
-                    v[6] = v_alpha * opac * vis * (lg * 0.69314718055994531f);  // ln(1-sigma)
+                float v_alpha = __builtin_fmaf(Tc, cdot, -(ra * bufv));
+                v_alpha = __builtin_fmaf(Tf * ra, vra_bg, v_alpha);
+                bufv = __builtin_fmaf(cdot, fac, bufv);
+                // geometry, opacity and beta get gradient only below the 0.999 clamp
+                const float va = (valid && ov <= 0.999f) ? v_alpha : 0.f;
+                const float pm1 = __builtin_amdgcn_exp2f((beta - 1.f) * lg);  // (1-sigma)^(beta-1)
+                const float v_sigma = -va * opac * beta * pm1;
+                v[2] = dx * dx * v_sigma;
+                v[3] = 2.f * dx * dy * v_sigma;
+                v[4] = dy * dy * v_sigma;
+                v[0] = 2.f * v_sigma * __builtin_fmaf(g1.z, dx, g1.w * dy);
+                v[1] = 2.f * v_sigma * __builtin_fmaf(g1.w, dx, g2.x * dy);
+                v[5] = vis * va;
+                v[6] = va * opac * vis * (lg * 0.69314718055994531f);  // ln(1-sigma)
+                // wave totals, scattered over the four 16-lane quarters; one lane
+                // per quarter adds its totals into the primitive's accumulator
+                float tot[NG];
+                wave_totals<NV>(v, tot);
+                if (q_leader) {
+#pragma unroll
+                    for (int r = 0; r < NG; ++r)
+                        if (q_idx[r] >= 0) atomicAdd(&s_acc[q_idx[r]][t], tot[r]);
+                    if (lane == 0) s_hit[t] = 1;
                 }
-#pragma unroll
-                for (int q = 0; q < CDIM; ++q) buf[q] = __builtin_fmaf(s_col[q][t], fac, buf[q]);
-            }
-            // wave totals, scattered over the four 16-lane quarters; one lane
-            // per quarter adds its totals into the primitive's accumulator
-            float tot[NG];
-            wave_totals<NV>(v, tot);
-            if (q_leader) {
-#pragma unroll
-                for (int r = 0; r < NG; ++r)
-                    if (q_idx[r] >= 0) atomicAdd(&s_acc[q_idx[r]][t], tot[r]);
-                if (lane == 0) s_hit[t] = 1;
             }
         }
         __syncthreads();

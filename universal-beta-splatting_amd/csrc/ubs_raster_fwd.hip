@@ -32,7 +32,6 @@ __global__ __launch_bounds__(TS *TS) void raster_fwd_kernel(
     __shared__ float4 s_geo2[BS];      // c, opacity, beta, (unused)
     __shared__ float s_col[CDIM][BS];
     __shared__ uint32_t s_mask[BS];
-    __shared__ uint16_t s_list[NW][BS];
 
     uint32_t logical;
     if (!xcd_tile(blockIdx.x, C * th * tw, logical)) return;
@@ -95,47 +94,36 @@ __global__ __launch_bounds__(TS *TS) void raster_fwd_kernel(
             s_mask[tid] = subblock_mask_exact<SB>(xy.x, xy.y, ca, cb, cc, hx, hy, x0, y0);
         }
         __syncthreads();
-        // ---- per-wave compaction of the batch to this sub-block's primitives
-        int n_mine = 0;
-        for (int c0 = 0; c0 < cnt; c0 += 64) {
+        // ---- front-to-back blend over this sub-block's primitives of the
+        // batch: one ballot per 64 entries, then its set bits in order (the
+        // entry index is wave-uniform, so its LDS reads are broadcasts)
+        bool wave_done = false;
+        for (int c0 = 0; c0 < cnt && !wave_done; c0 += 64) {
             const int e = c0 + lane;
-            const bool hit = e < cnt && ((s_mask[e] >> wave) & 1u);
-            const uint64_t bal = __ballot(hit);
-            if (hit) {
-                const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-                s_list[wave][n_mine + __popcll(bal & lt)] = (uint16_t)e;
-            }
-            n_mine += __popcll(bal);
-        }
-        // (s_list[wave] is written and read by this wave only: LDS is in order)
-        // ---- front-to-back blend over the compacted list
-        // software-pipelined: the next entry's LDS reads are issued before the
-        // current entry is blended
-        int t_nx = n_mine > 0 ? s_list[wave][0] : 0;
-        float4 g1_nx = s_geo[t_nx], g2_nx = s_geo2[t_nx];
-        for (int k = 0; k < n_mine; ++k) {
-            if (__ballot(!done) == 0ull) break;
-            const int t = t_nx;
-            const float4 g1 = g1_nx;
-            const float4 g2 = g2_nx;
-            if (k + 1 < n_mine) {
-                t_nx = s_list[wave][k + 1];
-                g1_nx = s_geo[t_nx];
-                g2_nx = s_geo2[t_nx];
-            }
-            const float dx = g1.x - px, dy = g1.y - py;
-            const float sigma = __builtin_fmaf(g1.w * dx, dy, __builtin_fmaf(g1.z * dx, dx, (g2.x * dy) * dy));
-            if (!done && sigma >= 0.f && sigma < 1.f) {
-                const float alpha = fminf(0.999f, g2.y * beta_pow(1.f - sigma, g2.z));
-                const float nT = T * (1.f - alpha);
-                if (nT <= 1e-4f) {
-                    done = true;
-                } else {
-                    const float vis = alpha * T;
+            uint64_t todo = __ballot(e < cnt && ((s_mask[e] >> wave) & 1u));
+            for (int k = 0; todo != 0ull; ++k) {
+                if ((k & 7) == 0 && __ballot(!done) == 0ull) {  // whole wave done
+                    wave_done = true;
+                    break;
+                }
+                const int t = c0 + __builtin_ctzll(todo);
+                todo &= todo - 1ull;
+                const float4 g1 = s_geo[t];
+                const float4 g2 = s_geo2[t];
+                const float dx = g1.x - px, dy = g1.y - py;
+                const float sigma = __builtin_fmaf(g1.w * dx, dy, __builtin_fmaf(g1.z * dx, dx, (g2.x * dy) * dy));
+                if (!done && sigma >= 0.f && sigma < 1.f) {
+                    const float alpha = fminf(0.999f, g2.y * beta_pow(1.f - sigma, g2.z));
+                    const float nT = T * (1.f - alpha);
+                    if (nT <= 1e-4f) {
+                        done = true;
+                    } else {
+                        const float vis = alpha * T;
 #pragma unroll
-                    for (int q = 0; q < CDIM; ++q) acc[q] = __builtin_fmaf(s_col[q][t], vis, acc[q]);
-                    cur = batch_start + t;
-                    T = nT;
+                        for (int q = 0; q < CDIM; ++q) acc[q] = __builtin_fmaf(s_col[q][t], vis, acc[q]);
+                        cur = batch_start + t;
+                        T = nT;
+                    }
                 }
             }
         }
