@@ -332,3 +332,22 @@ def test_raster_bwd_variants(device, C, channels, ts, bg, mask):
     for a, b, n in zip(got, want, ("v_means2d", "v_conics", "v_colors", "v_opac", "v_betas")):
         assert np.abs(b).max() > 0, n
         assert rel_l2(host(a), b) < 2e-3, (n, rel_l2(host(a), b))
+
+
+def test_wrong_dtypes_raise(device):
+    """fp64 / int64 buffers are rejected before any launch (the reference's
+    data_ptr<float>() / data_ptr<int32_t>() checks), never read as fp32."""
+    C, N = 1, 16
+    f = lambda *s: torch.zeros(*s, device="cuda")
+    off = torch.zeros(C, 2, 2, dtype=torch.int32, device="cuda")
+    flat = torch.zeros(0, dtype=torch.int32, device="cuda")
+    with pytest.raises(RuntimeError, match="expected scalar type"):
+        W.rasterize_to_pixels(f(C, N, 2), f(C, N, 3), f(C, N, 3), f(C, N), f(C, N), 32, 32, 16, off, flat,
+                              backgrounds=f(C, 3).double())
+    with pytest.raises(RuntimeError, match="expected scalar type"):
+        W.rasterize_to_pixels(f(C, N, 2), f(C, N, 3), f(C, N, 3), f(C, N), f(C, N), 32, 32, 16, off.long(), flat)
+    with pytest.raises(RuntimeError, match="expected scalar type"):
+        W.fully_fused_projection(f(N, 3).double(), f(N, 6).double(), None, None, f(1, 4, 4).double(),
+                                 f(1, 3, 3).double(), 32, 32)
+    with pytest.raises(RuntimeError, match="expected scalar type"):
+        W.isect_offset_encode(torch.zeros(4, dtype=torch.int32, device="cuda"), 1, 2, 2)

@@ -10,7 +10,17 @@ import torch
 from gsplat import scenes
 from oracle import ubs_oracle as O
 
-torch.set_default_dtype(torch.float64)
+
+
+@pytest.fixture(autouse=True)
+def _float64_default():
+    """Dense torch references here run in FP64; restore the process default
+    afterwards (a leaked float64 default makes later tests create float64
+    tensors the fp32 kernels reject)."""
+    old = torch.get_default_dtype()
+    torch.set_default_dtype(torch.float64)
+    yield
+    torch.set_default_dtype(old)
 
 
 def dense_L(rot, s, ltri, ri, rj):

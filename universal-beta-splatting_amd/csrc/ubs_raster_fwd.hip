@@ -3,9 +3,10 @@
 // One workgroup per (camera, tile), TS x TS threads.  Each wave64 owns an 8x8
 // pixel sub-block of the tile (lane l -> pixel (l & 7, l >> 3) of it), which
 // is the unit of culling: a batch of TS*TS primitives is staged in LDS, every
-// primitive gets a bit mask of the sub-blocks its sigma < 1 support box
-// reaches, and each wave compacts the batch to the primitives that touch its
-// sub-block (ballot + popcount) before the per-pixel front-to-back loop.
+// primitive gets a bit mask of the sub-blocks its sigma < 1 ellipse reaches
+// (bounding-box test refined by an exact ellipse/rectangle test), and each
+// wave compacts the batch to the primitives that touch its sub-block (ballot
+// + popcount) before the per-pixel front-to-back loop.
 // Pixels stop at the first primitive that would take transmittance to
 // <= 1e-4 (exclusive), as rasterize_to_pixels_fwd.cu:150-172 does; a wave
 // leaves the batch as soon as all of its pixels are done and the workgroup
@@ -32,6 +33,7 @@ __global__ __launch_bounds__(TS *TS) void raster_fwd_kernel(
     __shared__ float4 s_geo2[BS];      // c, opacity, beta, (unused)
     __shared__ float s_col[CDIM][BS];
     __shared__ uint32_t s_mask[BS];
+    __shared__ uint16_t s_list[NW][BS];
 
     uint32_t logical;
     if (!xcd_tile(blockIdx.x, C * th * tw, logical)) return;
@@ -94,38 +96,43 @@ __global__ __launch_bounds__(TS *TS) void raster_fwd_kernel(
             s_mask[tid] = subblock_mask_exact<SB>(xy.x, xy.y, ca, cb, cc, hx, hy, x0, y0);
         }
         __syncthreads();
-        // ---- front-to-back blend over this sub-block's primitives of the
-        // batch: one ballot per 64 entries, then its set bits in order (the
-        // entry index is wave-uniform, so its LDS reads are broadcasts)
-        bool wave_done = false;
-        for (int c0 = 0; c0 < cnt && !wave_done; c0 += 64) {
+        // ---- per-wave compaction of the batch to this sub-block's primitives
+        // (a list in LDS measured faster here than walking ballot bits, whose
+        // scalar bookkeeping competes with this short loop body)
+        int n_mine = 0;
+        for (int c0 = 0; c0 < cnt; c0 += 64) {
             const int e = c0 + lane;
-            uint64_t todo = __ballot(e < cnt && ((s_mask[e] >> wave) & 1u));
-            for (int k = 0; todo != 0ull; ++k) {
-                if ((k & 7) == 0 && __ballot(!done) == 0ull) {  // whole wave done
-                    wave_done = true;
-                    break;
-                }
-                const int t = c0 + __builtin_ctzll(todo);
-                todo &= todo - 1ull;
-                const float4 g1 = s_geo[t];
-                const float4 g2 = s_geo2[t];
-                const float dx = g1.x - px, dy = g1.y - py;
-                const float sigma = __builtin_fmaf(g1.w * dx, dy, __builtin_fmaf(g1.z * dx, dx, (g2.x * dy) * dy));
-                if (!done && sigma >= 0.f && sigma < 1.f) {
-                    const float alpha = fminf(0.999f, g2.y * beta_pow(1.f - sigma, g2.z));
-                    const float nT = T * (1.f - alpha);
-                    if (nT <= 1e-4f) {
-                        done = true;
-                    } else {
-                        const float vis = alpha * T;
-#pragma unroll
-                        for (int q = 0; q < CDIM; ++q) acc[q] = __builtin_fmaf(s_col[q][t], vis, acc[q]);
-                        cur = batch_start + t;
-                        T = nT;
-                    }
-                }
+            const bool hit = e < cnt && ((s_mask[e] >> wave) & 1u);
+            const uint64_t bal = __ballot(hit);
+            if (hit) {
+                const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+                s_list[wave][n_mine + __popcll(bal & lt)] = (uint16_t)e;
             }
+            n_mine += __popcll(bal);
+        }
+        // (s_list[wave] is written and read by this wave only: LDS is in order)
+        // ---- front-to-back blend over the compacted list, branch-free per
+        // pixel: lanes that skip the primitive blend alpha = 0
+        for (int k = 0; k < n_mine; ++k) {
+            if ((k & 7) == 0 && __ballot(!done) == 0ull) break;  // whole wave done
+            const int t = s_list[wave][k];
+            const float4 g1 = s_geo[t];
+            const float4 g2 = s_geo2[t];
+            const float dx = g1.x - px, dy = g1.y - py;
+            const float sigma = __builtin_fmaf(g1.w * dx, dy, __builtin_fmaf(g1.z * dx, dx, (g2.x * dy) * dy));
+            const bool ok = !done && sigma >= 0.f && sigma < 1.f;
+            if (__ballot(ok) == 0ull) continue;
+            const float a0 = fminf(0.999f, g2.y * beta_pow(ok ? 1.f - sigma : 1.f, g2.z));
+            const float alpha = ok ? a0 : 0.f;
+            const float nT = T * (1.f - alpha);
+            const bool stop = ok && nT <= 1e-4f;  // this primitive would end the pixel: excluded
+            const bool take = ok && !stop;
+            done = done || stop;
+            const float vis = take ? alpha * T : 0.f;
+#pragma unroll
+            for (int q = 0; q < CDIM; ++q) acc[q] = __builtin_fmaf(s_col[q][t], vis, acc[q]);
+            cur = take ? batch_start + t : cur;
+            T = take ? nT : T;
         }
     }
 

@@ -118,6 +118,17 @@ def stream_of(t: torch.Tensor):
     return _vp(torch.cuda.current_stream(t.device).cuda_stream)
 
 
+_F32, _I32, _I64 = torch.float32, torch.int32, torch.int64
+
+
+def require_dtype(dtype: torch.dtype, *tensors: Optional[torch.Tensor]) -> None:
+    """The kernels read raw fp32 / int32 / int64 buffers; any other dtype is an
+    error, as the reference's data_ptr<T>() checks make it."""
+    for t in tensors:
+        if t is not None and t.dtype != dtype:
+            raise RuntimeError(f"expected scalar type {dtype} but found {t.dtype}")
+
+
 def require_cuda(*tensors: Optional[torch.Tensor]) -> None:
     for t in tensors:
         if t is None:

@@ -14,7 +14,7 @@ from typing import Optional, Tuple
 import torch
 from torch import Tensor
 
-from ._backend import lib, ptr, require_cuda, stream_of
+from ._backend import _F32, _I32, _I64, lib, ptr, require_cuda, require_dtype, stream_of
 
 
 class LaunchTimer:
@@ -189,6 +189,8 @@ def isect_tiles(
     assert depths.shape == (C, N), depths.size()
     means2d, radii, depths = means2d.contiguous(), radii.contiguous(), depths.contiguous()
     require_cuda(means2d, radii, depths)
+    require_dtype(_F32, means2d, depths)
+    require_dtype(_I32, radii)
     dev = means2d.device
     L = lib()
     st = stream_of(means2d)
@@ -245,6 +247,7 @@ def isect_offset_encode(isect_ids: Tensor, n_cameras: int, tile_width: int, tile
     """Per-(camera, tile) start index [C, th, tw] int32.  reference: _wrapper.py:346-363"""
     isect_ids = isect_ids.contiguous()
     require_cuda(isect_ids)
+    require_dtype(_I64, isect_ids)
     offsets = torch.empty((n_cameras, tile_height, tile_width), dtype=torch.int32, device=isect_ids.device)
     _call("ubs_isect_offset_encode", isect_ids.numel(), ptr(isect_ids), n_cameras, tile_width, tile_height,
           ptr(offsets), stream_of(isect_ids))
@@ -323,6 +326,8 @@ def _mask_ptr(masks: Optional[Tensor]):
 def raster_fwd_raw(means2d, conics, colors, opacities, betas, backgrounds, masks, width, height, tile_size,
                    isect_offsets, flatten_ids):
     require_cuda(means2d, conics, colors, opacities, betas, backgrounds, isect_offsets, flatten_ids)
+    require_dtype(_F32, means2d, conics, colors, opacities, betas, backgrounds)
+    require_dtype(_I32, isect_offsets, flatten_ids)
     C, N = opacities.shape
     channels = colors.shape[-1]
     th, tw = isect_offsets.shape[1:3]
@@ -344,6 +349,9 @@ def raster_bwd_raw(means2d, conics, colors, opacities, betas, backgrounds, masks
                    v_render_alphas):
     require_cuda(means2d, conics, colors, opacities, betas, backgrounds, isect_offsets, flatten_ids,
                  render_alphas, last_ids, T_final, v_render_colors, v_render_alphas)
+    require_dtype(_F32, means2d, conics, colors, opacities, betas, backgrounds, render_alphas, T_final,
+                  v_render_colors, v_render_alphas)
+    require_dtype(_I32, isect_offsets, flatten_ids, last_ids)
     C, N = opacities.shape
     channels = colors.shape[-1]
     th, tw = isect_offsets.shape[1:3]
@@ -364,6 +372,7 @@ def raster_bwd_raw(means2d, conics, colors, opacities, betas, backgrounds, masks
 def cond_fwd_raw(means, covars, opacities, betas, query):
     """cond_mean_convariance_opacity_fwd: (means [N,3], covars [N,3,3], opacities [N,1])."""
     require_cuda(means, covars, opacities, betas, query)
+    require_dtype(_F32, means, covars, opacities, betas, query)
     assert means.dim() == 2, "means must be [N,D]"
     assert covars.dim() == 3, "covars must be [N,D,D]"
     assert opacities.dim() == 2 and opacities.size(1) == 1, "opacities must be [N,1]"
@@ -384,6 +393,7 @@ def cond_fwd_raw(means, covars, opacities, betas, query):
 def cond_bwd_raw(means, covars, opacities, betas, query, v_m, v_V, v_o):
     """cond_mean_convariance_opacity_bwd: (v_means [N,D], v_covars [N,D,D], v_opacities [N,1], v_betas [N,C])."""
     require_cuda(means, covars, opacities, betas, query, v_m, v_V, v_o)
+    require_dtype(_F32, means, covars, opacities, betas, query, v_m, v_V, v_o)
     N, D = means.shape
     v_m, v_V, v_o = v_m.contiguous(), v_V.contiguous(), v_o.contiguous()
     gm = torch.empty_like(means)
@@ -399,6 +409,7 @@ def cond_bwd_raw(means, covars, opacities, betas, query, v_m, v_V, v_o):
 def rotmat_fwd_raw(l_triangle):
     """l_triangle_to_rotmat_fwd: [N,3] -> [N,3,3]."""
     require_cuda(l_triangle)
+    require_dtype(_F32, l_triangle)
     N = l_triangle.size(0)
     R = torch.empty((N, 3, 3), dtype=l_triangle.dtype, device=l_triangle.device)
     _call("ubs_l_triangle_to_rotmat_fwd", N, ptr(l_triangle), ptr(R), stream_of(l_triangle))
@@ -408,6 +419,7 @@ def rotmat_fwd_raw(l_triangle):
 def rotmat_bwd_raw(l_triangle, v_R):
     """l_triangle_to_rotmat_bwd: v_R [N,3,3] -> v_l_triangle [N,3]."""
     require_cuda(l_triangle, v_R)
+    require_dtype(_F32, l_triangle, v_R)
     v_R = v_R.contiguous()
     out = torch.empty_like(l_triangle)
     _call("ubs_l_triangle_to_rotmat_bwd", l_triangle.size(0), ptr(v_R), ptr(out), stream_of(v_R))
@@ -417,6 +429,7 @@ def rotmat_bwd_raw(l_triangle, v_R):
 def covar_fwd_raw(rot, scale, l_triangle, rest_i, rest_j, spatial_block):
     """rot_scale_l_triangle_to_covar_fwd: [N,D,D] (or [N,3,3] for the spatial block)."""
     require_cuda(rot, scale, l_triangle, rest_i, rest_j)
+    require_dtype(_F32, rot, scale, l_triangle)
     N, D = scale.shape
     rest_i = rest_i.to(torch.int32).contiguous()
     rest_j = rest_j.to(torch.int32).contiguous()
@@ -430,6 +443,7 @@ def covar_fwd_raw(rot, scale, l_triangle, rest_i, rest_j, spatial_block):
 def covar_bwd_raw(rot, scale, l_triangle, rest_i, rest_j, spatial_block, v_covar):
     """rot_scale_l_triangle_to_covar_bwd: (v_rot [N,3,3], v_scale [N,D], v_l_triangle [N,D(D-1)/2])."""
     require_cuda(rot, scale, l_triangle, rest_i, rest_j, v_covar)
+    require_dtype(_F32, rot, scale, l_triangle, v_covar)
     N, D = scale.shape
     rest_i = rest_i.to(torch.int32).contiguous()
     rest_j = rest_j.to(torch.int32).contiguous()
@@ -447,6 +461,7 @@ def projection_fwd_raw(means, covars, quats, scales, viewmats, Ks, width, height
                        radius_clip, calc_compensations, ortho, covar_layout=0):
     """fully_fused_projection_fwd: (radii, means2d, depths, conics, compensations or None)."""
     require_cuda(means, covars, quats, scales, viewmats, Ks)
+    require_dtype(_F32, means, covars, quats, scales, viewmats, Ks)
     C, N = viewmats.size(0), means.size(0)
     dev = means.device
     radii = torch.empty((C, N), dtype=torch.int32, device=dev)
@@ -466,6 +481,9 @@ def projection_bwd_raw(means, covars, quats, scales, viewmats, Ks, width, height
     """fully_fused_projection_bwd: (v_means, v_covars, v_quats, v_scales, v_viewmats); the
     entries of the absent covariance parametrisation and v_viewmats when not required are None."""
     require_cuda(means, covars, quats, scales, viewmats, Ks, radii, conics, v_means2d, v_depths, v_conics)
+    require_dtype(_F32, means, covars, quats, scales, viewmats, Ks, conics, compensations, v_means2d, v_depths,
+                  v_conics, v_compensations)
+    require_dtype(_I32, radii)
     C, N = viewmats.size(0), means.size(0)
     v_means = torch.empty_like(means)
     v_covars = torch.empty_like(covars) if covars is not None else None
