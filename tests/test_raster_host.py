@@ -1,4 +1,5 @@
-"""CPU check of the blend kernels' sub-block culling (csrc/ubs_raster.h).
+"""CPU checks of the blend kernels' host-compilable geometry (csrc/ubs_raster.h):
+the sub-block culling and the workgroup -> tile mapping.
 
 The exact ellipse/rectangle test decides which 8x8 pixel sub-blocks a
 primitive is evaluated on.  It must be conservative: no pixel whose fp32
@@ -73,3 +74,51 @@ def test_subblock_culling_is_conservative(tmp_path):
         tested, culled, bad = map(int, out.stdout.split())
         assert bad == 0, f"{bad} accepted pixels in culled sub-blocks"
         assert culled > 0.5 * tested  # the test has teeth: most candidate sub-blocks are culled
+
+
+MAP_DRIVER = r"""
+#include "ubs_raster.h"
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+int main(int argc, char **argv) {
+    const uint32_t C = atoi(argv[1]), tw = atoi(argv[2]), th = atoi(argv[3]);
+    const uint32_t total = C * tw * th, grid = ubs::xcd_grid(C, tw, th);
+    std::vector<int> seen(total, 0), xcd(total, -1);
+    for (uint32_t b = 0; b < grid; ++b) {
+        uint32_t t;
+        if (!ubs::xcd_tile(b, C, tw, th, t)) continue;
+        if (t >= total) { printf("out of range %u\n", t); return 1; }
+        seen[t] += 1;
+        xcd[t] = b % ubs::kNumXcd;
+    }
+    for (uint32_t t = 0; t < total; ++t)
+        if (seen[t] != 1) { printf("tile %u covered %d times\n", t, seen[t]); return 1; }
+    for (uint32_t t = 0; t < total; ++t) printf("%d ", xcd[t]);
+    printf("\n");
+    return 0;
+}
+"""
+
+
+@pytest.mark.skipif(shutil.which("hipcc") is None, reason="hipcc not on PATH")
+def test_workgroup_tile_map_is_a_balanced_bijection(tmp_path):
+    """Every (camera, tile) is blended by exactly one workgroup of the launch
+    grid, and a centre-heavy image spreads evenly over the 8 XCDs."""
+    import numpy as np
+
+    src = tmp_path / "map.cpp"
+    src.write_text(MAP_DRIVER)
+    exe = tmp_path / "map"
+    subprocess.run(["hipcc", "--cuda-host-only", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"),
+                    "-I" + os.path.join(ROOT, "universal-beta-splatting_amd", "csrc"), str(src), "-o", str(exe)],
+                   check=True, capture_output=True, timeout=300)
+    for C, tw, th in ((1, 120, 68), (1, 1, 1), (3, 5, 7), (2, 85, 64), (1, 16, 16), (8, 240, 135)):
+        out = subprocess.run([str(exe), str(C), str(tw), str(th)], capture_output=True, text=True, timeout=300)
+        assert out.returncode == 0, out.stdout
+        if (C, tw, th) == (1, 120, 68):
+            xcd = np.array(out.stdout.split(), dtype=np.int64).reshape(th, tw)
+            yy, xx = np.mgrid[0:th, 0:tw]
+            work = np.exp(-(((xx - tw / 2) / (tw / 5)) ** 2 + ((yy - th / 2) / (th / 5)) ** 2))  # centre-heavy
+            per = np.bincount(xcd.ravel(), weights=work.ravel(), minlength=8)
+            assert per.max() / per.mean() < 1.1, per

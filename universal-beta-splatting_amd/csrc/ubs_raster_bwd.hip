@@ -51,7 +51,7 @@ __global__ __launch_bounds__(TS *TS) void raster_bwd_kernel(
     __shared__ int32_t s_wmax[NW];
 
     uint32_t logical;
-    if (!xcd_tile(blockIdx.x, C * th * tw, logical)) return;
+    if (!xcd_tile(blockIdx.x, C, tw, th, logical)) return;
     const uint32_t cam = logical / (th * tw);
     const uint32_t tile = logical % (th * tw);
     const uint32_t ty = tile / tw, tx = tile % tw;
@@ -224,13 +224,12 @@ static int launch_bwd(bool f64t, uint32_t C, uint32_t N, int64_t n_isects, uint3
                       uint32_t tw, uint32_t th, const int32_t *offs, const int32_t *flat, const float *ra,
                       const int32_t *last, const float *Tf, const float *vrc, const float *vra, float *vm2,
                       float *vcon, float *vcol, float *vop, float *vbe, hipStream_t st) {
-    const uint32_t total = C * th * tw;
     if (f64t)
-        raster_bwd_kernel<TS, CDIM, true><<<xcd_grid(total), TS * TS, 0, st>>>(
+        raster_bwd_kernel<TS, CDIM, true><<<xcd_grid(C, tw, th), TS * TS, 0, st>>>(
             C, N, n_isects, channels, coff, means2d, conics, colors, opac, betas, bg, masks, W, H, tw, th, offs, flat,
             ra, last, Tf, vrc, vra, vm2, vcon, vcol, vop, vbe);
     else
-        raster_bwd_kernel<TS, CDIM, false><<<xcd_grid(total), TS * TS, 0, st>>>(
+        raster_bwd_kernel<TS, CDIM, false><<<xcd_grid(C, tw, th), TS * TS, 0, st>>>(
             C, N, n_isects, channels, coff, means2d, conics, colors, opac, betas, bg, masks, W, H, tw, th, offs, flat,
             ra, last, Tf, vrc, vra, vm2, vcon, vcol, vop, vbe);
     UBS_LAUNCH_CHECK("raster_bwd_kernel");

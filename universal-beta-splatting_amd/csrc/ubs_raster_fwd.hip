@@ -36,7 +36,7 @@ __global__ __launch_bounds__(TS *TS) void raster_fwd_kernel(
     __shared__ uint16_t s_list[NW][BS];
 
     uint32_t logical;
-    if (!xcd_tile(blockIdx.x, C * th * tw, logical)) return;
+    if (!xcd_tile(blockIdx.x, C, tw, th, logical)) return;
     const uint32_t cam = logical / (th * tw);
     const uint32_t tile = logical % (th * tw);
     const uint32_t ty = tile / tw, tx = tile % tw;
@@ -155,8 +155,7 @@ static int launch_fwd(uint32_t C, uint32_t N, int64_t n_isects, uint32_t channel
                       const float *betas, const float *bg, const uint8_t *masks, uint32_t W, uint32_t H,
                       uint32_t tw, uint32_t th, const int32_t *offs, const int32_t *flat, float *rc, float *ra,
                       int32_t *last, float *Tf, hipStream_t st) {
-    const uint32_t total = C * th * tw;
-    raster_fwd_kernel<TS, CDIM><<<xcd_grid(total), TS * TS, 0, st>>>(
+    raster_fwd_kernel<TS, CDIM><<<xcd_grid(C, tw, th), TS * TS, 0, st>>>(
         C, N, n_isects, channels, coff, means2d, conics, colors, opac, betas, bg, masks, W, H, tw, th, offs, flat,
         rc, ra, last, Tf);
     UBS_LAUNCH_CHECK("raster_fwd_kernel");
