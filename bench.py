@@ -71,6 +71,10 @@ def algorithmic_bytes(name: str, c: dict, n_isect: int, C: int = 1) -> float:
         return N * (4.0 * (D + D * D + 1 + 2 * CD) + 52.0)
     if name == "ubs_cond_mean_convariance_opacity_bwd":
         return N * (4.0 * (D + D * D + 1 + 2 * CD) + 52.0 + 4.0 * (D + D * D + 1 + CD))
+    if name == "ubs_beta_activate_fwd":  # leaves + quats in; mean, scale, opacity, beta, query, rot out
+        return N * 4.0 * ((3 + (D - 3) + D + 1 + (D - 2) + 4) + (D + D + 1 + (D - 2) + (D - 3) + 9))
+    if name == "ubs_beta_activate_bwd":  # raw leaves + output grads in; leaf grads out
+        return N * 4.0 * ((D + 1 + (D - 2) + 4) + (D + D + 1 + (D - 2) + 9) + (3 + (D - 3) + D + 1 + (D - 2) + 4))
     if name == "ubs_rot_scale_l_triangle_to_covar_fwd":
         return N * 4.0 * (9 + D + D * (D - 1) // 2 + D * D)
     if name == "ubs_rot_scale_l_triangle_to_covar_bwd":

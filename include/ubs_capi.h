@@ -40,6 +40,34 @@ int ubs_version(void);
 const char *ubs_last_error(void);
 
 /* ------------------------------------------------------------------------
+ * Parameter activations and the view/time query (one pass per primitive).
+ * ------------------------------------------------------------------------ */
+
+/* Leaves xyz [N,3], mean_rest [N,D-3], scale_raw [N,D], opacity_raw [N,1],
+ * beta_raw [N,D-2]; optional quats [N,4] (w,x,y,z); cam_pos device [3].
+ * -> mean [N,D] = cat(xyz, mean_rest), scale [N,D] = softplus(scale_raw)
+ *    (x above 20), opacity [N,1] = sigmoid, beta [N,D-2] = 4 exp(beta_raw),
+ *    query [N,D-3] = normalize(xyz - cam_pos) (, timestamp for D = 7; may be
+ *    NULL, requires D = 6 or 7), rot [N,3,3] = R(quats / |quats|) (may be NULL).
+ * Replaces the torch elementwise ops BetaModel runs before the rasteriser:
+ * scene/beta_model.py:36-52 and :103-121 (activations), :675-690 (query). */
+int ubs_beta_activate_fwd(int64_t N, int64_t D, const float *xyz, const float *mean_rest, const float *scale_raw,
+                          const float *opacity_raw, const float *beta_raw, const float *quats, const float *cam_pos,
+                          float timestamp, float *mean, float *scale, float *opacity, float *beta, float *query,
+                          float *rot, void *stream);
+
+/* Backward of ubs_beta_activate_fwd (the query gets no gradient, as the
+ * conditioning backward drops it).  Any of v_mean, v_scale, v_opacity,
+ * v_beta, v_rot may be NULL (zero); v_quats may be NULL (not computed).
+ * -> v_xyz [N,3], v_mean_rest [N,D-3], v_scale_raw [N,D], v_opacity_raw [N,1],
+ *    v_beta_raw [N,D-2], v_quats [N,4].  Same reference lines as above. */
+int ubs_beta_activate_bwd(int64_t N, int64_t D, const float *scale_raw, const float *opacity_raw,
+                          const float *beta_raw, const float *quats, const float *v_mean, const float *v_scale,
+                          const float *v_opacity, const float *v_beta, const float *v_rot, float *v_xyz,
+                          float *v_mean_rest, float *v_scale_raw, float *v_opacity_raw, float *v_beta_raw,
+                          float *v_quats, void *stream);
+
+/* ------------------------------------------------------------------------
  * Covariance construction.
  * ------------------------------------------------------------------------ */
 

@@ -351,3 +351,54 @@ def test_wrong_dtypes_raise(device):
                                  f(1, 3, 3).double(), 32, 32)
     with pytest.raises(RuntimeError, match="expected scalar type"):
         W.isect_offset_encode(torch.zeros(4, dtype=torch.int32, device="cuda"), 1, 2, 2)
+
+
+# ------------------------------------------------------------- activations
+@pytest.mark.parametrize("D,quats", [(6, True), (7, True), (6, False)])
+def test_beta_activations_vs_torch(device, D, quats):
+    """The fused activation/query/rotation kernel against the torch ops it
+    replaces (BetaModel's getters and render query), forward and backward,
+    both in fp32 on the GPU."""
+    import torch.nn.functional as F
+
+    from gsplat.beta import quat_to_rotmat
+
+    g = torch.Generator(device="cpu").manual_seed(D)
+    N = 5000
+    mk = lambda *s, sc=1.0: (torch.randn(*s, generator=g) * sc).cuda().requires_grad_(True)
+    xyz, rest, sraw = mk(N, 3), mk(N, D - 3), mk(N, D, sc=8.0)  # scale 8: exercises softplus' x > 20 branch
+    oraw, braw = mk(N, 1), mk(N, D - 2, sc=0.5)
+    q = mk(N, 4) if quats else None
+    cam = torch.tensor([0.3, -4.0, 1.0], device="cuda")
+    t = 0.37
+    mean, scale, opac, beta, query, rot = W.beta_activations(xyz, rest, sraw, oraw, braw, q, cam, t)
+    leaves = [xyz, rest, sraw, oraw, braw] + ([q] if quats else [])
+    ref_leaves = [l.detach().clone().requires_grad_(True) for l in leaves]
+    rx, rr, rs, ro, rb = ref_leaves[:5]
+    r_mean = torch.cat([rx, rr], -1)
+    r_scale = F.softplus(rs)
+    r_opac = torch.sigmoid(ro)
+    r_beta = 4.0 * torch.exp(rb)
+    with torch.no_grad():
+        vd = rx - cam[None]
+        r_query = vd / vd.norm(dim=-1, keepdim=True)
+        if D == 7:
+            r_query = torch.cat([r_query, torch.full((N, 1), t, device="cuda")], -1)
+    assert torch.equal(mean, r_mean)
+    for a, b, n in ((scale, r_scale, "scale"), (opac, r_opac, "opacity"), (beta, r_beta, "beta"),
+                    (query, r_query, "query")):
+        torch.testing.assert_close(a, b, rtol=2e-6, atol=1e-7, msg=n)
+    outs, refs = [mean, scale, opac, beta], [r_mean, r_scale, r_opac, r_beta]
+    if quats:
+        r_rot = quat_to_rotmat(ref_leaves[5])
+        torch.testing.assert_close(rot, r_rot, rtol=1e-5, atol=1e-6)  # O(1) entries: a few ulp
+        outs.append(rot)
+        refs.append(r_rot)
+    else:
+        assert rot is None
+    grads = [torch.randn(o.shape, generator=g).cuda() for o in outs]
+    torch.autograd.backward(outs, grads)
+    torch.autograd.backward(refs, grads)
+    for a, b in zip(leaves, ref_leaves):
+        torch.testing.assert_close(a.grad, b.grad, rtol=1e-5, atol=1e-5)
+    assert not query.requires_grad

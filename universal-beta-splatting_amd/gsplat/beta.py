@@ -18,6 +18,7 @@ import torch.nn.functional as F
 from torch import Tensor
 
 from .cuda._wrapper import (
+    beta_activations,
     cond_mean_convariance_opacity,
     l_triangle_to_rotmat,
     rot_scale_l_triangle_to_covar,
@@ -86,30 +87,23 @@ def camera_center(viewmat: Tensor) -> Tensor:
 
 def conditioned(params: BetaParams, viewmat: Tensor, timestamp: Optional[float] = None, rest=None):
     """Activations + query + conditioning: (means [N,3], covars [N,3,3],
-    opacities [N], beta_xyz [N], scale [N,D])."""
+    opacities [N], beta_xyz [N], scale [N,D]).
+
+    The activations, the query and the quaternion rotation run as one kernel
+    (beta_activations); the rotation from l_triangle, the covariance and the
+    conditioning are the reference's ops."""
     D = params.D
-    scale = F.softplus(params.scale_raw)
-    opac = torch.sigmoid(params.opacity_raw)
-    beta = 4.0 * torch.exp(params.beta_raw)
-    if params.quats is not None:
-        rot = quat_to_rotmat(params.quats)
-    else:
+    if D not in (6, 7):
+        raise NotImplementedError("Only implemented for 6D or 7D query")
+    with torch.no_grad():
+        cam_pos = camera_center(viewmat).to(torch.float32).contiguous()
+    mean, scale, opac, beta, query, rot = beta_activations(
+        params.xyz, params.mean_rest, params.scale_raw, params.opacity_raw, params.beta_raw, params.quats,
+        cam_pos, float(timestamp or 0.0))
+    if rot is None:
         rot = l_triangle_to_rotmat(params.l_triangle[:, :3])
     ri, rj = rest if rest is not None else rest_indices(D, params.xyz.device)
     cov = rot_scale_l_triangle_to_covar(rot, scale, params.l_triangle, ri, rj)
-    mean = torch.cat([params.xyz, params.mean_rest], dim=-1)
-    with torch.no_grad():
-        cam_pos = camera_center(viewmat)
-    view_dir = params.xyz - cam_pos[None]
-    view_dir = view_dir / view_dir.norm(dim=-1, keepdim=True)
-    if D == 6:
-        query = view_dir
-    elif D == 7:
-        ts = torch.full((view_dir.shape[0], 1), float(timestamp or 0.0), device=view_dir.device,
-                        dtype=view_dir.dtype)
-        query = torch.cat([view_dir, ts], dim=-1)
-    else:
-        raise NotImplementedError("Only implemented for 6D or 7D query")
     m, V, o = cond_mean_convariance_opacity(mean, cov, opac, beta[:, 1:], query)
     return m, V, o.squeeze(-1), beta[:, 0], scale
 

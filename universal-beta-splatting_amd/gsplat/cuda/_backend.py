@@ -29,6 +29,10 @@ _vp = ctypes.c_void_p
 _SIGNATURES = {
     "ubs_version": (_c_int, []),
     "ubs_last_error": (ctypes.c_char_p, []),
+    "ubs_beta_activate_fwd": (
+        _c_int, [_c_i64, _c_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_f32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "ubs_beta_activate_bwd": (
+        _c_int, [_c_i64, _c_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "ubs_l_triangle_to_rotmat_fwd": (_c_int, [_c_i64, _vp, _vp, _vp]),
     "ubs_l_triangle_to_rotmat_bwd": (_c_int, [_c_i64, _vp, _vp, _vp]),
     "ubs_rot_scale_l_triangle_to_covar_fwd": (

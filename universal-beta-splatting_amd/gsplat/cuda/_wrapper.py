@@ -86,6 +86,29 @@ def l_triangle_to_rotmat(l_triangle: Tensor) -> Tensor:
     return _LTriangleToRotmat.apply(l_triangle.contiguous())
 
 
+def beta_activations(
+    xyz: Tensor,  # [N, 3]
+    mean_rest: Tensor,  # [N, D-3]
+    scale_raw: Tensor,  # [N, D]
+    opacity_raw: Tensor,  # [N, 1]
+    beta_raw: Tensor,  # [N, D-2]
+    quats: Optional[Tensor],  # [N, 4] or None
+    cam_pos: Optional[Tensor],  # [3] or None (no query)
+    timestamp: float = 0.0,
+) -> Tuple[Tensor, Tensor, Tensor, Tensor, Optional[Tensor], Optional[Tensor]]:
+    """BetaModel's activations, view/time query and quaternion rotation in one
+    kernel: (mean [N,D], scale [N,D], opacity [N,1], beta [N,D-2],
+    query [N,D-3] or None, rot [N,3,3] or None).  The query has no gradient.
+    reference: scene/beta_model.py:36-52,103-121 (activations), :675-690 (query)"""
+    D = scale_raw.shape[1]
+    if cam_pos is not None and D not in (6, 7):
+        raise NotImplementedError("Only implemented for 6D or 7D query")
+    return _BetaActivations.apply(xyz.contiguous(), mean_rest.contiguous(), scale_raw.contiguous(),
+                                  opacity_raw.contiguous(), beta_raw.contiguous(),
+                                  None if quats is None else quats.contiguous(),
+                                  None if cam_pos is None else cam_pos.contiguous(), float(timestamp))
+
+
 def rot_scale_l_triangle_to_covar(
     rot: Tensor,  # [N, 3, 3]
     scale: Tensor,  # [N, D]
@@ -369,6 +392,47 @@ def raster_bwd_raw(means2d, conics, colors, opacities, betas, backgrounds, masks
     return v_means2d, v_conics, v_colors, v_opacities, v_betas
 
 
+def activate_fwd_raw(xyz, mean_rest, scale_raw, opacity_raw, beta_raw, quats, cam_pos, timestamp):
+    """ubs_beta_activate_fwd: (mean, scale, opacity, beta, query or None, rot or None)."""
+    require_cuda(xyz, mean_rest, scale_raw, opacity_raw, beta_raw, quats, cam_pos)
+    require_dtype(_F32, xyz, mean_rest, scale_raw, opacity_raw, beta_raw, quats, cam_pos)
+    N, D = scale_raw.shape
+    assert xyz.shape == (N, 3) and mean_rest.shape == (N, D - 3), (xyz.shape, mean_rest.shape)
+    assert opacity_raw.shape == (N, 1) and beta_raw.shape == (N, D - 2), (opacity_raw.shape, beta_raw.shape)
+    assert quats is None or quats.shape == (N, 4), quats.shape
+    assert cam_pos is None or cam_pos.numel() == 3
+    dev, dt = xyz.device, xyz.dtype
+    mean = torch.empty((N, D), dtype=dt, device=dev)
+    scale = torch.empty((N, D), dtype=dt, device=dev)
+    opacity = torch.empty((N, 1), dtype=dt, device=dev)
+    beta = torch.empty((N, D - 2), dtype=dt, device=dev)
+    query = torch.empty((N, D - 3), dtype=dt, device=dev) if cam_pos is not None else None
+    rot = torch.empty((N, 3, 3), dtype=dt, device=dev) if quats is not None else None
+    _call("ubs_beta_activate_fwd", N, D, ptr(xyz), ptr(mean_rest), ptr(scale_raw), ptr(opacity_raw),
+          ptr(beta_raw), ptr(quats), ptr(cam_pos), float(timestamp), ptr(mean), ptr(scale), ptr(opacity),
+          ptr(beta), ptr(query), ptr(rot), stream_of(xyz))
+    return mean, scale, opacity, beta, query, rot
+
+
+def activate_bwd_raw(scale_raw, opacity_raw, beta_raw, quats, v_mean, v_scale, v_opacity, v_beta, v_rot):
+    """ubs_beta_activate_bwd: (v_xyz, v_mean_rest, v_scale_raw, v_opacity_raw, v_beta_raw, v_quats or None)."""
+    grads = [None if g is None else g.contiguous() for g in (v_mean, v_scale, v_opacity, v_beta, v_rot)]
+    require_cuda(scale_raw, opacity_raw, beta_raw, quats, *grads)
+    require_dtype(_F32, scale_raw, opacity_raw, beta_raw, quats, *grads)
+    N, D = scale_raw.shape
+    dev, dt = scale_raw.device, scale_raw.dtype
+    v_xyz = torch.empty((N, 3), dtype=dt, device=dev)
+    v_mean_rest = torch.empty((N, D - 3), dtype=dt, device=dev)
+    v_scale_raw = torch.empty_like(scale_raw)
+    v_opacity_raw = torch.empty_like(opacity_raw)
+    v_beta_raw = torch.empty_like(beta_raw)
+    v_quats = torch.empty_like(quats) if quats is not None else None
+    _call("ubs_beta_activate_bwd", N, D, ptr(scale_raw), ptr(opacity_raw), ptr(beta_raw), ptr(quats),
+          *[ptr(g) for g in grads], ptr(v_xyz), ptr(v_mean_rest), ptr(v_scale_raw), ptr(v_opacity_raw),
+          ptr(v_beta_raw), ptr(v_quats), stream_of(scale_raw))
+    return v_xyz, v_mean_rest, v_scale_raw, v_opacity_raw, v_beta_raw, v_quats
+
+
 def cond_fwd_raw(means, covars, opacities, betas, query):
     """cond_mean_convariance_opacity_fwd: (means [N,3], covars [N,3,3], opacities [N,1])."""
     require_cuda(means, covars, opacities, betas, query)
@@ -506,6 +570,26 @@ def projection_bwd_raw(means, covars, quats, scales, viewmats, Ks, width, height
 # ----------------------------------------------------------------------------
 # autograd Functions (reference: _wrapper.py:573-684, 807-1050)
 # ----------------------------------------------------------------------------
+class _BetaActivations(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, xyz, mean_rest, scale_raw, opacity_raw, beta_raw, quats, cam_pos, timestamp):
+        mean, scale, opacity, beta, query, rot = activate_fwd_raw(
+            xyz, mean_rest, scale_raw, opacity_raw, beta_raw, quats, cam_pos, timestamp)
+        ctx.save_for_backward(scale_raw, opacity_raw, beta_raw, quats)
+        if query is not None:
+            ctx.mark_non_differentiable(query)
+        return mean, scale, opacity, beta, query, rot
+
+    @staticmethod
+    def backward(ctx, v_mean, v_scale, v_opacity, v_beta, v_query, v_rot):
+        scale_raw, opacity_raw, beta_raw, quats = ctx.saved_tensors
+        if quats is not None and not ctx.needs_input_grad[5]:
+            quats = None
+        gx, gm, gs, go, gb, gq = activate_bwd_raw(scale_raw, opacity_raw, beta_raw, quats, v_mean, v_scale,
+                                                  v_opacity, v_beta, v_rot)
+        return gx, gm, gs, go, gb, gq, None, None
+
+
 class _CondMeanConvarianceOpacity(torch.autograd.Function):
     @staticmethod
     def forward(ctx, means, covars, opacities, betas, query):
