@@ -82,6 +82,22 @@ def algorithmic_bytes(name: str, c: dict, n_isect: int, C: int = 1) -> float:
     return float("nan")
 
 
+def measured_counters(entry: str, workload: str, n_isect: int):
+    """(HBM bytes per call, VALU-busy share of the entry's main kernel) from the
+    committed rocprofv3 --pmc summary of this same workload, or (None, None)
+    when there is none for it."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "pmc_traffic.json")) as f:
+            d = json.load(f)
+    except (OSError, ValueError):
+        return None, None
+    if d.get("workload") != workload or d.get("n_isects") != n_isect or entry not in d.get("entries", {}):
+        return None, None
+    e = d["entries"][entry]
+    busy = d.get("valu", {}).get(e["kernels"][0], {}).get("valu_busy") if e.get("kernels") else None
+    return e["traffic_bytes"], busy
+
+
 def cpu_baseline(cfg: dict, seed: int = 0):
     """The oracle's full step (float32, OpenMP on the host cores) on the same
     workload, one frame."""
@@ -216,9 +232,14 @@ def main():
     avg_ms = per_call[dominant][0]
     alg = algorithmic_bytes(dominant, cfg, n_isect)
     achieved = alg / (avg_ms * 1e-3) / 1e9
+    traffic, valu_busy = measured_counters(dominant, cfg["workload"], n_isect)
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": alg}
+    if traffic is not None:
+        roofline["traffic_source"] = ("profiles/pmc_traffic.json: rocprofv3 FETCH_SIZE and WRITE_SIZE of this "
+                                      "workload, scaled against a 1 GiB device copy")
+        roofline["valu_busy"] = valu_busy  # the blend is VALU-bound, not HBM-bound
     stages = {k.replace("ubs_", ""): {"avg_ms": round(v[0], 4), "per_step": v[1],
                                       "GB_per_s": round(algorithmic_bytes(k, cfg, n_isect) / (v[0] * 1e-3) / 1e9, 1)}
               for k, v in sorted(per_call.items(), key=lambda kv: -kv[1][0] * kv[1][1])}

@@ -41,6 +41,16 @@ def test_every_declared_symbol_has_a_ctypes_signature():
     assert not missing, missing
 
 
+def test_ctypes_signatures_match_header_arity():
+    """Every ctypes binding passes exactly the parameters the C declaration has."""
+    from gsplat.cuda._backend import _SIGNATURES
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    for name, params in re.findall(r"\b(ubs_[a-z0-9_]+)\s*\(([^)]*)\)\s*;", text):
+        params = params.strip()
+        n = 0 if params in ("", "void") else params.count(",") + 1
+        assert len(_SIGNATURES[name][1]) == n, (name, len(_SIGNATURES[name][1]), n)
+
+
 def test_argument_validation_without_gpu(L):
     sel = ctypes.c_int(0)
     rc = L.ubs_radix_sort_pairs(10, None, None, None, None, 10, 5, None, 0, ctypes.byref(sel), None)
@@ -54,10 +64,11 @@ def test_argument_validation_without_gpu(L):
     assert rc == 1 and "D must be in" in L.last_error()
     rc = L.ubs_cond_mean_convariance_opacity_fwd(10, 3, *([None] * 8), None)
     assert rc == 1
-    rc = L.ubs_rasterize_to_pixels_fwd(1, 10, 0, 3, *([None] * 7), 64, 64, 12, 4, 4, *([None] * 6), None)
+    rc = L.ubs_rasterize_to_pixels_fwd(1, 10, 0, 3, *([None] * 7), 64, 64, 12, 4, 4, *([None] * 6), None, 0, None)
     assert rc == 1 and "tile_size" in L.last_error()
-    rc = L.ubs_rasterize_to_pixels_fwd(1, 10, 0, 3, *([None] * 7), 64, 64, 16, 2, 4, *([None] * 6), None)
+    rc = L.ubs_rasterize_to_pixels_fwd(1, 10, 0, 3, *([None] * 7), 64, 64, 16, 2, 4, *([None] * 6), None, 0, None)
     assert rc == 1 and "cover" in L.last_error()
+    assert L.ubs_rasterize_workspace_bytes(2, 1000, 1) == 2 * L.ubs_rasterize_workspace_bytes(2, 1000, 0) >= 64000
     rc = L.ubs_fully_fused_projection_fwd(1, 10, None, None, 7, *([None] * 4), 64, 64, 0.3, 0.01, 1e10, 0.0, 0,
                                           *([None] * 5), None)
     assert rc == 1

@@ -402,3 +402,31 @@ def test_beta_activations_vs_torch(device, D, quats):
     for a, b in zip(leaves, ref_leaves):
         torch.testing.assert_close(a.grad, b.grad, rtol=1e-5, atol=1e-5)
     assert not query.requires_grad
+
+
+@pytest.mark.parametrize("C,channels,ts", [(2, 3, 16), (1, 35, 8), (1, 4, 32)])
+def test_blend_packed_records_match_unpacked(device, C, channels, ts):
+    """The packed-record gathers (default) against the four-array gathers."""
+    W_, H_ = 136, 104
+    x = _raster_inputs(lambda **k: opaque_scene(1200, opacity=(0.2, 0.95), **k), W_, H_, C=C, ts=ts,
+                       channels=channels)
+    ins = [cu(x[k]) for k in ("m2", "cn", "col", "op", "be")]
+    g = np.random.Generator(np.random.PCG64(7))
+    bgs = cu(g.uniform(0, 1, (C, channels)).astype(np.float32))
+    res = {}
+    try:
+        for packed in (False, True):
+            W.set_blend_packed_records(packed)
+            fwd = W.raster_fwd_raw(*ins, bgs, None, W_, H_, ts, cu(x["off"]), cu(x["flat"]))
+            vrc = torch.from_numpy(np.random.Generator(np.random.PCG64(8)).normal(0, 1, fwd[0].shape).astype(
+                np.float32)).cuda()
+            vra = torch.ones_like(fwd[1])
+            bwd = W.raster_bwd_raw(*ins, bgs, None, W_, H_, ts, cu(x["off"]), cu(x["flat"]), fwd[1], fwd[2], fwd[3],
+                                   vrc, vra)
+            res[packed] = (fwd, bwd)
+    finally:
+        W.set_blend_packed_records(True)
+    for a, b in zip(res[False][0], res[True][0]):
+        assert torch.equal(a, b)
+    for a, b, n in zip(res[False][1], res[True][1], ("v_means2d", "v_conics", "v_colors", "v_opac", "v_betas")):
+        assert rel_l2(host(b), host(a).astype(np.float64)) < 1e-5, n

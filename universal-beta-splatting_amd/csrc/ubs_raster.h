@@ -149,6 +149,41 @@ __device__ __forceinline__ uint32_t subblock_mask_exact(float mx, float my, floa
     return out;
 }
 
+// Packed per-(camera, primitive) records of the blend: r0 = (x, y, a, b),
+// r1 = (c, opacity, beta, 0) for the forward inputs, and the same layout for
+// the backward's gradients (v_x, v_y, v_a, v_b | v_c, v_opacity, v_beta, 0).
+// One 32-byte record per intersection gather instead of four arrays.
+constexpr size_t kRecordFloats = 8;
+
+inline size_t raster_workspace_bytes(uint32_t C, uint32_t N, int backward) {
+    const size_t one = ((size_t)C * N * kRecordFloats * sizeof(float) + 255) & ~(size_t)255;
+    return backward ? 2 * one : one;
+}
+
+static __global__ void raster_pack_kernel(uint64_t CN, const float *__restrict__ means2d,
+                                          const float *__restrict__ conics, const float *__restrict__ opacities,
+                                          const float *__restrict__ betas, float4 *__restrict__ rec) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= CN) return;
+    const float2 xy = reinterpret_cast<const float2 *>(means2d)[i];
+    rec[2 * i] = make_float4(xy.x, xy.y, conics[3 * i], conics[3 * i + 1]);
+    rec[2 * i + 1] = make_float4(conics[3 * i + 2], opacities[i], betas[i], 0.f);
+}
+
+static __global__ void raster_unpack_grads_kernel(uint64_t CN, const float4 *__restrict__ vrec,
+                                                  float *__restrict__ v_means2d, float *__restrict__ v_conics,
+                                                  float *__restrict__ v_opacities, float *__restrict__ v_betas) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= CN) return;
+    const float4 r0 = vrec[2 * i], r1 = vrec[2 * i + 1];
+    reinterpret_cast<float2 *>(v_means2d)[i] = make_float2(r0.x, r0.y);
+    v_conics[3 * i] = r0.z;
+    v_conics[3 * i + 1] = r0.w;
+    v_conics[3 * i + 2] = r1.x;
+    v_opacities[i] = r1.y;
+    v_betas[i] = r1.z;
+}
+
 // Channel chunks the kernels are instantiated for; wider colour vectors are
 // blended chunk by chunk (the transmittance does not depend on colour).  The
 // widest chunk is bounded by LDS: 32 channels for 8x8 and 16x16 tiles, 4 for

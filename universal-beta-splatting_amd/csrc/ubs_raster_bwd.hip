@@ -36,18 +36,18 @@ __global__ __launch_bounds__(TS *TS) void raster_bwd_kernel(
     const float *__restrict__ render_alphas, const int32_t *__restrict__ last_ids, const float *__restrict__ T_final,
     const float *__restrict__ v_render_colors, const float *__restrict__ v_render_alphas,
     float *__restrict__ v_means2d, float *__restrict__ v_conics, float *__restrict__ v_colors,
-    float *__restrict__ v_opacities, float *__restrict__ v_betas) {
+    float *__restrict__ v_opacities, float *__restrict__ v_betas, const float4 *__restrict__ rec,
+    float *__restrict__ vrec) {
     constexpr int BS = TS * TS;
     constexpr int SB = TS / 8;
     constexpr int NW = BS / 64;
     constexpr int NV = 7 + CDIM;  // xy(2) conic(3) opacity beta colour(CDIM)
-    constexpr int NG = WaveTotals<NV>::kGroups;
+    constexpr int NG2 = WaveTotals<2 * NV>::kGroups;
     __shared__ float4 s_geo[BS];   // x, y, a, b
     __shared__ float4 s_geo2[BS];  // c, opacity, beta, id (bits)
     __shared__ float s_col[CDIM][BS];
     __shared__ uint32_t s_mask[BS];
     __shared__ float s_acc[NV][BS + 1];  // +1: the four quarter lanes hit distinct banks
-    __shared__ uint8_t s_hit[BS];
     __shared__ int32_t s_wmax[NW];
 
     uint32_t logical;
@@ -93,14 +93,19 @@ __global__ __launch_bounds__(TS *TS) void raster_bwd_kernel(
         if (lane == 0) s_wmax[wave] = wm;
     }
     for (int k = tid; k < NV * (BS + 1); k += BS) (&s_acc[0][0])[k] = 0.f;
-    if (tid < BS) s_hit[tid] = 0;
     __syncthreads();
     const int32_t wave_bin_max = s_wmax[wave];
     // the value each quarter leader (lanes 0, 16, 32, 48) adds per group
-    const bool q_leader = (lane & 15) == 0;
-    int q_idx[NG];
+    // after wave_totals every lane of quarter Q holds the totals of all groups;
+    // lane 16 Q + r adds group 16 p + r's in pass p, so ceil(NG2 / 16)
+    // ds_add_f32 (one for up to 14 colour channels) store the 2 NV totals of a
+    // pair of primitives
+    constexpr int NPASS = (NG2 + 15) / 16;
+    const int my_r = lane & 15;
+    int my_idx[NPASS];
 #pragma unroll
-    for (int r = 0; r < NG; ++r) q_idx[r] = wave_total_index<NV>(r, lane >> 4);
+    for (int p = 0; p < NPASS; ++p)
+        my_idx[p] = 16 * p + my_r < NG2 ? wave_total_index<2 * NV>(16 * p + my_r, lane >> 4) : -1;
 
     const int32_t nb = range_end > range_start ? (range_end - range_start + BS - 1) / BS : 0;
     for (int32_t b = 0; b < nb; ++b) {
@@ -108,10 +113,19 @@ __global__ __launch_bounds__(TS *TS) void raster_bwd_kernel(
         const int32_t cnt = min(BS, batch_end + 1 - range_start);
         if (tid < cnt) {
             const int32_t g = flatten_ids[batch_end - tid];
-            const float2 xy = reinterpret_cast<const float2 *>(means2d)[g];
-            const float ca = conics[(int64_t)g * 3], cb = conics[(int64_t)g * 3 + 1], cc = conics[(int64_t)g * 3 + 2];
+            float2 xy;
+            float ca, cb, cc, op, be;
+            if (rec) {  // one 32-byte record per (camera, primitive)
+                const float4 r0 = rec[2 * (int64_t)g], r1 = rec[2 * (int64_t)g + 1];
+                xy = make_float2(r0.x, r0.y);
+                ca = r0.z; cb = r0.w; cc = r1.x; op = r1.y; be = r1.z;
+            } else {
+                xy = reinterpret_cast<const float2 *>(means2d)[g];
+                ca = conics[(int64_t)g * 3]; cb = conics[(int64_t)g * 3 + 1]; cc = conics[(int64_t)g * 3 + 2];
+                op = opacities[g]; be = betas[g];
+            }
             s_geo[tid] = make_float4(xy.x, xy.y, ca, cb);
-            s_geo2[tid] = make_float4(cc, opacities[g], betas[g], __int_as_float(g));
+            s_geo2[tid] = make_float4(cc, op, be, __int_as_float(g));
             const float *cp = colors + (int64_t)g * channels + coff;
 #pragma unroll
             for (int k = 0; k < CDIM; ++k) s_col[k][tid] = cp[k];
@@ -124,76 +138,101 @@ __global__ __launch_bounds__(TS *TS) void raster_bwd_kernel(
         // whose isect index is at most the wave's largest last contributor:
         // one ballot per 64 entries, then its set bits in order (the entry
         // index is wave-uniform, so its LDS reads are broadcasts)
+        // entry t is at or before this pixel's last contributor iff t >= t_min
+        const int32_t t_min = inside ? batch_end - bin_final : INT32_MAX;
+        // Gradient terms of this pixel for batch entry t into v[o, o + NV);
+        // advances T and bufv.  Branch-free: lanes without a contribution run
+        // the same instructions with alpha = 0, (1 - sigma) = 1 and a zero
+        // v_alpha, so they produce exact zeros and leave T and bufv unchanged.
+        // Returns whether any lane of the wave contributes.
+        auto pixel_grads = [&](const int t, float (&v)[2 * NV], const int o) -> bool {
+            const float4 g1 = s_geo[t];
+            const float4 g2 = s_geo2[t];
+            const float dx = g1.x - px, dy = g1.y - py;
+            const float sigma = __builtin_fmaf(2.f * g1.w * dx, dy, __builtin_fmaf(g1.z * dx, dx, (g2.x * dy) * dy));
+            const bool valid = t >= t_min && sigma >= 0.f && sigma < 1.f;
+            const float opac = g2.y, beta = g2.z;
+            const float oms = valid ? 1.f - sigma : 1.f;
+            const float lg = __builtin_amdgcn_logf(oms);  // log2(1 - sigma)
+            const float vis = __builtin_amdgcn_exp2f(beta * lg);
+            const float ov = opac * vis;
+            const float alpha = valid ? fminf(0.999f, ov) : 0.f;
+            float ra;
+            if constexpr (F64T) {
+                const double rad = 1.0 / (1.0 - (double)alpha);
+                T = valid ? T * rad : T;
+                ra = (float)rad;
+            } else {
+                ra = __builtin_amdgcn_rcpf(1.f - alpha);
+                T = valid ? T * ra : T;
+            }
+            const float Tc = (float)T;
+            const float fac = alpha * Tc;
+            // sum_k (c_k T - buf_k ra) v_k = T (c . v) - ra (buf . v), with
+            // bufv = buf . v carried as one running sum
+            float cdot = 0.f;
+#pragma unroll
+            for (int q = 0; q < CDIM; ++q) {
+                v[o + 7 + q] = fac * vrc[q];
+                cdot = __builtin_fmaf(s_col[q][t], vrc[q], cdot);
+            }
+            float v_alpha = __builtin_fmaf(Tc, cdot, -(ra * bufv));
+            v_alpha = __builtin_fmaf(Tf * ra, vra_bg, v_alpha);
+            bufv = __builtin_fmaf(cdot, fac, bufv);
+            // geometry, opacity and beta get gradient only below the 0.999 clamp
+            const float va = (valid && ov <= 0.999f) ? v_alpha : 0.f;
+            const float pm1 = __builtin_amdgcn_exp2f((beta - 1.f) * lg);  // (1-sigma)^(beta-1)
+            // v_sigma = -opac beta va pm1; the per-primitive factors (-opac beta,
+            // the 2 of the symmetric terms, opac ln 2) are applied once per
+            // (tile, primitive) at the flush, not per pixel
+            const float sg = va * pm1;
+            const float u = dx * sg, w = dy * sg;
+            v[o + 2] = u * dx;                                // x -opac beta
+            v[o + 3] = u * dy;                                // x -2 opac beta
+            v[o + 4] = w * dy;                                // x -opac beta
+            v[o + 0] = __builtin_fmaf(g1.z, u, g1.w * w);     // x -2 opac beta
+            v[o + 1] = __builtin_fmaf(g1.w, u, g2.x * w);     // x -2 opac beta
+            v[o + 5] = vis * va;
+            v[o + 6] = v[o + 5] * lg;                         // x opac ln 2 (lg = log2(1 - sigma))
+            return __ballot(valid) != 0ull;
+        };
         for (int c0 = 0; c0 < cnt; c0 += 64) {
             const int e = c0 + lane;
             uint64_t todo = __ballot(e < cnt && ((s_mask[e] >> wave) & 1u) && (batch_end - e) <= wave_bin_max);
+            // two primitives per step: their gradient chains interleave and one
+            // reduce-scatter of 2 NV values serves both
             while (todo != 0ull) {
-                const int t = c0 + __builtin_ctzll(todo);
+                const int t0 = c0 + __builtin_ctzll(todo);
                 todo &= todo - 1ull;
-                const float4 g1 = s_geo[t];
-                const float4 g2 = s_geo2[t];
-                const float dx = g1.x - px, dy = g1.y - py;
-                const float sigma = __builtin_fmaf(2.f * g1.w * dx, dy, __builtin_fmaf(g1.z * dx, dx, (g2.x * dy) * dy));
-                const bool valid = inside && (batch_end - t) <= bin_final && sigma >= 0.f && sigma < 1.f;
-                if (__ballot(valid) == 0ull) continue;
-                // Branch-free: lanes without a contribution run the same
-                // instructions with alpha = 0, (1 - sigma) = 1 and a zero v_alpha,
-                // so they add exact zeros and leave T and the colour sum unchanged.
-                float v[NV];
-                const float opac = g2.y, beta = g2.z;
-                const float oms = valid ? 1.f - sigma : 1.f;
-                const float lg = __builtin_amdgcn_logf(oms);  // log2(1 - sigma)
-                const float vis = __builtin_amdgcn_exp2f(beta * lg);
-                const float ov = opac * vis;
-                const float alpha = valid ? fminf(0.999f, ov) : 0.f;
-                float ra;
-                if constexpr (F64T) {
-                    const double rad = 1.0 / (1.0 - (double)alpha);
-                    T = valid ? T * rad : T;
-                    ra = (float)rad;
+                const bool two = todo != 0ull;
+                const int t1 = two ? c0 + __builtin_ctzll(todo) : t0;
+                if (two) todo &= todo - 1ull;
+                float v[2 * NV];
+                bool any = pixel_grads(t0, v, 0);
+                if (two) {
+                    any = pixel_grads(t1, v, NV) || any;
                 } else {
-                    ra = __builtin_amdgcn_rcpf(1.f - alpha);
-                    T = valid ? T * ra : T;
-                }
-                const float Tc = (float)T;
-                const float fac = alpha * Tc;
-                // sum_k (c_k T - buf_k ra) v_k = T (c . v) - ra (buf . v), with
-                // bufv = buf . v carried as one running sum
-                float cdot = 0.f;
 #pragma unroll
-                for (int q = 0; q < CDIM; ++q) {
-                    v[7 + q] = fac * vrc[q];
-                    cdot = __builtin_fmaf(s_col[q][t], vrc[q], cdot);
+                    for (int q = 0; q < NV; ++q) v[NV + q] = 0.f;
                 }
-                float v_alpha = __builtin_fmaf(Tc, cdot, -(ra * bufv));
-                v_alpha = __builtin_fmaf(Tf * ra, vra_bg, v_alpha);
-                bufv = __builtin_fmaf(cdot, fac, bufv);
-                // geometry, opacity and beta get gradient only below the 0.999 clamp
-                const float va = (valid && ov <= 0.999f) ? v_alpha : 0.f;
-                const float pm1 = __builtin_amdgcn_exp2f((beta - 1.f) * lg);  // (1-sigma)^(beta-1)
-                const float v_sigma = -va * opac * beta * pm1;
-                v[2] = dx * dx * v_sigma;
-                v[3] = 2.f * dx * dy * v_sigma;
-                v[4] = dy * dy * v_sigma;
-                v[0] = 2.f * v_sigma * __builtin_fmaf(g1.z, dx, g1.w * dy);
-                v[1] = 2.f * v_sigma * __builtin_fmaf(g1.w, dx, g2.x * dy);
-                v[5] = vis * va;
-                v[6] = va * opac * vis * (lg * 0.69314718055994531f);  // ln(1-sigma)
-                // wave totals, scattered over the four 16-lane quarters; one lane
-                // per quarter adds its totals into the primitive's accumulator
-                float tot[NG];
-                wave_totals<NV>(v, tot);
-                if (q_leader) {
+                if (!any) continue;
+                // wave totals, scattered over the four 16-lane quarters, added into
+                // the primitives' accumulators by one ds_add_f32
+                float tot[NG2];
+                wave_totals<2 * NV>(v, tot);
 #pragma unroll
-                    for (int r = 0; r < NG; ++r)
-                        if (q_idx[r] >= 0) atomicAdd(&s_acc[q_idx[r]][t], tot[r]);
-                    if (lane == 0) s_hit[t] = 1;
+                for (int p = 0; p < NPASS; ++p) {
+                    float mine = tot[16 * p];
+#pragma unroll
+                    for (int r = 16 * p + 1; r < NG2 && r < 16 * p + 16; ++r) mine = my_r == r - 16 * p ? tot[r] : mine;
+                    const int idx = my_idx[p];
+                    if (idx >= 0) atomicAdd(&s_acc[idx % NV][idx < NV ? t0 : t1], mine);
                 }
             }
         }
         __syncthreads();
         // ---- flush this batch's accumulators to the per-primitive gradients
-        if (tid < cnt && s_hit[tid]) {
+        if (tid < cnt) {
             const int32_t g = __float_as_int(s_geo2[tid].w);
             float a[NV];
 #pragma unroll
@@ -201,14 +240,30 @@ __global__ __launch_bounds__(TS *TS) void raster_bwd_kernel(
                 a[q] = s_acc[q][tid];
                 s_acc[q][tid] = 0.f;
             }
-            s_hit[tid] = 0;
-            if (a[0] != 0.f) atomic_add_f32(v_means2d + (int64_t)g * 2, a[0]);
-            if (a[1] != 0.f) atomic_add_f32(v_means2d + (int64_t)g * 2 + 1, a[1]);
-            if (a[2] != 0.f) atomic_add_f32(v_conics + (int64_t)g * 3, a[2]);
-            if (a[3] != 0.f) atomic_add_f32(v_conics + (int64_t)g * 3 + 1, a[3]);
-            if (a[4] != 0.f) atomic_add_f32(v_conics + (int64_t)g * 3 + 2, a[4]);
-            if (a[5] != 0.f) atomic_add_f32(v_opacities + g, a[5]);
-            if (a[6] != 0.f) atomic_add_f32(v_betas + g, a[6]);
+            {
+                const float4 gp = s_geo2[tid];  // c, opacity, beta, id
+                const float k1 = -gp.y * gp.z, k2 = 2.f * k1;
+                a[0] *= k2;
+                a[1] *= k2;
+                a[2] *= k1;
+                a[3] *= k2;
+                a[4] *= k1;
+                a[6] *= gp.y * 0.69314718055994531f;
+            }
+            if (vrec) {  // the primitive's 7 scalar gradients share one 32-byte record
+                float *r = vrec + (int64_t)g * kRecordFloats;
+#pragma unroll
+                for (int q = 0; q < 7; ++q)
+                    if (a[q] != 0.f) atomic_add_f32(r + q, a[q]);
+            } else {
+                if (a[0] != 0.f) atomic_add_f32(v_means2d + (int64_t)g * 2, a[0]);
+                if (a[1] != 0.f) atomic_add_f32(v_means2d + (int64_t)g * 2 + 1, a[1]);
+                if (a[2] != 0.f) atomic_add_f32(v_conics + (int64_t)g * 3, a[2]);
+                if (a[3] != 0.f) atomic_add_f32(v_conics + (int64_t)g * 3 + 1, a[3]);
+                if (a[4] != 0.f) atomic_add_f32(v_conics + (int64_t)g * 3 + 2, a[4]);
+                if (a[5] != 0.f) atomic_add_f32(v_opacities + g, a[5]);
+                if (a[6] != 0.f) atomic_add_f32(v_betas + g, a[6]);
+            }
 #pragma unroll
             for (int q = 0; q < CDIM; ++q)
                 if (a[7 + q] != 0.f) atomic_add_f32(v_colors + (int64_t)g * channels + coff + q, a[7 + q]);
@@ -223,15 +278,16 @@ static int launch_bwd(bool f64t, uint32_t C, uint32_t N, int64_t n_isects, uint3
                       const float *betas, const float *bg, const uint8_t *masks, uint32_t W, uint32_t H,
                       uint32_t tw, uint32_t th, const int32_t *offs, const int32_t *flat, const float *ra,
                       const int32_t *last, const float *Tf, const float *vrc, const float *vra, float *vm2,
-                      float *vcon, float *vcol, float *vop, float *vbe, hipStream_t st) {
+                      float *vcon, float *vcol, float *vop, float *vbe, const float4 *rec, float *vrec,
+                      hipStream_t st) {
     if (f64t)
         raster_bwd_kernel<TS, CDIM, true><<<xcd_grid(C, tw, th), TS * TS, 0, st>>>(
             C, N, n_isects, channels, coff, means2d, conics, colors, opac, betas, bg, masks, W, H, tw, th, offs, flat,
-            ra, last, Tf, vrc, vra, vm2, vcon, vcol, vop, vbe);
+            ra, last, Tf, vrc, vra, vm2, vcon, vcol, vop, vbe, rec, vrec);
     else
         raster_bwd_kernel<TS, CDIM, false><<<xcd_grid(C, tw, th), TS * TS, 0, st>>>(
             C, N, n_isects, channels, coff, means2d, conics, colors, opac, betas, bg, masks, W, H, tw, th, offs, flat,
-            ra, last, Tf, vrc, vra, vm2, vcon, vcol, vop, vbe);
+            ra, last, Tf, vrc, vra, vm2, vcon, vcol, vop, vbe, rec, vrec);
     UBS_LAUNCH_CHECK("raster_bwd_kernel");
     return UBS_OK;
 }
@@ -243,10 +299,10 @@ static int launch_bwd_chunk(int cd, bool f64t, uint32_t C, uint32_t N, int64_t n
                             uint32_t H, uint32_t tw, uint32_t th, const int32_t *offs, const int32_t *flat,
                             const float *ra, const int32_t *last, const float *Tf, const float *vrc,
                             const float *vra, float *vm2, float *vcon, float *vcol, float *vop, float *vbe,
-                            hipStream_t st) {
+                            const float4 *rec, float *vrec, hipStream_t st) {
 #define UBS_BWD_ARGS                                                                                            \
     f64t, C, N, n_isects, channels, coff, means2d, conics, colors, opac, betas, bg, masks, W, H, tw, th, offs,  \
-        flat, ra, last, Tf, vrc, vra, vm2, vcon, vcol, vop, vbe, st
+        flat, ra, last, Tf, vrc, vra, vm2, vcon, vcol, vop, vbe, rec, vrec, st
     switch (cd) {
     case 1: return launch_bwd<TS, 1>(UBS_BWD_ARGS);
     case 2: return launch_bwd<TS, 2>(UBS_BWD_ARGS);
@@ -292,7 +348,8 @@ int ubs_rasterize_to_pixels_bwd(uint32_t C, uint32_t N, int64_t n_isects, uint32
                                 const int32_t *tile_offsets, const int32_t *flatten_ids, const float *render_alphas,
                                 const int32_t *last_ids, const float *T_final, const float *v_render_colors,
                                 const float *v_render_alphas, float *v_means2d, float *v_conics, float *v_colors,
-                                float *v_opacities, float *v_betas, void *stream) {
+                                float *v_opacities, float *v_betas, void *workspace, size_t workspace_bytes,
+                                void *stream) {
     hipStream_t st = as_stream(stream);
     UBS_REQUIRE(tile_size == 8 || tile_size == 16 || tile_size == 32, "tile_size must be 8, 16 or 32 (got %u)",
                 tile_size);
@@ -301,12 +358,35 @@ int ubs_rasterize_to_pixels_bwd(uint32_t C, uint32_t N, int64_t n_isects, uint32
     UBS_REQUIRE(n_isects >= 0 && n_isects < (1ll << 31), "n_isects out of range");
     const uint64_t CN = (uint64_t)C * N;
     UBS_REQUIRE(CN == 0 || (v_means2d && v_conics && v_colors && v_opacities && v_betas), "null gradient output");
-    int e = zero_async(v_means2d, CN * 2 * sizeof(float), st);
-    if (!e) e = zero_async(v_conics, CN * 3 * sizeof(float), st);
-    if (!e) e = zero_async(v_colors, CN * channels * sizeof(float), st);
-    if (!e) e = zero_async(v_opacities, CN * sizeof(float), st);
-    if (!e) e = zero_async(v_betas, CN * sizeof(float), st);
-    if (e) return e;
+    const bool packed = workspace != nullptr && n_isects != 0 && (uint64_t)C * W * H != 0 && CN != 0;
+    if (workspace) {
+        const size_t need = raster_workspace_bytes(C, N, 1);
+        if (workspace_bytes < need) {
+            set_error("blend workspace of %zu bytes, %zu needed", workspace_bytes, need);
+            return UBS_ERR_WORKSPACE;
+        }
+    }
+    const float4 *rec = nullptr;
+    float *vrec = nullptr;
+    int e = zero_async(v_colors, CN * channels * sizeof(float), st);
+    if (packed) {
+        // records in the first half of the workspace, packed gradients in the
+        // second; the scalar gradient tensors are written by the unpack below
+        float4 *r = static_cast<float4 *>(workspace);
+        vrec = reinterpret_cast<float *>(static_cast<char *>(workspace) + raster_workspace_bytes(C, N, 0));
+        if (!e) e = zero_async(vrec, CN * kRecordFloats * sizeof(float), st);
+        if (e) return e;
+        UBS_REQUIRE(means2d && conics && opacities && betas, "null pointer");
+        raster_pack_kernel<<<grid_for((int64_t)CN), kThreads, 0, st>>>(CN, means2d, conics, opacities, betas, r);
+        UBS_LAUNCH_CHECK("raster_pack_kernel");
+        rec = r;
+    } else {
+        if (!e) e = zero_async(v_means2d, CN * 2 * sizeof(float), st);
+        if (!e) e = zero_async(v_conics, CN * 3 * sizeof(float), st);
+        if (!e) e = zero_async(v_opacities, CN * sizeof(float), st);
+        if (!e) e = zero_async(v_betas, CN * sizeof(float), st);
+        if (e) return e;
+    }
     if (n_isects == 0 || (uint64_t)C * W * H == 0) return UBS_OK;
     UBS_REQUIRE((uint64_t)C * th * tw < (1ull << 31), "too many tiles");
     UBS_REQUIRE(means2d && conics && colors && opacities && betas && tile_offsets && flatten_ids && last_ids &&
@@ -320,19 +400,24 @@ int ubs_rasterize_to_pixels_bwd(uint32_t C, uint32_t N, int64_t n_isects, uint32
             e = launch_bwd_chunk<16>(cd, f64t, C, N, n_isects, channels, coff, means2d, conics, colors, opacities,
                                      betas, backgrounds, masks, W, H, tw, th, tile_offsets, flatten_ids,
                                      render_alphas, last_ids, T_final, v_render_colors, v_render_alphas, v_means2d,
-                                     v_conics, v_colors, v_opacities, v_betas, st);
+                                     v_conics, v_colors, v_opacities, v_betas, rec, vrec, st);
         else if (tile_size == 8)
             e = launch_bwd_chunk<8>(cd, f64t, C, N, n_isects, channels, coff, means2d, conics, colors, opacities,
                                     betas, backgrounds, masks, W, H, tw, th, tile_offsets, flatten_ids,
                                     render_alphas, last_ids, T_final, v_render_colors, v_render_alphas, v_means2d,
-                                    v_conics, v_colors, v_opacities, v_betas, st);
+                                    v_conics, v_colors, v_opacities, v_betas, rec, vrec, st);
         else
             e = launch_bwd_chunk<32>(cd, f64t, C, N, n_isects, channels, coff, means2d, conics, colors, opacities,
                                      betas, backgrounds, masks, W, H, tw, th, tile_offsets, flatten_ids,
                                      render_alphas, last_ids, T_final, v_render_colors, v_render_alphas, v_means2d,
-                                     v_conics, v_colors, v_opacities, v_betas, st);
+                                     v_conics, v_colors, v_opacities, v_betas, rec, vrec, st);
         if (e) return e;
         coff += (uint32_t)cd;
+    }
+    if (packed) {
+        raster_unpack_grads_kernel<<<grid_for((int64_t)CN), kThreads, 0, st>>>(
+            CN, reinterpret_cast<const float4 *>(vrec), v_means2d, v_conics, v_opacities, v_betas);
+        UBS_LAUNCH_CHECK("raster_unpack_grads_kernel");
     }
     return UBS_OK;
 }

@@ -25,7 +25,7 @@ __global__ __launch_bounds__(TS *TS) void raster_fwd_kernel(
     const uint8_t *__restrict__ masks, uint32_t W, uint32_t H, uint32_t tw, uint32_t th,
     const int32_t *__restrict__ tile_offsets, const int32_t *__restrict__ flatten_ids,
     float *__restrict__ render_colors, float *__restrict__ render_alphas, int32_t *__restrict__ last_ids,
-    float *__restrict__ T_final) {
+    float *__restrict__ T_final, const float4 *__restrict__ rec) {
     constexpr int BS = TS * TS;        // threads = batch size
     constexpr int SB = TS / 8;         // sub-blocks per tile side
     constexpr int NW = BS / 64;        // waves
@@ -84,10 +84,19 @@ __global__ __launch_bounds__(TS *TS) void raster_fwd_kernel(
         // ---- stage one primitive per thread
         if (tid < cnt) {
             const int32_t g = flatten_ids[batch_start + tid];
-            const float2 xy = reinterpret_cast<const float2 *>(means2d)[g];
-            const float ca = conics[(int64_t)g * 3], cb = conics[(int64_t)g * 3 + 1], cc = conics[(int64_t)g * 3 + 2];
+            float2 xy;
+            float ca, cb, cc, op, be;
+            if (rec) {  // one 32-byte record per (camera, primitive)
+                const float4 r0 = rec[2 * (int64_t)g], r1 = rec[2 * (int64_t)g + 1];
+                xy = make_float2(r0.x, r0.y);
+                ca = r0.z; cb = r0.w; cc = r1.x; op = r1.y; be = r1.z;
+            } else {
+                xy = reinterpret_cast<const float2 *>(means2d)[g];
+                ca = conics[(int64_t)g * 3]; cb = conics[(int64_t)g * 3 + 1]; cc = conics[(int64_t)g * 3 + 2];
+                op = opacities[g]; be = betas[g];
+            }
             s_geo[tid] = make_float4(xy.x, xy.y, ca, 2.f * cb);
-            s_geo2[tid] = make_float4(cc, opacities[g], betas[g], 0.f);
+            s_geo2[tid] = make_float4(cc, op, be, 0.f);
             const float *cp = colors + (int64_t)g * channels + coff;
 #pragma unroll
             for (int k = 0; k < CDIM; ++k) s_col[k][tid] = cp[k];
@@ -154,10 +163,10 @@ static int launch_fwd(uint32_t C, uint32_t N, int64_t n_isects, uint32_t channel
                       const float *means2d, const float *conics, const float *colors, const float *opac,
                       const float *betas, const float *bg, const uint8_t *masks, uint32_t W, uint32_t H,
                       uint32_t tw, uint32_t th, const int32_t *offs, const int32_t *flat, float *rc, float *ra,
-                      int32_t *last, float *Tf, hipStream_t st) {
+                      int32_t *last, float *Tf, const float4 *rec, hipStream_t st) {
     raster_fwd_kernel<TS, CDIM><<<xcd_grid(C, tw, th), TS * TS, 0, st>>>(
         C, N, n_isects, channels, coff, means2d, conics, colors, opac, betas, bg, masks, W, H, tw, th, offs, flat,
-        rc, ra, last, Tf);
+        rc, ra, last, Tf, rec);
     UBS_LAUNCH_CHECK("raster_fwd_kernel");
     return UBS_OK;
 }
@@ -167,11 +176,11 @@ static int launch_fwd_chunk(int cd, uint32_t C, uint32_t N, int64_t n_isects, ui
                             const float *means2d, const float *conics, const float *colors, const float *opac,
                             const float *betas, const float *bg, const uint8_t *masks, uint32_t W, uint32_t H,
                             uint32_t tw, uint32_t th, const int32_t *offs, const int32_t *flat, float *rc,
-                            float *ra, int32_t *last, float *Tf, hipStream_t st) {
+                            float *ra, int32_t *last, float *Tf, const float4 *rec, hipStream_t st) {
 #define UBS_FWD_CASE(K)                                                                                     \
     case K:                                                                                                 \
         return launch_fwd<TS, K>(C, N, n_isects, channels, coff, means2d, conics, colors, opac, betas, bg,  \
-                                 masks, W, H, tw, th, offs, flat, rc, ra, last, Tf, st);
+                                 masks, W, H, tw, th, offs, flat, rc, ra, last, Tf, rec, st);
     switch (cd) {
         UBS_FWD_CASE(1)
         UBS_FWD_CASE(2)
@@ -180,17 +189,17 @@ static int launch_fwd_chunk(int cd, uint32_t C, uint32_t N, int64_t n_isects, ui
     case 8:
         if constexpr (TS <= 16)
             return launch_fwd<TS, 8>(C, N, n_isects, channels, coff, means2d, conics, colors, opac, betas, bg,
-                                     masks, W, H, tw, th, offs, flat, rc, ra, last, Tf, st);
+                                     masks, W, H, tw, th, offs, flat, rc, ra, last, Tf, rec, st);
         break;
     case 16:
         if constexpr (TS <= 16)
             return launch_fwd<TS, 16>(C, N, n_isects, channels, coff, means2d, conics, colors, opac, betas, bg,
-                                      masks, W, H, tw, th, offs, flat, rc, ra, last, Tf, st);
+                                      masks, W, H, tw, th, offs, flat, rc, ra, last, Tf, rec, st);
         break;
     case 32:
         if constexpr (TS <= 16)
             return launch_fwd<TS, 32>(C, N, n_isects, channels, coff, means2d, conics, colors, opac, betas, bg,
-                                      masks, W, H, tw, th, offs, flat, rc, ra, last, Tf, st);
+                                      masks, W, H, tw, th, offs, flat, rc, ra, last, Tf, rec, st);
         break;
     default:
         break;
@@ -210,7 +219,8 @@ extern "C" int ubs_rasterize_to_pixels_fwd(uint32_t C, uint32_t N, int64_t n_ise
                                            const uint8_t *masks, uint32_t W, uint32_t H, uint32_t tile_size,
                                            uint32_t tw, uint32_t th, const int32_t *tile_offsets,
                                            const int32_t *flatten_ids, float *render_colors, float *render_alphas,
-                                           int32_t *last_ids, float *T_final, void *stream) {
+                                           int32_t *last_ids, float *T_final, void *workspace,
+                                           size_t workspace_bytes, void *stream) {
     hipStream_t st = as_stream(stream);
     UBS_REQUIRE(tile_size == 8 || tile_size == 16 || tile_size == 32, "tile_size must be 8, 16 or 32 (got %u)",
                 tile_size);
@@ -222,6 +232,21 @@ extern "C" int ubs_rasterize_to_pixels_fwd(uint32_t C, uint32_t N, int64_t n_ise
     UBS_REQUIRE(means2d && conics && colors && opacities && betas && tile_offsets && render_colors && render_alphas &&
                     last_ids && (flatten_ids || n_isects == 0),
                 "null pointer");
+    const float4 *rec = nullptr;
+    if (workspace) {
+        const size_t need = raster_workspace_bytes(C, N, 0);
+        if (workspace_bytes < need) {
+            set_error("blend workspace of %zu bytes, %zu needed", workspace_bytes, need);
+            return UBS_ERR_WORKSPACE;
+        }
+        const uint64_t CN = (uint64_t)C * N;
+        if (CN && n_isects) {
+            raster_pack_kernel<<<grid_for((int64_t)CN), kThreads, 0, st>>>(CN, means2d, conics, opacities, betas,
+                                                                           static_cast<float4 *>(workspace));
+            UBS_LAUNCH_CHECK("raster_pack_kernel");
+            rec = static_cast<const float4 *>(workspace);
+        }
+    }
     uint32_t coff = 0;
     while (coff < channels) {
         const int cd = pick_chunk(channels - coff, tile_size);
@@ -229,17 +254,21 @@ extern "C" int ubs_rasterize_to_pixels_fwd(uint32_t C, uint32_t N, int64_t n_ise
         if (tile_size == 16)
             e = launch_fwd_chunk<16>(cd, C, N, n_isects, channels, coff, means2d, conics, colors, opacities, betas,
                                      backgrounds, masks, W, H, tw, th, tile_offsets, flatten_ids, render_colors,
-                                     render_alphas, last_ids, T_final, st);
+                                     render_alphas, last_ids, T_final, rec, st);
         else if (tile_size == 8)
             e = launch_fwd_chunk<8>(cd, C, N, n_isects, channels, coff, means2d, conics, colors, opacities, betas,
                                     backgrounds, masks, W, H, tw, th, tile_offsets, flatten_ids, render_colors,
-                                    render_alphas, last_ids, T_final, st);
+                                    render_alphas, last_ids, T_final, rec, st);
         else
             e = launch_fwd_chunk<32>(cd, C, N, n_isects, channels, coff, means2d, conics, colors, opacities, betas,
                                      backgrounds, masks, W, H, tw, th, tile_offsets, flatten_ids, render_colors,
-                                     render_alphas, last_ids, T_final, st);
+                                     render_alphas, last_ids, T_final, rec, st);
         if (e) return e;
         coff += (uint32_t)cd;
     }
     return UBS_OK;
+}
+
+extern "C" size_t ubs_rasterize_workspace_bytes(uint32_t C, uint32_t N, int backward) {
+    return raster_workspace_bytes(C, N, backward);
 }

@@ -60,12 +60,13 @@ _SIGNATURES = {
         _c_int, [_c_i64, _vp, _vp, _vp, _vp, _c_int, _c_int, _vp, _c_size, ctypes.POINTER(_c_int), _vp]),
     "ubs_radix_sort_status": (_c_int, [_vp, ctypes.POINTER(_c_int), _vp]),
     "ubs_isect_offset_encode": (_c_int, [_c_i64, _vp, _c_u32, _c_u32, _c_u32, _vp, _vp]),
+    "ubs_rasterize_workspace_bytes": (_c_size, [_c_u32, _c_u32, _c_int]),
     "ubs_rasterize_to_pixels_fwd": (
         _c_int, [_c_u32, _c_u32, _c_i64, _c_u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_u32, _c_u32, _c_u32,
-                 _c_u32, _c_u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+                 _c_u32, _c_u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_size, _vp]),
     "ubs_rasterize_to_pixels_bwd": (
         _c_int, [_c_u32, _c_u32, _c_i64, _c_u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_u32, _c_u32, _c_u32,
-                 _c_u32, _c_u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+                 _c_u32, _c_u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_size, _vp]),
     "ubs_set_backward_fp64_transmittance": (_c_int, [_c_int]),
     "ubs_set_cond_backward_exact": (_c_int, [_c_int]),
 }

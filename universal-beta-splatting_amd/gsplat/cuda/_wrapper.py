@@ -346,6 +346,23 @@ def _mask_ptr(masks: Optional[Tensor]):
     return ptr(masks), masks
 
 
+_packed_records = True
+
+
+def set_blend_packed_records(enable: bool) -> None:
+    """Gather 32-byte packed primitive records in the blend kernels (default)
+    or the four separate input arrays (same results, more gather traffic)."""
+    global _packed_records
+    _packed_records = bool(enable)
+
+
+def _blend_workspace(C: int, N: int, backward: bool, device):
+    if not _packed_records or C * N == 0:
+        return None, 0
+    nbytes = int(lib().ubs_rasterize_workspace_bytes(C, N, 1 if backward else 0))
+    return torch.empty((nbytes,), dtype=torch.uint8, device=device), nbytes
+
+
 def raster_fwd_raw(means2d, conics, colors, opacities, betas, backgrounds, masks, width, height, tile_size,
                    isect_offsets, flatten_ids):
     require_cuda(means2d, conics, colors, opacities, betas, backgrounds, isect_offsets, flatten_ids)
@@ -360,10 +377,11 @@ def raster_fwd_raw(means2d, conics, colors, opacities, betas, backgrounds, masks
     last_ids = torch.empty((C, height, width), dtype=torch.int32, device=dev)
     T_final = torch.empty((C, height, width), dtype=torch.float32, device=dev)
     mp = _mask_ptr(masks)
+    ws, ws_bytes = _blend_workspace(C, N, False, dev)
     _call("ubs_rasterize_to_pixels_fwd", C, N, flatten_ids.numel(), channels, ptr(means2d), ptr(conics),
           ptr(colors), ptr(opacities), ptr(betas), ptr(backgrounds), mp[0] if mp else None, width, height,
           tile_size, tw, th, ptr(isect_offsets), ptr(flatten_ids), ptr(render_colors), ptr(render_alphas),
-          ptr(last_ids), ptr(T_final), stream_of(means2d))
+          ptr(last_ids), ptr(T_final), ptr(ws), ws_bytes, stream_of(means2d))
     return render_colors, render_alphas, last_ids, T_final
 
 
@@ -384,11 +402,12 @@ def raster_bwd_raw(means2d, conics, colors, opacities, betas, backgrounds, masks
     v_opacities = torch.empty_like(opacities)
     v_betas = torch.empty_like(betas)
     mp = _mask_ptr(masks)
+    ws, ws_bytes = _blend_workspace(C, N, True, means2d.device)
     _call("ubs_rasterize_to_pixels_bwd", C, N, flatten_ids.numel(), channels, ptr(means2d), ptr(conics),
           ptr(colors), ptr(opacities), ptr(betas), ptr(backgrounds), mp[0] if mp else None, width, height,
           tile_size, tw, th, ptr(isect_offsets), ptr(flatten_ids), ptr(render_alphas), ptr(last_ids),
           ptr(T_final), ptr(v_render_colors), ptr(v_render_alphas), ptr(v_means2d), ptr(v_conics),
-          ptr(v_colors), ptr(v_opacities), ptr(v_betas), stream_of(means2d))
+          ptr(v_colors), ptr(v_opacities), ptr(v_betas), ptr(ws), ws_bytes, stream_of(means2d))
     return v_means2d, v_conics, v_colors, v_opacities, v_betas
 
 
