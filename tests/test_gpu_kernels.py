@@ -224,7 +224,7 @@ def test_isect_unsorted_and_edges(device):
             assert_same(host(a), b, f"{n} sort={sort}")
 
 
-@pytest.mark.parametrize("n", [0, 1, 63, 4095, 4096, 4097, 100003])
+@pytest.mark.parametrize("n", [0, 1, 63, 4095, 4096, 6143, 6144, 6145, 12289, 100003])
 @pytest.mark.parametrize("end_bit", [7, 33, 46, 64])
 def test_radix_sort_stable(device, n, end_bit):
     g = np.random.Generator(np.random.PCG64(n * 131 + end_bit))
