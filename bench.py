@@ -3,9 +3,10 @@
 
 One step = one training step of the BetaModel render path on one view:
 parameter activations, view-direction query, covariance construction,
-conditioning, projection, tile intersection + sort + ranges, blend forward,
-L1 loss against a target image, and the backward of all of it down to the
-leaf parameters.  Inputs are synthetic (seeded) and resident in HBM before the
+conditioning, projection (fused into one launch each way by default,
+--path chain runs them as the reference's separate ops), tile intersection +
+sort + ranges, blend forward, L1 loss against a target image, and the
+backward of all of it down to the leaf parameters.  Inputs are synthetic (seeded) and resident in HBM before the
 timed region.  Prints one JSON line: throughput, latency percentiles, the
 roofline of the dominant kernel and a CPU baseline from the oracle.
 
@@ -75,6 +76,11 @@ def algorithmic_bytes(name: str, c: dict, n_isect: int, C: int = 1) -> float:
         return N * 4.0 * ((3 + (D - 3) + D + 1 + (D - 2) + 4) + (D + D + 1 + (D - 2) + (D - 3) + 9))
     if name == "ubs_beta_activate_bwd":  # raw leaves + output grads in; leaf grads out
         return N * 4.0 * ((D + 1 + (D - 2) + 4) + (D + D + 1 + (D - 2) + 9) + (3 + (D - 3) + D + 1 + (D - 2) + 4))
+    leaf_floats = 3 + (D - 3) + D + D * (D - 1) // 2 + 1 + (D - 2) + 4  # with quats
+    if name == "ubs_beta_project_fwd":  # leaves once; radii, means2d, depth, conic, opacity, beta per view
+        return N * 4.0 * leaf_floats + C * N * 36.0
+    if name == "ubs_beta_project_bwd":  # leaves once; radii, conic and 5 gradients per view; leaf grads
+        return N * 4.0 * 2 * leaf_floats + C * N * 48.0
     if name == "ubs_rot_scale_l_triangle_to_covar_fwd":
         return N * 4.0 * (9 + D + D * (D - 1) // 2 + D * D)
     if name == "ubs_rot_scale_l_triangle_to_covar_bwd":
@@ -145,6 +151,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--path", default="fused", choices=("fused", "chain"),
+                    help="fused: one-launch Beta projection (render_views); chain: the reference's op chain "
+                         "(render: activations, rotation, covariance, conditioning, projection as separate ops)")
     ap.add_argument("--fp64-transmittance", action="store_true",
                     help="recover transmittance in fp64 in the blend backward")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -180,17 +189,25 @@ def main():
     leaves = list(params.leaves().values())
     W.set_backward_fp64_transmittance(args.fp64_transmittance)
 
+    vms_t, Ks_t = vm[None].contiguous(), K[None].contiguous()
+    ts_t = torch.tensor([t_view], dtype=torch.float32, device=dev)
+
+    def render():
+        if args.path == "fused":
+            return beta.render_views(params, vms_t, Ks_t, cfg["W"], cfg["H"], timestamps=ts_t)
+        return beta.render(params, vm, K, cfg["W"], cfg["H"], timestamp=t_view, rest=rest)
+
     def step():
         for p in leaves:
             p.grad = None
-        img, alpha, meta = beta.render(params, vm, K, cfg["W"], cfg["H"], timestamp=t_view, rest=rest)
+        img, alpha, meta = render()
         loss = (img[0] - tgt).abs().mean()
         loss.backward()
         return meta
 
     def fwd_only():
         with torch.no_grad():
-            return beta.render(params, vm, K, cfg["W"], cfg["H"], timestamp=t_view, rest=rest)
+            return render()
 
     for _ in range(args.warmup):
         meta = step()
@@ -263,7 +280,7 @@ def main():
         "data": "synthetic (seeded scene and camera generator, gsplat/scenes.py)",
         "config": {"workload": cfg["workload"], "primitives": cfg["N"], "dims": cfg["D"],
                    "width": cfg["W"], "height": cfg["H"], "views": 1, "tile_size": 16,
-                   "n_isects": n_isect, "visible_primitives": n_vis},
+                   "n_isects": n_isect, "visible_primitives": n_vis, "projection_path": args.path},
         "fwd_frames_per_s": round(1e3 / fwd_ms, 3),
         "fwd_bwd_mpix_per_s": round(cfg["W"] * cfg["H"] * 1e3 / ms_per_step / 1e6, 2),
         "latency_ms": {"p50": round(step_ms[len(step_ms) // 2], 3),

@@ -77,7 +77,7 @@ int ubs_beta_activate_bwd(int64_t N, int64_t D, const float *scale_raw, const fl
 int ubs_l_triangle_to_rotmat_fwd(int64_t N, const float *l_triangle, float *rotmat, void *stream);
 
 /* v_rotmat [N,3,3] -> v_l_triangle [N,3].
- * Replaces l_triangle_to_rotmat_bwd_tensor, l_triagnle_to_rotmat_bwd.cu:38-67. */
+ * Replaces l_triangle_to_rotmat_bwd_tensor, l_triagnle_to_rotmat_bwd.cu:27-58. */
 int ubs_l_triangle_to_rotmat_bwd(int64_t N, const float *v_rotmat, float *v_l_triangle, void *stream);
 
 /* Sigma = L L^T, L[:3,:3] = rot diag(scale[:3]), L[i,i] = scale[i] (i>=3),
@@ -162,6 +162,58 @@ int ubs_fully_fused_projection_bwd(uint32_t C, uint32_t N, const float *means, c
                                    const float *v_depths, const float *v_conics,
                                    const float *v_compensations, float *v_means, float *v_covars,
                                    float *v_quats, float *v_scales, float *v_viewmats, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Fused Beta projection: leaves of a D-dim Beta scene (D = 6 or 7) -> the
+ * per-view 2-D primitives, for C views in one launch.  Replaces, per view,
+ * the chain BetaModel.render runs before the blend (scene/beta_model.py:
+ * 660-722): the activations and query (:36-52, :103-121, :675-690),
+ * l_triangle_to_rotmat_fwd_tensor (l_triagnle_to_rotmat_fwd.cu:38-67) or the
+ * quaternion rotation, rot_scale_l_triangle_to_covar_fwd_tensor
+ * (rot_scale_l_triangle_to_covar_fwd.cu:191-235),
+ * cond_mean_convariance_opacity_fwd_tensor
+ * (cond_mean_convariance_opacity_fwd.cu:292-349), the [N,3,3] -> [N,6]
+ * gather and repeat of rendering.py:54-56,100-101 and
+ * fully_fused_projection_fwd_tensor (fully_fused_projection_fwd.cu:179-256).
+ * With C = 1 every output equals that chain's bit for bit.
+ * ------------------------------------------------------------------------ */
+
+/* Leaves as BetaModel stores them, pre-activation: xyz [N,3], mean_rest
+ * [N,D-3], scale_raw [N,D], l_triangle [N,D(D-1)/2] (canonical rest pattern of
+ * beta_model.py:69-73), opacity_raw [N,1], beta_raw [N,D-2]; quats [N,4] or
+ * NULL (then the rotation is I + skew(l_triangle[:, :3])).
+ * Per view: cam_pos [C,3] (camera centre), timestamps [C] (7-D; NULL -> 0),
+ * viewmats [C,4,4], Ks [C,3,3]; image size width x height, or per view
+ * image_wh int32 [C,2] when not NULL.
+ * Outputs [C,N]: radii int32 (0 = culled; the other 2-D outputs of a culled
+ * entry are left untouched), means2d [C,N,2], depths, conics [C,N,3],
+ * opacities (conditioned) and betas (beta_0) for every entry. */
+int ubs_beta_project_fwd(int64_t N, int64_t D, uint32_t C, const float *xyz, const float *mean_rest,
+                         const float *scale_raw, const float *l_triangle, const float *opacity_raw,
+                         const float *beta_raw, const float *quats, const float *cam_pos, const float *timestamps,
+                         const float *viewmats, const float *Ks, int32_t width, int32_t height,
+                         const int32_t *image_wh, float eps2d, float near_plane, float far_plane, float radius_clip,
+                         int ortho, int32_t *radii, float *means2d, float *depths, float *conics, float *opacities,
+                         float *betas, void *stream);
+
+/* Backward: the saved radii and conics and the gradients [C,N] of means2d,
+ * depths, conics, opacities and betas -> the leaf gradients (each written
+ * once, summed over the views): v_xyz [N,3], v_mean_rest [N,D-3],
+ * v_scale_raw [N,D], v_l_triangle [N,D(D-1)/2], v_opacity_raw [N,1],
+ * v_beta_raw [N,D-2], v_quats [N,4] (required when quats is given).
+ * Replaces the backward of the chain above (fully_fused_projection_bwd.cu:
+ * 242-348, cond_mean_convariance_opacity_bwd.cu:565-630,
+ * rot_scale_l_triangle_to_covar_bwd.cu:238-293, l_triagnle_to_rotmat_bwd.cu:
+ * 27-58 and torch's activation backward); follows ubs_set_cond_backward_exact. */
+int ubs_beta_project_bwd(int64_t N, int64_t D, uint32_t C, const float *xyz, const float *mean_rest,
+                         const float *scale_raw, const float *l_triangle, const float *opacity_raw,
+                         const float *beta_raw, const float *quats, const float *cam_pos, const float *timestamps,
+                         const float *viewmats, const float *Ks, int32_t width, int32_t height,
+                         const int32_t *image_wh, float eps2d, int ortho, const int32_t *radii, const float *conics,
+                         const float *v_means2d, const float *v_depths, const float *v_conics,
+                         const float *v_opacities, const float *v_betas, float *v_xyz, float *v_mean_rest,
+                         float *v_scale_raw, float *v_l_triangle, float *v_opacity_raw, float *v_beta_raw,
+                         float *v_quats, void *stream);
 
 /* ------------------------------------------------------------------------
  * Tile intersection (isect_tiles.cu:99-285 split at its host sync).

@@ -15,27 +15,9 @@
 // primitive; every input row is read once and every output row written once.
 #pragma clang fp contract(off)
 
-#include "ubs_common.h"
+#include "ubs_activate_math.h"
 
 namespace ubs {
-
-__device__ __forceinline__ float softplus_f(float x) { return x > 20.f ? x : log1pf(expf(x)); }
-// d softplus / dx = sigmoid(x), 1 above the threshold (torch's softplus_backward)
-__device__ __forceinline__ float softplus_grad(float x) {
-    if (x > 20.f) return 1.f;
-    const float z = expf(x);
-    return z / (z + 1.f);
-}
-__device__ __forceinline__ float sigmoid_f(float x) { return 1.f / (1.f + expf(-x)); }
-
-// R(q / |q|), row-major; the product formula of quat_to_rotmat.
-__device__ __forceinline__ void quat_rot(const float (&q)[4], float (&R)[9]) {
-    const float inv = 1.f / fmaxf(sqrtf((q[0] * q[0] + q[1] * q[1]) + (q[2] * q[2] + q[3] * q[3])), 1e-12f);
-    const float w = q[0] * inv, x = q[1] * inv, y = q[2] * inv, z = q[3] * inv;
-    R[0] = 1.f - 2.f * (y * y + z * z); R[1] = 2.f * (x * y - w * z);       R[2] = 2.f * (x * z + w * y);
-    R[3] = 2.f * (x * y + w * z);       R[4] = 1.f - 2.f * (x * x + z * z); R[5] = 2.f * (y * z - w * x);
-    R[6] = 2.f * (x * z - w * y);       R[7] = 2.f * (y * z + w * x);       R[8] = 1.f - 2.f * (x * x + y * y);
-}
 
 template <int D>
 __global__ __launch_bounds__(kThreads) void activate_fwd_kernel(
@@ -56,10 +38,10 @@ __global__ __launch_bounds__(kThreads) void activate_fwd_kernel(
 #pragma unroll
     for (int k = 0; k < D - 2; ++k) beta[n * (D - 2) + k] = 4.f * expf(beta_raw[n * (D - 2) + k]);
     if (query) {
-        const float d0 = p0 - cam[0], d1 = p1 - cam[1], d2 = p2 - cam[2];
-        const float nrm = sqrtf((d0 * d0 + d1 * d1) + d2 * d2);
+        float qd[3];
+        view_dir(p0, p1, p2, cam, qd);
         float *q = query + n * (D - 3);
-        q[0] = d0 / nrm; q[1] = d1 / nrm; q[2] = d2 / nrm;
+        q[0] = qd[0]; q[1] = qd[1]; q[2] = qd[2];
         if constexpr (D == 7) q[3] = timestamp;
     }
     if (rot) {
@@ -102,23 +84,12 @@ __global__ __launch_bounds__(kThreads) void activate_bwd_kernel(
         float q[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) q[k] = quats[n * 4 + k];
-        const float len = fmaxf(sqrtf((q[0] * q[0] + q[1] * q[1]) + (q[2] * q[2] + q[3] * q[3])), 1e-12f);
-        const float inv = 1.f / len;
-        const float w = q[0] * inv, x = q[1] * inv, y = q[2] * inv, z = q[3] * inv;
-        float g[9];
+        float g[9], vq[4];
 #pragma unroll
         for (int k = 0; k < 9; ++k) g[k] = v_rot ? v_rot[n * 9 + k] : 0.f;
-        // dL/d(w,x,y,z) of the unit quaternion
-        const float gw = 2.f * (x * (g[7] - g[5]) + y * (g[2] - g[6]) + z * (g[3] - g[1]));
-        const float gx = 2.f * (-2.f * x * (g[4] + g[8]) + y * (g[1] + g[3]) + z * (g[2] + g[6]) + w * (g[7] - g[5]));
-        const float gy = 2.f * (x * (g[1] + g[3]) - 2.f * y * (g[0] + g[8]) + z * (g[5] + g[7]) + w * (g[2] - g[6]));
-        const float gz = 2.f * (x * (g[2] + g[6]) + y * (g[5] + g[7]) - 2.f * z * (g[0] + g[4]) + w * (g[3] - g[1]));
-        // through the normalisation: (g - u (u . g)) / |q|
-        const float dot = (w * gw + x * gx) + (y * gy + z * gz);
-        v_quats[n * 4 + 0] = (gw - w * dot) * inv;
-        v_quats[n * 4 + 1] = (gx - x * dot) * inv;
-        v_quats[n * 4 + 2] = (gy - y * dot) * inv;
-        v_quats[n * 4 + 3] = (gz - z * dot) * inv;
+        quat_rot_vjp(q, g, vq);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v_quats[n * 4 + k] = vq[k];
     }
 }
 

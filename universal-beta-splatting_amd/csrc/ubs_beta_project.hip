@@ -1,0 +1,309 @@
+// Fused Beta projection on gfx950: from the leaf parameters of a D-dim Beta
+// scene (D = 6 static, 7 dynamic) straight to the per-view 2-D primitives the
+// tile intersection and the blend consume, for C views in one launch.
+//
+// One thread per primitive.  The per-primitive part -- activations
+// (scene/beta_model.py:36-52, :103-121), the rotation (unit quaternion or the
+// first-order l_triangle skew of l_triagnle_to_rotmat_fwd.cu:16-35) and
+// Sigma = L L^T (rot_scale_l_triangle_to_covar_fwd.cu:28-187) -- runs once,
+// in registers; then for every view: the query (beta_model.py:675-690),
+// the conditioning on it (cond_mean_convariance_opacity_fwd.cu:106-290) and
+// the EWA projection of the conditioned 3-D primitive
+// (fully_fused_projection_fwd.cu:20-177, reading the upper triangle of the
+// conditioned covariance as rendering.py:54-56 gathers it).
+//
+// The reference runs these as ~45 torch launches and 4 kernels per view with
+// [N,D,D] covariances, [N,3,3] rotations and conditioned [N,3,3] matrices
+// round-tripping through HBM; here the leaves (140 B per 6-D primitive) are
+// read once for all views and only the 2-D records (40 B per view) written.
+//
+// The backward recomputes the forward per view in registers and chains the
+// VJPs of the same functions (projection -> triangle fold -> conditioning),
+// sums the per-view gradients of the mean, covariance, opacity and Beta
+// parameters in registers, then applies the covariance, rotation and
+// activation VJPs once: each leaf gradient is written exactly once, no
+// atomics.  Every stage calls the same device functions as the separate
+// kernels (ubs_*_math.h), so with one view the fused results are
+// bit-identical to the drop-in chain.
+#pragma clang fp contract(off)
+
+#include "ubs_activate_math.h"
+#include "ubs_cond_math.h"
+#include "ubs_covar_math.h"
+#include "ubs_projection_math.h"
+
+namespace ubs {
+
+// Per-primitive state shared by every view.
+template <int D>
+struct BetaPrim {
+    float mean[D], scale[D], beta[D - 2], opac;
+    float R[9];
+    float L[D][D];
+    uint32_t P[D];
+    float Sig[D * D];
+};
+
+template <int D, bool QUAT>
+__device__ __forceinline__ void beta_prim_setup(int64_t n, const float *__restrict__ xyz,
+                                                const float *__restrict__ mean_rest,
+                                                const float *__restrict__ scale_raw,
+                                                const float *__restrict__ ltri,
+                                                const float *__restrict__ opacity_raw,
+                                                const float *__restrict__ beta_raw,
+                                                const float *__restrict__ quats, BetaPrim<D> &bp) {
+    constexpr int FULL_M = D * (D - 1) / 2;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) bp.mean[k] = xyz[n * 3 + k];
+#pragma unroll
+    for (int k = 3; k < D; ++k) bp.mean[k] = mean_rest[n * (D - 3) + (k - 3)];
+#pragma unroll
+    for (int k = 0; k < D; ++k) bp.scale[k] = softplus_f(scale_raw[n * D + k]);
+    bp.opac = sigmoid_f(opacity_raw[n]);
+#pragma unroll
+    for (int k = 0; k < D - 2; ++k) bp.beta[k] = 4.f * expf(beta_raw[n * (D - 2) + k]);
+    const float *lt = ltri + n * FULL_M;
+    if constexpr (QUAT) {
+        float q[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) q[k] = quats[n * 4 + k];
+        quat_rot(q, bp.R);
+    } else {
+        const float a0 = lt[0], a1 = lt[1], a2 = lt[2];
+        bp.R[0] = 1.f; bp.R[1] = a0;  bp.R[2] = a1;
+        bp.R[3] = -a0; bp.R[4] = 1.f; bp.R[5] = a2;
+        bp.R[6] = -a1; bp.R[7] = -a2; bp.R[8] = 1.f;
+    }
+    float xr[FULL_M - 3];
+#pragma unroll
+    for (int k = 0; k < FULL_M - 3; ++k) xr[k] = lt[3 + k];
+    load_L<D>(bp.R, bp.scale, xr, nullptr, FULL_M - 3, true, bp.L, bp.P);
+    covar_fwd_body<D>(bp.L, bp.P, false, bp.Sig);
+}
+
+// Query of view c: unit direction from the camera centre (+ the view's time
+// for 7-D).
+template <int D>
+__device__ __forceinline__ void beta_query(const BetaPrim<D> &bp, const float *__restrict__ cam_pos,
+                                           const float *__restrict__ timestamps, uint32_t c,
+                                           float (&q)[D - 3]) {
+    float qd[3];
+    view_dir(bp.mean[0], bp.mean[1], bp.mean[2], cam_pos + (size_t)c * 3, qd);
+    q[0] = qd[0];
+    q[1] = qd[1];
+    q[2] = qd[2];
+    if constexpr (D == 7) q[3] = timestamps ? timestamps[c] : 0.f;
+}
+
+template <int D, bool QUAT>
+__global__ __launch_bounds__(kThreads) void beta_project_fwd_kernel(
+    int64_t N, uint32_t C, const float *__restrict__ xyz, const float *__restrict__ mean_rest,
+    const float *__restrict__ scale_raw, const float *__restrict__ ltri, const float *__restrict__ opacity_raw,
+    const float *__restrict__ beta_raw, const float *__restrict__ quats, const float *__restrict__ cam_pos,
+    const float *__restrict__ timestamps, const float *__restrict__ viewmats, const float *__restrict__ Ks,
+    int32_t W, int32_t H, const int32_t *__restrict__ image_wh, float eps2d, float near_plane, float far_plane,
+    float radius_clip, int ortho, int32_t *__restrict__ radii, float *__restrict__ means2d,
+    float *__restrict__ depths, float *__restrict__ conics, float *__restrict__ opacities,
+    float *__restrict__ betas) {
+    constexpr int CQ = D - 3;
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    BetaPrim<D> bp;
+    beta_prim_setup<D, QUAT>(n, xyz, mean_rest, scale_raw, ltri, opacity_raw, beta_raw, quats, bp);
+    for (uint32_t c = 0; c < C; ++c) {
+        const int64_t idx = (int64_t)c * N + n;
+        float q[CQ];
+        beta_query<D>(bp, cam_pos, timestamps, c, q);
+        CondRegs<CQ> st;
+        cond_setup<CQ>(bp.mean, bp.Sig, bp.beta + 1, q, st);
+        float m3[3], V9[9], oc;
+        cond_fwd_body<CQ>(st, bp.mean, bp.opac, m3, V9, &oc);
+        opacities[idx] = oc;
+        betas[idx] = bp.beta[0];
+        const int32_t Wc = image_wh ? image_wh[2 * c] : W, Hc = image_wh ? image_wh[2 * c + 1] : H;
+        ProjOut o;
+        if (!proj_fwd_body(m3, V9, 1, nullptr, nullptr, viewmats + (size_t)c * 16, Ks + (size_t)c * 9, 0u, Wc, Hc,
+                           eps2d, near_plane, far_plane, radius_clip, ortho != 0, o)) {
+            radii[idx] = 0;
+            continue;
+        }
+        radii[idx] = o.radius;
+        means2d[idx * 2] = o.m2[0];
+        means2d[idx * 2 + 1] = o.m2[1];
+        depths[idx] = o.depth;
+        conics[idx * 3] = o.conic[0];
+        conics[idx * 3 + 1] = o.conic[1];
+        conics[idx * 3 + 2] = o.conic[2];
+    }
+}
+
+template <int D, bool QUAT>
+__global__ __launch_bounds__(kThreads) void beta_project_bwd_kernel(
+    int64_t N, uint32_t C, const float *__restrict__ xyz, const float *__restrict__ mean_rest,
+    const float *__restrict__ scale_raw, const float *__restrict__ ltri, const float *__restrict__ opacity_raw,
+    const float *__restrict__ beta_raw, const float *__restrict__ quats, const float *__restrict__ cam_pos,
+    const float *__restrict__ timestamps, const float *__restrict__ viewmats, const float *__restrict__ Ks,
+    int32_t W, int32_t H, const int32_t *__restrict__ image_wh, float eps2d, int ortho, int exact,
+    const int32_t *__restrict__ radii, const float *__restrict__ conics, const float *__restrict__ v_means2d,
+    const float *__restrict__ v_depths, const float *__restrict__ v_conics, const float *__restrict__ v_opacities,
+    const float *__restrict__ v_betas, float *__restrict__ v_xyz, float *__restrict__ v_mean_rest,
+    float *__restrict__ v_scale_raw, float *__restrict__ v_ltri, float *__restrict__ v_opacity_raw,
+    float *__restrict__ v_beta_raw, float *__restrict__ v_quats) {
+    constexpr int CQ = D - 3;
+    constexpr int FULL_M = D * (D - 1) / 2;
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    BetaPrim<D> bp;
+    beta_prim_setup<D, QUAT>(n, xyz, mean_rest, scale_raw, ltri, opacity_raw, beta_raw, quats, bp);
+    // gradients of the activated mean, covariance, opacity and Beta
+    // parameters, summed over the views
+    float gmean[D], gSig[D * D], gop = 0.f, gbeta[D - 2];
+#pragma unroll
+    for (int k = 0; k < D; ++k) gmean[k] = 0.f;
+#pragma unroll
+    for (int k = 0; k < D * D; ++k) gSig[k] = 0.f;
+#pragma unroll
+    for (int k = 0; k < D - 2; ++k) gbeta[k] = 0.f;
+    for (uint32_t c = 0; c < C; ++c) {
+        const int64_t idx = (int64_t)c * N + n;
+        // a culled view has no 2-D record, so every gradient it could pass is 0
+        if (radii[idx] <= 0) continue;
+        float q[CQ];
+        beta_query<D>(bp, cam_pos, timestamps, c, q);
+        CondRegs<CQ> st;
+        cond_setup<CQ>(bp.mean, bp.Sig, bp.beta + 1, q, st);
+        float m3[3], V9[9], oc;
+        cond_fwd_body<CQ>(st, bp.mean, bp.opac, m3, V9, &oc);
+        const int32_t Wc = image_wh ? image_wh[2 * c] : W, Hc = image_wh ? image_wh[2 * c + 1] : H;
+        float vmean[3], vS[3][3], vR[3][3], vt[3];
+        proj_bwd_body(m3, V9, 1, nullptr, nullptr, viewmats + (size_t)c * 16, Ks + (size_t)c * 9, 0u, Wc, Hc, eps2d,
+                      ortho != 0, conics + idx * 3, nullptr, v_means2d + idx * 2, v_depths[idx], v_conics + idx * 3,
+                      nullptr, vmean, vS, vR, vt);
+        // the projection reads the upper triangle of the conditioned
+        // covariance, so its gradient is the folded upper triangle
+        const float Gv[9] = {vS[0][0], vS[0][1] + vS[1][0], vS[0][2] + vS[2][0],
+                             0.f,      vS[1][1],            vS[1][2] + vS[2][1],
+                             0.f,      0.f,                 vS[2][2]};
+        float gm[D], gV[D * D], go, gb[CQ];
+        cond_bwd_body<CQ>(st, bp.opac, vmean, Gv, v_opacities[idx], exact, gm, gV, &go, gb);
+#pragma unroll
+        for (int k = 0; k < D; ++k) gmean[k] += gm[k];
+#pragma unroll
+        for (int k = 0; k < D * D; ++k) gSig[k] += gV[k];
+        gop += go;
+        gbeta[0] += v_betas[idx];
+#pragma unroll
+        for (int k = 0; k < CQ; ++k) gbeta[1 + k] += gb[k];
+    }
+    // covariance, rotation and activation VJPs, once per primitive
+    float dR[9], dS[D];
+    covar_bwd_body<D>(bp.R, bp.scale, bp.L, bp.P, false, gSig, true, nullptr, FULL_M - 3, dR, dS,
+                      v_ltri + n * FULL_M);
+    if constexpr (QUAT) {
+        float qv[4], vq[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) qv[k] = quats[n * 4 + k];
+        quat_rot_vjp(qv, dR, vq);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v_quats[n * 4 + k] = vq[k];
+    } else {
+        // l_triagnle_to_rotmat_bwd.cu:16-25
+        v_ltri[n * FULL_M + 0] = dR[1] - dR[3];
+        v_ltri[n * FULL_M + 1] = dR[2] - dR[6];
+        v_ltri[n * FULL_M + 2] = dR[5] - dR[7];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) v_xyz[n * 3 + k] = gmean[k];
+#pragma unroll
+    for (int k = 3; k < D; ++k) v_mean_rest[n * (D - 3) + (k - 3)] = gmean[k];
+#pragma unroll
+    for (int k = 0; k < D; ++k) v_scale_raw[n * D + k] = dS[k] * softplus_grad(scale_raw[n * D + k]);
+    {
+        const float o = sigmoid_f(opacity_raw[n]);
+        v_opacity_raw[n] = gop * (o * (1.f - o));
+    }
+#pragma unroll
+    for (int k = 0; k < D - 2; ++k) {
+        const int64_t i = n * (D - 2) + k;
+        v_beta_raw[i] = gbeta[k] * (4.f * expf(beta_raw[i]));
+    }
+}
+
+}  // namespace ubs
+
+using namespace ubs;
+
+extern "C" {
+
+int ubs_beta_project_fwd(int64_t N, int64_t D, uint32_t C, const float *xyz, const float *mean_rest,
+                         const float *scale_raw, const float *l_triangle, const float *opacity_raw,
+                         const float *beta_raw, const float *quats, const float *cam_pos, const float *timestamps,
+                         const float *viewmats, const float *Ks, int32_t width, int32_t height,
+                         const int32_t *image_wh, float eps2d, float near_plane, float far_plane, float radius_clip,
+                         int ortho, int32_t *radii, float *means2d, float *depths, float *conics, float *opacities,
+                         float *betas, void *stream) {
+    UBS_REQUIRE(D == 6 || D == 7, "the fused Beta projection is defined for D = 6 or 7 (got %lld)", (long long)D);
+    UBS_REQUIRE(N >= 0 && (uint64_t)N * C < (1ull << 31), "C*N must be < 2^31");
+    if (N == 0 || C == 0) return UBS_OK;
+    UBS_REQUIRE(xyz && mean_rest && scale_raw && l_triangle && opacity_raw && beta_raw && cam_pos && viewmats &&
+                    Ks && radii && means2d && depths && conics && opacities && betas,
+                "null pointer");
+    hipStream_t st = as_stream(stream);
+    const unsigned grid = grid_for(N);
+#define UBS_BP_FWD(DD, QQ)                                                                                       \
+    beta_project_fwd_kernel<DD, QQ><<<grid, kThreads, 0, st>>>(                                                  \
+        N, C, xyz, mean_rest, scale_raw, l_triangle, opacity_raw, beta_raw, quats, cam_pos, timestamps, viewmats, \
+        Ks, width, height, image_wh, eps2d, near_plane, far_plane, radius_clip, ortho, radii, means2d, depths,    \
+        conics, opacities, betas)
+    if (D == 6) {
+        if (quats) UBS_BP_FWD(6, true);
+        else UBS_BP_FWD(6, false);
+    } else {
+        if (quats) UBS_BP_FWD(7, true);
+        else UBS_BP_FWD(7, false);
+    }
+#undef UBS_BP_FWD
+    UBS_LAUNCH_CHECK("beta_project_fwd_kernel");
+    return UBS_OK;
+}
+
+int ubs_beta_project_bwd(int64_t N, int64_t D, uint32_t C, const float *xyz, const float *mean_rest,
+                         const float *scale_raw, const float *l_triangle, const float *opacity_raw,
+                         const float *beta_raw, const float *quats, const float *cam_pos, const float *timestamps,
+                         const float *viewmats, const float *Ks, int32_t width, int32_t height,
+                         const int32_t *image_wh, float eps2d, int ortho, const int32_t *radii, const float *conics,
+                         const float *v_means2d, const float *v_depths, const float *v_conics,
+                         const float *v_opacities, const float *v_betas, float *v_xyz, float *v_mean_rest,
+                         float *v_scale_raw, float *v_l_triangle, float *v_opacity_raw, float *v_beta_raw,
+                         float *v_quats, void *stream) {
+    UBS_REQUIRE(D == 6 || D == 7, "the fused Beta projection is defined for D = 6 or 7 (got %lld)", (long long)D);
+    UBS_REQUIRE(N >= 0 && (uint64_t)N * C < (1ull << 31), "C*N must be < 2^31");
+    UBS_REQUIRE(!quats || v_quats, "quats given without v_quats");
+    if (N == 0) return UBS_OK;
+    UBS_REQUIRE(xyz && mean_rest && scale_raw && l_triangle && opacity_raw && beta_raw && v_xyz && v_mean_rest &&
+                    v_scale_raw && v_l_triangle && v_opacity_raw && v_beta_raw,
+                "null pointer");
+    UBS_REQUIRE(C == 0 || (cam_pos && viewmats && Ks && radii && conics && v_means2d && v_depths && v_conics &&
+                           v_opacities && v_betas),
+                "null pointer");
+    hipStream_t st = as_stream(stream);
+    const unsigned grid = grid_for(N);
+    const int exact = cond_backward_exact();
+#define UBS_BP_BWD(DD, QQ)                                                                                       \
+    beta_project_bwd_kernel<DD, QQ><<<grid, kThreads, 0, st>>>(                                                  \
+        N, C, xyz, mean_rest, scale_raw, l_triangle, opacity_raw, beta_raw, quats, cam_pos, timestamps, viewmats, \
+        Ks, width, height, image_wh, eps2d, ortho, exact, radii, conics, v_means2d, v_depths, v_conics,           \
+        v_opacities, v_betas, v_xyz, v_mean_rest, v_scale_raw, v_l_triangle, v_opacity_raw, v_beta_raw, v_quats)
+    if (D == 6) {
+        if (quats) UBS_BP_BWD(6, true);
+        else UBS_BP_BWD(6, false);
+    } else {
+        if (quats) UBS_BP_BWD(7, true);
+        else UBS_BP_BWD(7, false);
+    }
+#undef UBS_BP_BWD
+    UBS_LAUNCH_CHECK("beta_project_bwd_kernel");
+    return UBS_OK;
+}
+
+}  // extern "C"

@@ -1,5 +1,7 @@
 """Beta-model glue around the rasteriser: parameter activations, view/time
-query, conditioning and the rasterization() call.
+query, conditioning and the rasterization() call (render), and the fused
+multi-view path (render_views) that does all of it but the blend in one
+kernel per direction.
 
 Mirrors BetaModel.render (scene/beta_model.py:660-722) and its getters
 (:36-52, :103-159): scale = softplus, opacity = sigmoid, beta = 4 exp(b),
@@ -10,6 +12,7 @@ conditioning op, as in the reference.
 """
 from __future__ import annotations
 
+import math
 from dataclasses import dataclass
 from typing import Dict, Optional
 
@@ -19,8 +22,13 @@ from torch import Tensor
 
 from .cuda._wrapper import (
     beta_activations,
+    beta_project,
+    camera_centers,
     cond_mean_convariance_opacity,
+    isect_offset_encode,
+    isect_tiles,
     l_triangle_to_rotmat,
+    rasterize_to_pixels,
     rot_scale_l_triangle_to_covar,
 )
 from .rendering import rasterization
@@ -119,3 +127,43 @@ def render(params: BetaParams, viewmat: Tensor, K: Tensor, width: int, height: i
         means=m, l_triagnles=params.l_triangle, scales=scale, opacities=o, betas=beta0,
         colors=params.rgb, viewmats=viewmat[None], Ks=K[None], width=width, height=height,
         backgrounds=bg, covars=V, **raster_kwargs)
+
+
+def render_views(params: BetaParams, viewmats: Tensor, Ks: Tensor, width: int, height: int,
+                 timestamps: Optional[Tensor] = None, backgrounds: Optional[Tensor] = None,
+                 tile_size: int = 16, near_plane: float = 0.01, far_plane: float = 1e10,
+                 radius_clip: float = 0.0, eps2d: float = 0.3):
+    """C views of one scene in one pass: the fused Beta projection
+    (beta_project: activations, query, covariance, per-view conditioning and
+    projection in one launch), then the reference's intersection, sort and
+    blend over all C views.  Returns (render_colors [C,H,W,3],
+    render_alphas [C,H,W,1], meta) like rasterization(); with one view the
+    images and every leaf gradient equal render()'s bit for bit.
+
+    Each view is conditioned on its own query, which is what BetaModel.render
+    does one view at a time (scene/beta_model.py:660-722); gradients of all
+    views add up in the leaves."""
+    D = params.D
+    if D not in (6, 7):
+        raise NotImplementedError("Only implemented for 6D or 7D query")
+    C = viewmats.shape[0]
+    assert viewmats.shape == (C, 4, 4) and Ks.shape == (C, 3, 3), (viewmats.shape, Ks.shape)
+    with torch.no_grad():
+        cam_pos = camera_centers(viewmats.detach().to(torch.float32))
+    radii, means2d, depths, conics, opacities, betas = beta_project(
+        params.xyz, params.mean_rest, params.scale_raw, params.l_triangle, params.opacity_raw, params.beta_raw,
+        params.quats, viewmats, Ks, width, height, cam_pos=cam_pos, timestamps=timestamps, eps2d=eps2d,
+        near_plane=near_plane, far_plane=far_plane, radius_clip=radius_clip)
+    tw = math.ceil(width / float(tile_size))
+    th = math.ceil(height / float(tile_size))
+    tiles_per_gauss, isect_ids, flatten_ids = isect_tiles(means2d, radii, depths, tile_size, tw, th, n_cameras=C)
+    isect_offsets = isect_offset_encode(isect_ids, C, tw, th)
+    colors = params.rgb.expand(C, -1, -1)
+    render_colors, render_alphas = rasterize_to_pixels(
+        means2d, conics, colors, opacities, betas, width, height, tile_size, isect_offsets, flatten_ids,
+        backgrounds=backgrounds)
+    meta = dict(radii=radii, means2d=means2d, depths=depths, conics=conics, opacities=opacities, betas=betas,
+                tile_width=tw, tile_height=th, tiles_per_gauss=tiles_per_gauss, isect_ids=isect_ids,
+                flatten_ids=flatten_ids, isect_offsets=isect_offsets, width=width, height=height,
+                tile_size=tile_size, n_cameras=C)
+    return render_colors, render_alphas, meta

@@ -122,6 +122,60 @@ def rot_scale_l_triangle_to_covar(
         rot.contiguous(), scale.contiguous(), l_triangle.contiguous(), rest_i, rest_j, spatial_block)
 
 
+def camera_centers(viewmats: Tensor) -> Tensor:
+    """World positions [C,3] of world->camera [C,4,4] matrices, -(R^T t) per
+    view (the camera centre BetaModel's query is taken from)."""
+    return torch.stack([-(vm[:3, :3].transpose(0, 1) @ vm[:3, 3]) for vm in viewmats])
+
+
+def beta_project(
+    xyz: Tensor,  # [N, 3]
+    mean_rest: Tensor,  # [N, D-3]
+    scale_raw: Tensor,  # [N, D]
+    l_triangle: Tensor,  # [N, D(D-1)/2]
+    opacity_raw: Tensor,  # [N, 1]
+    beta_raw: Tensor,  # [N, D-2]
+    quats: Optional[Tensor],  # [N, 4] or None
+    viewmats: Tensor,  # [C, 4, 4]
+    Ks: Tensor,  # [C, 3, 3]
+    width: int,
+    height: int,
+    cam_pos: Optional[Tensor] = None,  # [C, 3]; camera_centers(viewmats) when None
+    timestamps: Optional[Tensor] = None,  # [C] (7-D); 0 when None
+    image_sizes: Optional[Tensor] = None,  # int32 [C, 2] (W, H) per view, else width x height
+    eps2d: float = 0.3,
+    near_plane: float = 0.01,
+    far_plane: float = 1e10,
+    radius_clip: float = 0.0,
+    ortho: bool = False,
+) -> Tuple[Tensor, Tensor, Tensor, Tensor, Tensor, Tensor]:
+    """Fused Beta projection of a 6-D or 7-D scene for C views, from the leaf
+    parameters (pre-activation, BetaModel's layout) to the per-view 2-D
+    primitives: (radii int32 [C,N], means2d [C,N,2], depths [C,N],
+    conics [C,N,3], opacities [C,N], betas [C,N]).
+
+    One launch does, per primitive, what BetaModel.render runs per view before
+    the blend (scene/beta_model.py:660-722): activations, query, rotation
+    (quaternion, or I + skew(l_triangle[:, :3]) when quats is None),
+    Sigma = L L^T, conditioning on the view's query, and the EWA projection of
+    rasterization() (rendering.py:54-101, covariances read as their upper
+    triangle).  With one view every output equals that chain's bit for bit;
+    gradients flow to every leaf, summed over the views."""
+    D = scale_raw.shape[1]
+    if D not in (6, 7):
+        raise NotImplementedError("Only implemented for 6D or 7D query")
+    if cam_pos is None:
+        with torch.no_grad():
+            cam_pos = camera_centers(viewmats.detach().float())
+    return _BetaProject.apply(
+        xyz.contiguous(), mean_rest.contiguous(), scale_raw.contiguous(), l_triangle.contiguous(),
+        opacity_raw.contiguous(), beta_raw.contiguous(), None if quats is None else quats.contiguous(),
+        viewmats.contiguous(), Ks.contiguous(), cam_pos.detach().contiguous(),
+        None if timestamps is None else timestamps.detach().float().contiguous(),
+        None if image_sizes is None else image_sizes.to(torch.int32).contiguous(), int(width), int(height),
+        float(eps2d), float(near_plane), float(far_plane), float(radius_clip), bool(ortho))
+
+
 def fully_fused_projection(
     means: Tensor,  # [N, 3]
     covars: Optional[Tensor],  # [N, 6] or None
@@ -586,9 +640,99 @@ def projection_bwd_raw(means, covars, quats, scales, viewmats, Ks, width, height
     return v_means, v_covars, v_quats, v_scales, v_viewmats
 
 
+def beta_project_fwd_raw(xyz, mean_rest, scale_raw, l_triangle, opacity_raw, beta_raw, quats, cam_pos, timestamps,
+                         viewmats, Ks, width, height, image_wh, eps2d, near_plane, far_plane, radius_clip, ortho):
+    """ubs_beta_project_fwd: (radii, means2d, depths, conics, opacities, betas), each [C,N,...]."""
+    require_cuda(xyz, mean_rest, scale_raw, l_triangle, opacity_raw, beta_raw, quats, cam_pos, timestamps,
+                 viewmats, Ks, image_wh)
+    require_dtype(_F32, xyz, mean_rest, scale_raw, l_triangle, opacity_raw, beta_raw, quats, cam_pos, timestamps,
+                  viewmats, Ks)
+    require_dtype(_I32, image_wh)
+    N, D = scale_raw.shape
+    C = viewmats.size(0)
+    assert xyz.shape == (N, 3) and mean_rest.shape == (N, D - 3), (xyz.shape, mean_rest.shape)
+    assert l_triangle.shape == (N, D * (D - 1) // 2), l_triangle.shape
+    assert opacity_raw.shape == (N, 1) and beta_raw.shape == (N, D - 2), (opacity_raw.shape, beta_raw.shape)
+    assert quats is None or quats.shape == (N, 4), quats.shape
+    assert viewmats.shape == (C, 4, 4) and Ks.shape == (C, 3, 3), (viewmats.shape, Ks.shape)
+    assert cam_pos.shape == (C, 3), cam_pos.shape
+    assert timestamps is None or timestamps.shape == (C,), timestamps.shape
+    assert image_wh is None or image_wh.shape == (C, 2), image_wh.shape
+    dev, dt = xyz.device, xyz.dtype
+    radii = torch.empty((C, N), dtype=torch.int32, device=dev)
+    means2d = torch.empty((C, N, 2), dtype=dt, device=dev)
+    depths = torch.empty((C, N), dtype=dt, device=dev)
+    conics = torch.empty((C, N, 3), dtype=dt, device=dev)
+    opacities = torch.empty((C, N), dtype=dt, device=dev)
+    betas = torch.empty((C, N), dtype=dt, device=dev)
+    _call("ubs_beta_project_fwd", N, D, C, ptr(xyz), ptr(mean_rest), ptr(scale_raw), ptr(l_triangle),
+          ptr(opacity_raw), ptr(beta_raw), ptr(quats), ptr(cam_pos), ptr(timestamps), ptr(viewmats), ptr(Ks),
+          width, height, ptr(image_wh), eps2d, near_plane, far_plane, radius_clip, int(ortho), ptr(radii),
+          ptr(means2d), ptr(depths), ptr(conics), ptr(opacities), ptr(betas), stream_of(xyz))
+    return radii, means2d, depths, conics, opacities, betas
+
+
+def beta_project_bwd_raw(xyz, mean_rest, scale_raw, l_triangle, opacity_raw, beta_raw, quats, cam_pos, timestamps,
+                         viewmats, Ks, width, height, image_wh, eps2d, ortho, radii, conics, v_means2d, v_depths,
+                         v_conics, v_opacities, v_betas):
+    """ubs_beta_project_bwd: (v_xyz, v_mean_rest, v_scale_raw, v_l_triangle, v_opacity_raw, v_beta_raw,
+    v_quats or None)."""
+    grads = [g.contiguous() for g in (v_means2d, v_depths, v_conics, v_opacities, v_betas)]
+    require_cuda(xyz, mean_rest, scale_raw, l_triangle, opacity_raw, beta_raw, quats, cam_pos, timestamps,
+                 viewmats, Ks, image_wh, radii, conics, *grads)
+    require_dtype(_F32, xyz, mean_rest, scale_raw, l_triangle, opacity_raw, beta_raw, quats, cam_pos, timestamps,
+                  viewmats, Ks, conics, *grads)
+    require_dtype(_I32, image_wh, radii)
+    N, D = scale_raw.shape
+    C = viewmats.size(0)
+    v_xyz = torch.empty_like(xyz)
+    v_mean_rest = torch.empty_like(mean_rest)
+    v_scale_raw = torch.empty_like(scale_raw)
+    v_l_triangle = torch.empty_like(l_triangle)
+    v_opacity_raw = torch.empty_like(opacity_raw)
+    v_beta_raw = torch.empty_like(beta_raw)
+    v_quats = torch.empty_like(quats) if quats is not None else None
+    _call("ubs_beta_project_bwd", N, D, C, ptr(xyz), ptr(mean_rest), ptr(scale_raw), ptr(l_triangle),
+          ptr(opacity_raw), ptr(beta_raw), ptr(quats), ptr(cam_pos), ptr(timestamps), ptr(viewmats), ptr(Ks),
+          width, height, ptr(image_wh), eps2d, int(ortho), ptr(radii), ptr(conics), *[ptr(g) for g in grads],
+          ptr(v_xyz), ptr(v_mean_rest), ptr(v_scale_raw), ptr(v_l_triangle), ptr(v_opacity_raw), ptr(v_beta_raw),
+          ptr(v_quats), stream_of(xyz))
+    return v_xyz, v_mean_rest, v_scale_raw, v_l_triangle, v_opacity_raw, v_beta_raw, v_quats
+
+
 # ----------------------------------------------------------------------------
 # autograd Functions (reference: _wrapper.py:573-684, 807-1050)
 # ----------------------------------------------------------------------------
+class _BetaProject(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, xyz, mean_rest, scale_raw, l_triangle, opacity_raw, beta_raw, quats, viewmats, Ks, cam_pos,
+                timestamps, image_wh, width, height, eps2d, near_plane, far_plane, radius_clip, ortho):
+        radii, means2d, depths, conics, opacities, betas = beta_project_fwd_raw(
+            xyz, mean_rest, scale_raw, l_triangle, opacity_raw, beta_raw, quats, cam_pos, timestamps, viewmats, Ks,
+            width, height, image_wh, eps2d, near_plane, far_plane, radius_clip, ortho)
+        ctx.save_for_backward(xyz, mean_rest, scale_raw, l_triangle, opacity_raw, beta_raw, quats, viewmats, Ks,
+                              cam_pos, timestamps, image_wh, radii, conics)
+        ctx.width, ctx.height, ctx.eps2d, ctx.ortho = width, height, eps2d, ortho
+        ctx.mark_non_differentiable(radii)
+        return radii, means2d, depths, conics, opacities, betas
+
+    @staticmethod
+    def backward(ctx, v_radii, v_means2d, v_depths, v_conics, v_opacities, v_betas):
+        (xyz, mean_rest, scale_raw, l_triangle, opacity_raw, beta_raw, quats, viewmats, Ks, cam_pos, timestamps,
+         image_wh, radii, conics) = ctx.saved_tensors
+        C, N = radii.shape
+
+        def g(t, shape):
+            return torch.zeros(shape, dtype=xyz.dtype, device=xyz.device) if t is None else t
+
+        grads = beta_project_bwd_raw(
+            xyz, mean_rest, scale_raw, l_triangle, opacity_raw, beta_raw, quats, cam_pos, timestamps, viewmats, Ks,
+            ctx.width, ctx.height, image_wh, ctx.eps2d, ctx.ortho, radii, conics, g(v_means2d, (C, N, 2)),
+            g(v_depths, (C, N)), g(v_conics, (C, N, 3)), g(v_opacities, (C, N)), g(v_betas, (C, N)))
+        out = [gr if need else None for gr, need in zip(grads, ctx.needs_input_grad[:7])]
+        return (*out, None, None, None, None, None, None, None, None, None, None, None, None)
+
+
 class _BetaActivations(torch.autograd.Function):
     @staticmethod
     def forward(ctx, xyz, mean_rest, scale_raw, opacity_raw, beta_raw, quats, cam_pos, timestamp):
