@@ -132,7 +132,7 @@ def render(params: BetaParams, viewmat: Tensor, K: Tensor, width: int, height: i
 def render_views(params: BetaParams, viewmats: Tensor, Ks: Tensor, width: int, height: int,
                  timestamps: Optional[Tensor] = None, backgrounds: Optional[Tensor] = None,
                  tile_size: int = 16, near_plane: float = 0.01, far_plane: float = 1e10,
-                 radius_clip: float = 0.0, eps2d: float = 0.3):
+                 radius_clip: float = 0.0, eps2d: float = 0.3, cam_pos: Optional[Tensor] = None):
     """C views of one scene in one pass: the fused Beta projection
     (beta_project: activations, query, covariance, per-view conditioning and
     projection in one launch), then the reference's intersection, sort and
@@ -142,14 +142,17 @@ def render_views(params: BetaParams, viewmats: Tensor, Ks: Tensor, width: int, h
 
     Each view is conditioned on its own query, which is what BetaModel.render
     does one view at a time (scene/beta_model.py:660-722); gradients of all
-    views add up in the leaves."""
+    views add up in the leaves.  cam_pos [C,3] are the camera centres the
+    queries are taken from (default: derived from viewmats)."""
     D = params.D
     if D not in (6, 7):
         raise NotImplementedError("Only implemented for 6D or 7D query")
     C = viewmats.shape[0]
     assert viewmats.shape == (C, 4, 4) and Ks.shape == (C, 3, 3), (viewmats.shape, Ks.shape)
     with torch.no_grad():
-        cam_pos = camera_centers(viewmats.detach().to(torch.float32))
+        if cam_pos is None:
+            cam_pos = camera_centers(viewmats.detach().to(torch.float32))
+        cam_pos = cam_pos.detach().to(torch.float32).contiguous()
     radii, means2d, depths, conics, opacities, betas = beta_project(
         params.xyz, params.mean_rest, params.scale_raw, params.l_triangle, params.opacity_raw, params.beta_raw,
         params.quats, viewmats, Ks, width, height, cam_pos=cam_pos, timestamps=timestamps, eps2d=eps2d,
