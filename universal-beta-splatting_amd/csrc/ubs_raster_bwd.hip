@@ -145,12 +145,12 @@ __global__ __launch_bounds__(TS *TS) void raster_bwd_kernel(
         // the same instructions with alpha = 0, (1 - sigma) = 1 and a zero
         // v_alpha, so they produce exact zeros and leave T and bufv unchanged.
         // Returns whether any lane of the wave contributes.
-        auto pixel_grads = [&](const int t, float (&v)[2 * NV], const int o) -> bool {
+        auto pixel_grads = [&](const int t, float (&v)[2 * NV], const int o, const bool on) -> bool {
             const float4 g1 = s_geo[t];
             const float4 g2 = s_geo2[t];
             const float dx = g1.x - px, dy = g1.y - py;
             const float sigma = __builtin_fmaf(2.f * g1.w * dx, dy, __builtin_fmaf(g1.z * dx, dx, (g2.x * dy) * dy));
-            const bool valid = t >= t_min && sigma >= 0.f && sigma < 1.f;
+            const bool valid = on && t >= t_min && sigma >= 0.f && sigma < 1.f;
             const float opac = g2.y, beta = g2.z;
             const float oms = valid ? 1.f - sigma : 1.f;
             const float lg = __builtin_amdgcn_logf(oms);  // log2(1 - sigma)
@@ -170,9 +170,10 @@ __global__ __launch_bounds__(TS *TS) void raster_bwd_kernel(
             const float fac = alpha * Tc;
             // sum_k (c_k T - buf_k ra) v_k = T (c . v) - ra (buf . v), with
             // bufv = buf . v carried as one running sum
-            float cdot = 0.f;
+            float cdot = s_col[0][t] * vrc[0];
+            v[o + 7] = fac * vrc[0];
 #pragma unroll
-            for (int q = 0; q < CDIM; ++q) {
+            for (int q = 1; q < CDIM; ++q) {
                 v[o + 7 + q] = fac * vrc[q];
                 cdot = __builtin_fmaf(s_col[q][t], vrc[q], cdot);
             }
@@ -208,13 +209,10 @@ __global__ __launch_bounds__(TS *TS) void raster_bwd_kernel(
                 const int t1 = two ? c0 + __builtin_ctzll(todo) : t0;
                 if (two) todo &= todo - 1ull;
                 float v[2 * NV];
-                bool any = pixel_grads(t0, v, 0);
-                if (two) {
-                    any = pixel_grads(t1, v, NV) || any;
-                } else {
-#pragma unroll
-                    for (int q = 0; q < NV; ++q) v[NV + q] = 0.f;
-                }
+                // a lone last primitive runs as a pair with a switched-off partner
+                // (exact zeros), which keeps the loop body free of branches
+                bool any = pixel_grads(t0, v, 0, true);
+                any = pixel_grads(t1, v, NV, two) || any;
                 if (!any) continue;
                 // wave totals, scattered over the four 16-lane quarters, added into
                 // the primitives' accumulators by one ds_add_f32

@@ -121,27 +121,33 @@ __global__ __launch_bounds__(TS *TS) void raster_fwd_kernel(
         }
         // (s_list[wave] is written and read by this wave only: LDS is in order)
         // ---- front-to-back blend over the compacted list, branch-free per
-        // pixel: lanes that skip the primitive blend alpha = 0
-        for (int k = 0; k < n_mine; ++k) {
-            if ((k & 7) == 0 && __ballot(!done) == 0ull) break;  // whole wave done
-            const int t = s_list[wave][k];
-            const float4 g1 = s_geo[t];
-            const float4 g2 = s_geo2[t];
-            const float dx = g1.x - px, dy = g1.y - py;
-            const float sigma = __builtin_fmaf(g1.w * dx, dy, __builtin_fmaf(g1.z * dx, dx, (g2.x * dy) * dy));
-            const bool ok = !done && sigma >= 0.f && sigma < 1.f;
-            if (__ballot(ok) == 0ull) continue;
-            const float a0 = fminf(0.999f, g2.y * beta_pow(ok ? 1.f - sigma : 1.f, g2.z));
-            const float alpha = ok ? a0 : 0.f;
-            const float nT = T * (1.f - alpha);
-            const bool stop = ok && nT <= 1e-4f;  // this primitive would end the pixel: excluded
-            const bool take = ok && !stop;
-            done = done || stop;
-            const float vis = take ? alpha * T : 0.f;
+        // pixel: lanes that skip the primitive blend alpha = 0.  The loop body
+        // has no branch at all (the all-done exit is checked once per 8
+        // primitives): skipping the body when no lane hits the primitive cost
+        // more in scalar mask bookkeeping than it saved (1.39 vs 1.18 ms at
+        // 1080p / 1M primitives)
+        for (int k0 = 0; k0 < n_mine; k0 += 8) {
+            if (__ballot(!done) == 0ull) break;  // whole wave done (checked every 8)
+            const int kend = min(k0 + 8, n_mine);
+            for (int k = k0; k < kend; ++k) {
+                const int t = s_list[wave][k];
+                const float4 g1 = s_geo[t];
+                const float4 g2 = s_geo2[t];
+                const float dx = g1.x - px, dy = g1.y - py;
+                const float sigma = __builtin_fmaf(g1.w * dx, dy, __builtin_fmaf(g1.z * dx, dx, (g2.x * dy) * dy));
+                const bool ok = !done && sigma >= 0.f && sigma < 1.f;
+                const float a0 = fminf(0.999f, g2.y * beta_pow(ok ? 1.f - sigma : 1.f, g2.z));
+                const float alpha = ok ? a0 : 0.f;
+                const float nT = T * (1.f - alpha);
+                const bool stop = ok && nT <= 1e-4f;  // this primitive would end the pixel: excluded
+                const bool take = ok && !stop;
+                done = done || stop;
+                const float vis = take ? alpha * T : 0.f;
 #pragma unroll
-            for (int q = 0; q < CDIM; ++q) acc[q] = __builtin_fmaf(s_col[q][t], vis, acc[q]);
-            cur = take ? batch_start + t : cur;
-            T = take ? nT : T;
+                for (int q = 0; q < CDIM; ++q) acc[q] = __builtin_fmaf(s_col[q][t], vis, acc[q]);
+                cur = take ? batch_start + t : cur;
+                T = take ? nT : T;
+            }
         }
     }
 
