@@ -43,9 +43,13 @@ __global__ __launch_bounds__(TS *TS) void raster_bwd_kernel(
     constexpr int NW = BS / 64;
     constexpr int NV = 7 + CDIM;  // xy(2) conic(3) opacity beta colour(CDIM)
     constexpr int NG2 = WaveTotals<2 * NV>::kGroups;
-    __shared__ float4 s_geo[BS];   // x, y, a, b
+    __shared__ float4 s_geo[BS];   // x, y, a, 2b
     __shared__ float4 s_geo2[BS];  // c, opacity, beta, id (bits)
-    __shared__ float s_col[CDIM][BS];
+    // up to 4 channels: one float4 per primitive at the stride of s_geo, so one
+    // address serves all three LDS reads of a primitive
+    constexpr bool kColVec = CDIM <= 4;
+    __shared__ float4 s_colv[kColVec ? BS : 1];
+    __shared__ float s_col[kColVec ? 1 : CDIM][kColVec ? 1 : BS];
     __shared__ uint32_t s_mask[BS];
     __shared__ float s_acc[NV][BS + 1];  // +1: the four quarter lanes hit distinct banks
     __shared__ int32_t s_wmax[NW];
@@ -85,6 +89,7 @@ __global__ __launch_bounds__(TS *TS) void raster_bwd_kernel(
     }
     const float vra = (inside && first_chunk) ? v_render_alphas[pix] : 0.f;
     const float vra_bg = vra - bg_dot;  // d/d(T_final) of the pixel's outputs
+    const float tf_vra = Tf * vra_bg;
     float bufv = 0.f;                   // sum over later primitives of c . v_render_color * alpha T
 
     // the largest last contributor over the wave bounds the useful range
@@ -124,11 +129,16 @@ __global__ __launch_bounds__(TS *TS) void raster_bwd_kernel(
                 ca = conics[(int64_t)g * 3]; cb = conics[(int64_t)g * 3 + 1]; cc = conics[(int64_t)g * 3 + 2];
                 op = opacities[g]; be = betas[g];
             }
-            s_geo[tid] = make_float4(xy.x, xy.y, ca, cb);
+            s_geo[tid] = make_float4(xy.x, xy.y, ca, 2.f * cb);  // 2b, as the forward
             s_geo2[tid] = make_float4(cc, op, be, __int_as_float(g));
             const float *cp = colors + (int64_t)g * channels + coff;
+            if constexpr (kColVec) {
+                s_colv[tid] = make_float4(cp[0], CDIM > 1 ? cp[1] : 0.f, CDIM > 2 ? cp[2] : 0.f,
+                                          CDIM > 3 ? cp[3] : 0.f);
+            } else {
 #pragma unroll
-            for (int k = 0; k < CDIM; ++k) s_col[k][tid] = cp[k];
+                for (int k = 0; k < CDIM; ++k) s_col[k][tid] = cp[k];
+            }
             float hx, hy;
             support_box(ca, cb, cc, hx, hy);
             s_mask[tid] = subblock_mask_exact<SB>(xy.x, xy.y, ca, cb, cc, hx, hy, x0, y0);
@@ -144,12 +154,12 @@ __global__ __launch_bounds__(TS *TS) void raster_bwd_kernel(
         // advances T and bufv.  Branch-free: lanes without a contribution run
         // the same instructions with alpha = 0, (1 - sigma) = 1 and a zero
         // v_alpha, so they produce exact zeros and leave T and bufv unchanged.
-        // Returns whether any lane of the wave contributes.
+        // Returns whether this pixel contributes.
         auto pixel_grads = [&](const int t, float (&v)[2 * NV], const int o, const bool on) -> bool {
             const float4 g1 = s_geo[t];
             const float4 g2 = s_geo2[t];
             const float dx = g1.x - px, dy = g1.y - py;
-            const float sigma = __builtin_fmaf(2.f * g1.w * dx, dy, __builtin_fmaf(g1.z * dx, dx, (g2.x * dy) * dy));
+            const float sigma = __builtin_fmaf(g1.w * dx, dy, __builtin_fmaf(g1.z * dx, dx, (g2.x * dy) * dy));
             const bool valid = on && t >= t_min && sigma >= 0.f && sigma < 1.f;
             const float opac = g2.y, beta = g2.z;
             const float oms = valid ? 1.f - sigma : 1.f;
@@ -170,15 +180,25 @@ __global__ __launch_bounds__(TS *TS) void raster_bwd_kernel(
             const float fac = alpha * Tc;
             // sum_k (c_k T - buf_k ra) v_k = T (c . v) - ra (buf . v), with
             // bufv = buf . v carried as one running sum
-            float cdot = s_col[0][t] * vrc[0];
+            float col[CDIM];
+            if constexpr (kColVec) {
+                const float4 cv = s_colv[t];
+                const float c4[4] = {cv.x, cv.y, cv.z, cv.w};
+#pragma unroll
+                for (int q = 0; q < CDIM; ++q) col[q] = c4[q];
+            } else {
+#pragma unroll
+                for (int q = 0; q < CDIM; ++q) col[q] = s_col[q][t];
+            }
+            float cdot = col[0] * vrc[0];
             v[o + 7] = fac * vrc[0];
 #pragma unroll
             for (int q = 1; q < CDIM; ++q) {
                 v[o + 7 + q] = fac * vrc[q];
-                cdot = __builtin_fmaf(s_col[q][t], vrc[q], cdot);
+                cdot = __builtin_fmaf(col[q], vrc[q], cdot);
             }
-            float v_alpha = __builtin_fmaf(Tc, cdot, -(ra * bufv));
-            v_alpha = __builtin_fmaf(Tf * ra, vra_bg, v_alpha);
+            // T (c . v) - ra (buf . v) + ra T_final (v_alpha_out - bg . v)
+            const float v_alpha = __builtin_fmaf(Tc, cdot, ra * (tf_vra - bufv));
             bufv = __builtin_fmaf(cdot, fac, bufv);
             // geometry, opacity and beta get gradient only below the 0.999 clamp
             const float va = (valid && ov <= 0.999f) ? v_alpha : 0.f;
@@ -191,11 +211,13 @@ __global__ __launch_bounds__(TS *TS) void raster_bwd_kernel(
             v[o + 2] = u * dx;                                // x -opac beta
             v[o + 3] = u * dy;                                // x -2 opac beta
             v[o + 4] = w * dy;                                // x -opac beta
-            v[o + 0] = __builtin_fmaf(g1.z, u, g1.w * w);     // x -2 opac beta
-            v[o + 1] = __builtin_fmaf(g1.w, u, g2.x * w);     // x -2 opac beta
+            // v_xy = (a u + b w, b u + c w) summed over the pixels: the conic is
+            // per primitive, so the sums of u and w suffice (combined at the flush)
+            v[o + 0] = u;
+            v[o + 1] = w;
             v[o + 5] = vis * va;
             v[o + 6] = v[o + 5] * lg;                         // x opac ln 2 (lg = log2(1 - sigma))
-            return __ballot(valid) != 0ull;
+            return valid;
         };
         for (int c0 = 0; c0 < cnt; c0 += 64) {
             const int e = c0 + lane;
@@ -211,8 +233,9 @@ __global__ __launch_bounds__(TS *TS) void raster_bwd_kernel(
                 float v[2 * NV];
                 // a lone last primitive runs as a pair with a switched-off partner
                 // (exact zeros), which keeps the loop body free of branches
-                bool any = pixel_grads(t0, v, 0, true);
-                any = pixel_grads(t1, v, NV, two) || any;
+                const bool c0v = pixel_grads(t0, v, 0, true);
+                const bool c1v = pixel_grads(t1, v, NV, two);
+                const bool any = __ballot(c0v || c1v) != 0ull;
                 if (!any) continue;
                 // wave totals, scattered over the four 16-lane quarters, added into
                 // the primitives' accumulators by one ds_add_f32
@@ -239,10 +262,13 @@ __global__ __launch_bounds__(TS *TS) void raster_bwd_kernel(
                 s_acc[q][tid] = 0.f;
             }
             {
+                const float4 gg = s_geo[tid];   // x, y, a, 2b
                 const float4 gp = s_geo2[tid];  // c, opacity, beta, id
                 const float k1 = -gp.y * gp.z, k2 = 2.f * k1;
-                a[0] *= k2;
-                a[1] *= k2;
+                const float su = a[0], sw = a[1];
+                const float b = 0.5f * gg.w;
+                a[0] = __builtin_fmaf(gg.z, su, b * sw) * k2;
+                a[1] = __builtin_fmaf(b, su, gp.x * sw) * k2;
                 a[2] *= k1;
                 a[3] *= k2;
                 a[4] *= k1;

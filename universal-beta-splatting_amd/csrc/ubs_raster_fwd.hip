@@ -31,9 +31,13 @@ __global__ __launch_bounds__(TS *TS) void raster_fwd_kernel(
     constexpr int NW = BS / 64;        // waves
     __shared__ float4 s_geo[BS];       // x, y, a, 2b
     __shared__ float4 s_geo2[BS];      // c, opacity, beta, (unused)
-    __shared__ float s_col[CDIM][BS];
+    // up to 4 channels: one float4 per primitive, addressed with the same
+    // byte offset as s_geo / s_geo2 (no per-primitive address arithmetic)
+    constexpr bool kColVec = CDIM <= 4;
+    __shared__ float4 s_colv[kColVec ? BS : 1];
+    __shared__ float s_col[kColVec ? 1 : CDIM][kColVec ? 1 : BS];
     __shared__ uint32_t s_mask[BS];
-    __shared__ uint16_t s_list[NW][BS];
+    __shared__ uint16_t s_list[NW][BS];  // byte offsets (16 x entry) of the wave's primitives
 
     uint32_t logical;
     if (!xcd_tile(blockIdx.x, C, tw, th, logical)) return;
@@ -98,8 +102,13 @@ __global__ __launch_bounds__(TS *TS) void raster_fwd_kernel(
             s_geo[tid] = make_float4(xy.x, xy.y, ca, 2.f * cb);
             s_geo2[tid] = make_float4(cc, op, be, 0.f);
             const float *cp = colors + (int64_t)g * channels + coff;
+            if constexpr (kColVec) {
+                s_colv[tid] = make_float4(cp[0], CDIM > 1 ? cp[1] : 0.f, CDIM > 2 ? cp[2] : 0.f,
+                                          CDIM > 3 ? cp[3] : 0.f);
+            } else {
 #pragma unroll
-            for (int k = 0; k < CDIM; ++k) s_col[k][tid] = cp[k];
+                for (int k = 0; k < CDIM; ++k) s_col[k][tid] = cp[k];
+            }
             float hx, hy;
             support_box(ca, cb, cc, hx, hy);
             s_mask[tid] = subblock_mask_exact<SB>(xy.x, xy.y, ca, cb, cc, hx, hy, x0, y0);
@@ -115,7 +124,7 @@ __global__ __launch_bounds__(TS *TS) void raster_fwd_kernel(
             const uint64_t bal = __ballot(hit);
             if (hit) {
                 const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-                s_list[wave][n_mine + __popcll(bal & lt)] = (uint16_t)e;
+                s_list[wave][n_mine + __popcll(bal & lt)] = (uint16_t)(e * 16);
             }
             n_mine += __popcll(bal);
         }
@@ -126,29 +135,39 @@ __global__ __launch_bounds__(TS *TS) void raster_fwd_kernel(
         // primitives): skipping the body when no lane hits the primitive cost
         // more in scalar mask bookkeeping than it saved (1.39 vs 1.18 ms at
         // 1080p / 1M primitives)
+        int cur_off = -1;  // byte offset of this batch's last contributor
         for (int k0 = 0; k0 < n_mine; k0 += 8) {
             if (__ballot(!done) == 0ull) break;  // whole wave done (checked every 8)
             const int kend = min(k0 + 8, n_mine);
             for (int k = k0; k < kend; ++k) {
-                const int t = s_list[wave][k];
-                const float4 g1 = s_geo[t];
-                const float4 g2 = s_geo2[t];
+                const int off = s_list[wave][k];
+                const float4 g1 = *reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(s_geo) + off);
+                const float4 g2 = *reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(s_geo2) + off);
                 const float dx = g1.x - px, dy = g1.y - py;
                 const float sigma = __builtin_fmaf(g1.w * dx, dy, __builtin_fmaf(g1.z * dx, dx, (g2.x * dy) * dy));
                 const bool ok = !done && sigma >= 0.f && sigma < 1.f;
-                const float a0 = fminf(0.999f, g2.y * beta_pow(ok ? 1.f - sigma : 1.f, g2.z));
+                // outside [0, 1) the power is NaN or > 1 and the select drops it
+                const float a0 = fminf(0.999f, g2.y * beta_pow(1.f - sigma, g2.z));
                 const float alpha = ok ? a0 : 0.f;
                 const float nT = T * (1.f - alpha);
                 const bool stop = ok && nT <= 1e-4f;  // this primitive would end the pixel: excluded
                 const bool take = ok && !stop;
                 done = done || stop;
                 const float vis = take ? alpha * T : 0.f;
+                if constexpr (kColVec) {
+                    const float4 cv = *reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(s_colv) + off);
+                    const float c4[4] = {cv.x, cv.y, cv.z, cv.w};
 #pragma unroll
-                for (int q = 0; q < CDIM; ++q) acc[q] = __builtin_fmaf(s_col[q][t], vis, acc[q]);
-                cur = take ? batch_start + t : cur;
+                    for (int q = 0; q < CDIM; ++q) acc[q] = __builtin_fmaf(c4[q], vis, acc[q]);
+                } else {
+#pragma unroll
+                    for (int q = 0; q < CDIM; ++q) acc[q] = __builtin_fmaf(s_col[q][off >> 4], vis, acc[q]);
+                }
+                cur_off = take ? off : cur_off;
                 T = take ? nT : T;
             }
         }
+        if (cur_off >= 0) cur = batch_start + (cur_off >> 4);
     }
 
     if (inside) {
