@@ -54,9 +54,11 @@ def algorithmic_bytes(name: str, c: dict, n_isect: int, C: int = 1) -> float:
         return 44.0 * n_isect + 24.0 * pix + 4.0 * C * tiles
     if name == "ubs_rasterize_to_pixels_bwd":
         return 44.0 * n_isect + 24.0 * pix + 40.0 * n_isect
-    if name == "ubs_radix_sort_pairs":
+    if name in ("ubs_radix_sort_pairs", "ubs_radix_sort_pairs_skip"):
         end_bit = 32 + int(tiles).bit_length() + int(C).bit_length()
         passes = math.ceil(end_bit / 8)
+        if name.endswith("_skip"):  # passes over depth bytes every key shares are not run
+            passes -= c.get("sort_skipped", 0)
         return n_isect * (8.0 + 24.0 * passes)
     if name == "ubs_isect_offset_encode":
         return 8.0 * n_isect + 4.0 * C * tiles
@@ -214,6 +216,9 @@ def main():
     torch.cuda.synchronize()
     n_isect = int(meta["isect_ids"].numel())
     n_vis = int((meta["radii"] > 0).sum().item())
+    dbits = meta["depths"][meta["radii"] > 0].contiguous().view(torch.int32).to(torch.int64) & 0xFFFFFFFF
+    lo, hi = int(dbits.min()), int(dbits.max())
+    cfg = dict(cfg, sort_skipped=bin(W.constant_depth_passes((hi << 32) | (~lo & 0xFFFFFFFF))).count("1"))
 
     # ---- timed region: K steps; per-launch HIP events on the launch stream
     timer = W.LaunchTimer()
@@ -280,7 +285,8 @@ def main():
         "data": "synthetic (seeded scene and camera generator, gsplat/scenes.py)",
         "config": {"workload": cfg["workload"], "primitives": cfg["N"], "dims": cfg["D"],
                    "width": cfg["W"], "height": cfg["H"], "views": 1, "tile_size": 16,
-                   "n_isects": n_isect, "visible_primitives": n_vis, "projection_path": args.path},
+                   "n_isects": n_isect, "visible_primitives": n_vis, "projection_path": args.path,
+                   "sort_passes_skipped": cfg["sort_skipped"]},
         "fwd_frames_per_s": round(1e3 / fwd_ms, 3),
         "fwd_bwd_mpix_per_s": round(cfg["W"] * cfg["H"] * 1e3 / ms_per_step / 1e6, 2),
         "latency_ms": {"p50": round(step_ms[len(step_ms) // 2], 3),

@@ -237,6 +237,19 @@ int ubs_isect_tiles_emit(uint32_t C, uint32_t N, const float *means2d, const int
                          uint32_t tile_width, uint32_t tile_height, int64_t *isect_ids,
                          int32_t *flatten_ids, void *stream);
 
+/* Pass 1 that also writes the bit range of the depths [C,N] of the entries
+ * with radii > 0, as uint32 bit patterns, into device memory range_dev[2]:
+ * range_dev[0] = ~min, range_dev[1] = max (both 0 when nothing is visible).
+ * Depth bytes on which min and max agree are the same in every isect id, so
+ * the sort can skip their passes (ubs_radix_sort_pairs_skip).  range_dev may
+ * be NULL (then depths is unused).  No counterpart in the reference (it sorts
+ * all 32 + prefix bits, isect_tiles.cu:224-278). */
+int ubs_isect_tiles_count_range(uint32_t C, uint32_t N, const float *means2d, const int32_t *radii,
+                                const float *depths, uint32_t tile_size, uint32_t tile_width,
+                                uint32_t tile_height, int32_t *tiles_per_gauss, int64_t *cum_tiles,
+                                int64_t *n_isects_dev, uint32_t *range_dev, void *workspace,
+                                size_t workspace_bytes, void *stream);
+
 /* Bits the (cam, tile) prefix of an isect id occupies: 32 + this = end_bit. */
 uint32_t ubs_isect_prefix_bits(uint32_t C, uint32_t tile_width, uint32_t tile_height);
 
@@ -251,6 +264,14 @@ size_t ubs_radix_sort_workspace_bytes(int64_t n, int begin_bit, int end_bit);
 int ubs_radix_sort_pairs(int64_t n, int64_t *keys, int64_t *keys_alt, int32_t *vals,
                          int32_t *vals_alt, int begin_bit, int end_bit, void *workspace,
                          size_t workspace_bytes, int *result_in_alt, void *stream);
+/* The same sort without the passes whose bit set in skip_passes (bit p =
+ * key bits [begin_bit + 8p, begin_bit + 8p + 8)): the caller asserts that
+ * every key has the same digit there, so the pass would be the identity (the
+ * isect path knows it for the high bytes of the depths from
+ * ubs_isect_tiles_count_range).  No counterpart in the reference. */
+int ubs_radix_sort_pairs_skip(int64_t n, int64_t *keys, int64_t *keys_alt, int32_t *vals,
+                              int32_t *vals_alt, int begin_bit, int end_bit, uint32_t skip_passes,
+                              void *workspace, size_t workspace_bytes, int *result_in_alt, void *stream);
 /* Reads the sort's device-side status word (0 = fine, nonzero = a look-back
  * wait hit its spin limit and the result is invalid).  Synchronises stream. */
 int ubs_radix_sort_status(const void *workspace, int *status, void *stream);

@@ -126,3 +126,23 @@ def test_c_shim_loads_under_another_package_name():
     finally:
         for k in [k for k in sys.modules if k.startswith("ubs_shim_probe_pkg")]:
             del sys.modules[k]
+
+
+def test_constant_depth_passes_host_logic():
+    """Which depth-byte sort passes isect_tiles skips, from the packed
+    (~min, max) pair: byte p is constant iff min and max agree on bits >= 8p."""
+    import struct
+
+    from gsplat.cuda._wrapper import constant_depth_passes
+
+    def pack(lo_f, hi_f):
+        lo = struct.unpack("<I", struct.pack("<f", lo_f))[0]
+        hi = struct.unpack("<I", struct.pack("<f", hi_f))[0]
+        v = ((hi << 32) | (~lo & 0xFFFFFFFF))
+        return v - (1 << 64) if v >= (1 << 63) else v  # as the int64 the device writes
+
+    assert constant_depth_passes(pack(2.5, 5.5)) == 0b1000  # 0x40200000 .. 0x40B00000
+    assert constant_depth_passes(pack(3.0, 3.0)) == 0b1111
+    assert constant_depth_passes(pack(0.5, 5.5)) == 0b0000  # 0x3F000000 .. 0x40B00000
+    assert constant_depth_passes(pack(3.0, 3.0000002)) == 0b1110  # differ in the lowest byte only
+    assert constant_depth_passes(0) == 0  # nothing visible: ~min = 0, max = 0 -> min > max

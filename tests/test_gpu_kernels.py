@@ -238,6 +238,50 @@ def test_radix_sort_stable(device, n, end_bit):
     assert_same(host(got_v), want_v, "values")
 
 
+@pytest.mark.parametrize("n", [5000, 100003])
+@pytest.mark.parametrize("const_bytes", [(3,), (1, 3), (0, 1, 2, 3)])
+def test_radix_sort_skips_constant_passes(device, n, const_bytes):
+    """Leaving out passes over digits every key shares gives the full sort's
+    result (checked against the oracle and against no skipping)."""
+    g = np.random.Generator(np.random.PCG64(n + 7 * len(const_bytes)))
+    keys = g.integers(0, 1 << 46, size=n, dtype=np.int64) & ~np.int64(0x3F)  # ties in the low bits
+    for b in const_bytes:
+        keys = (keys & ~np.int64(0xFF << (8 * b))) | np.int64(0x5A << (8 * b))
+    vals = np.arange(n, dtype=np.int32)
+    mask = sum(1 << b for b in const_bytes)
+    want_k, want_v = O.sort_pairs(keys, vals, 0, 46)
+    got_k, got_v = W.radix_sort_pairs(cu(keys), cu(vals), 0, 46, check_status=True, skip_passes=mask)
+    assert_same(host(got_k), want_k, "keys")
+    assert_same(host(got_v), want_v, "values")
+    full_k, full_v = W.radix_sort_pairs(cu(keys), cu(vals), 0, 46)
+    assert torch.equal(got_k, full_k) and torch.equal(got_v, full_v)
+
+
+def test_isect_depth_range(device):
+    g = np.random.Generator(np.random.PCG64(3))
+    C, N = 2, 5000
+    radii = g.integers(-1, 5, (C, N)).astype(np.int32)
+    depths = g.uniform(2.5, 5.5, (C, N)).astype(np.float32)
+    depths[radii <= 0] = 1e9  # culled entries must not widen the range
+    means2d = g.uniform(0, 200, (C, N, 2)).astype(np.float32)
+    info = torch.zeros(2, dtype=torch.int64, device="cuda")
+    m_d, r_d, d_d = cu(means2d), cu(radii), cu(depths)  # held: the call takes raw pointers
+    tpg = torch.empty((C, N), dtype=torch.int32, device="cuda")
+    cum = torch.empty((C * N,), dtype=torch.int64, device="cuda")
+    L = W.lib()
+    ws_bytes = int(L.ubs_isect_count_workspace_bytes(C, N))
+    ws = torch.empty((ws_bytes,), dtype=torch.uint8, device="cuda")
+    L.check(L.ubs_isect_tiles_count_range(C, N, W.ptr(m_d), W.ptr(r_d), W.ptr(d_d), 16, 13, 13, W.ptr(tpg),
+                                          W.ptr(cum), W.ptr(info), W._vp_at(info, 1), W.ptr(ws), ws_bytes,
+                                          W.stream_of(info)), "count_range")
+    n_isects, packed = (int(x) for x in info.cpu())
+    assert n_isects == int(tpg.sum())
+    bits = depths[radii > 0].view(np.uint32)
+    assert (~packed) & 0xFFFFFFFF == int(bits.min()) and (packed >> 32) & 0xFFFFFFFF == int(bits.max())
+    # 2.5 .. 5.5 share their top byte (0x40) only
+    assert W.constant_depth_passes(packed) == 0b1000
+
+
 # --------------------------------------------------------------------- raster
 def _raster_inputs(scene_fn, W_, H_, C=1, ts=16, channels=3, **kw):
     s = scene_fn(W=W_, H=H_, n_cams=C, **kw)

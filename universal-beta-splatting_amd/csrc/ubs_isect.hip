@@ -56,13 +56,28 @@ __device__ __forceinline__ int64_t block_exclusive_scan(int64_t v, int64_t *lds_
     return base + inc - v;
 }
 
-// Pass 1a: tiles_per_gauss and one int64 partial sum per block.
+// max over the block of one uint32 per thread (256 threads); thread 0 gets it
+__device__ __forceinline__ uint32_t block_max_u32(uint32_t v, uint32_t *lds_wave) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
+    if (lane_id() == 0) lds_wave[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int w = 1; w < kThreads / 64; ++w) v = max(v, lds_wave[w]);
+    return v;
+}
+
+// Pass 1a: tiles_per_gauss and one int64 partial sum per block; with depths,
+// also the block's bit range of the visible depths as uint32 patterns
+// (block_range[2b] = ~min, [2b+1] = max; ~min so that a max serves both).
 __global__ void isect_count_kernel(uint64_t total, const float *__restrict__ means2d,
-                                   const int32_t *__restrict__ radii, uint32_t ts, uint32_t tw, uint32_t th,
-                                   int32_t *__restrict__ tpg, int64_t *__restrict__ block_sums) {
+                                   const int32_t *__restrict__ radii, const float *__restrict__ depths,
+                                   uint32_t ts, uint32_t tw, uint32_t th, int32_t *__restrict__ tpg,
+                                   int64_t *__restrict__ block_sums, uint32_t *__restrict__ block_range) {
     __shared__ int64_t lds_wave[kThreads / 64];
+    __shared__ uint32_t lds_max[kThreads / 64];
     const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
     int64_t s = 0;
+    uint32_t lo_n = 0u, hi = 0u;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
         const uint64_t idx = base + (uint64_t)k * kThreads + threadIdx.x;  // coalesced
@@ -70,18 +85,48 @@ __global__ void isect_count_kernel(uint64_t total, const float *__restrict__ mea
             const int32_t n = tile_count(means2d, radii, idx, ts, tw, th);
             tpg[idx] = n;
             s += n;
+            if (depths && radii[idx] > 0) {
+                const uint32_t b = __float_as_uint(depths[idx]);
+                lo_n = max(lo_n, ~b);
+                hi = max(hi, b);
+            }
         }
     }
     int64_t tot;
     block_exclusive_scan(s, lds_wave, tot);
     if (threadIdx.x == 0) block_sums[blockIdx.x] = tot;
+    if (depths) {  // block-uniform
+        lo_n = block_max_u32(lo_n, lds_max);
+        __syncthreads();
+        hi = block_max_u32(hi, lds_max);
+        if (threadIdx.x == 0) {
+            block_range[2 * blockIdx.x] = lo_n;
+            block_range[2 * blockIdx.x + 1] = hi;
+        }
+    }
 }
 
 // Pass 1b: exclusive scan of the block sums in one block; the grand total
-// goes to *n_isects.
-__global__ void isect_scan_blocks_kernel(int64_t nb, int64_t *__restrict__ block_sums, int64_t *__restrict__ n_isects) {
+// goes to *n_isects, and the depth range of all blocks to range[0..1].
+__global__ void isect_scan_blocks_kernel(int64_t nb, int64_t *__restrict__ block_sums, int64_t *__restrict__ n_isects,
+                                         const uint32_t *__restrict__ block_range, uint32_t *__restrict__ range) {
     __shared__ int64_t lds_wave[kThreads / 64];
     __shared__ int64_t carry;
+    __shared__ uint32_t lds_max[kThreads / 64];
+    if (range) {  // grid-uniform
+        uint32_t lo_n = 0u, hi = 0u;
+        for (int64_t i = threadIdx.x; i < nb; i += kThreads) {
+            lo_n = max(lo_n, block_range[2 * i]);
+            hi = max(hi, block_range[2 * i + 1]);
+        }
+        lo_n = block_max_u32(lo_n, lds_max);
+        __syncthreads();
+        hi = block_max_u32(hi, lds_max);
+        if (threadIdx.x == 0) {
+            range[0] = lo_n;
+            range[1] = hi;
+        }
+    }
     if (threadIdx.x == 0) carry = 0;
     __syncthreads();
     for (int64_t start = 0; start < nb; start += kThreads) {
@@ -177,7 +222,8 @@ extern "C" {
 size_t ubs_isect_count_workspace_bytes(uint32_t C, uint32_t N) {
     const uint64_t total = (uint64_t)C * N;
     const uint64_t nb = (total + kScanTile - 1) / kScanTile;
-    return (size_t)std::max<uint64_t>(nb, 1) * sizeof(int64_t);
+    // per block: the int64 partial sum, then the (~min, max) depth bits
+    return (size_t)std::max<uint64_t>(nb, 1) * (sizeof(int64_t) + 2 * sizeof(uint32_t));
 }
 
 uint32_t ubs_isect_prefix_bits(uint32_t C, uint32_t tw, uint32_t th) {
@@ -187,11 +233,24 @@ uint32_t ubs_isect_prefix_bits(uint32_t C, uint32_t tw, uint32_t th) {
 int ubs_isect_tiles_count(uint32_t C, uint32_t N, const float *means2d, const int32_t *radii, uint32_t ts,
                           uint32_t tw, uint32_t th, int32_t *tpg, int64_t *cum, int64_t *n_isects_dev,
                           void *workspace, size_t ws_bytes, void *stream) {
+    return ubs_isect_tiles_count_range(C, N, means2d, radii, nullptr, ts, tw, th, tpg, cum, n_isects_dev, nullptr,
+                                       workspace, ws_bytes, stream);
+}
+
+int ubs_isect_tiles_count_range(uint32_t C, uint32_t N, const float *means2d, const int32_t *radii,
+                                const float *depths, uint32_t ts, uint32_t tw, uint32_t th, int32_t *tpg,
+                                int64_t *cum, int64_t *n_isects_dev, uint32_t *range_dev, void *workspace,
+                                size_t ws_bytes, void *stream) {
     hipStream_t st = as_stream(stream);
     const uint64_t total = (uint64_t)C * N;
     UBS_REQUIRE(n_isects_dev, "n_isects_dev is null");
     UBS_REQUIRE(ts > 0, "tile_size must be > 0");
     UBS_REQUIRE(bit_width_u32(tw * th) + bit_width_u32(C) <= 32, "camera and tile ids need more than 32 bits");
+    UBS_REQUIRE(!range_dev || depths, "range_dev needs depths");
+    if (range_dev) {
+        int e = zero_async(range_dev, 2 * sizeof(uint32_t), st);
+        if (e) return e;
+    }
     if (total == 0) return zero_async(n_isects_dev, sizeof(int64_t), st);
     UBS_REQUIRE(total < (1ull << 31), "C*N must be < 2^31");
     UBS_REQUIRE(means2d && radii && tpg && cum, "null pointer");
@@ -201,9 +260,11 @@ int ubs_isect_tiles_count(uint32_t C, uint32_t N, const float *means2d, const in
         return UBS_ERR_WORKSPACE;
     }
     int64_t *bs = reinterpret_cast<int64_t *>(workspace);
-    isect_count_kernel<<<(unsigned)nb, kThreads, 0, st>>>(total, means2d, radii, ts, tw, th, tpg, bs);
+    uint32_t *br = reinterpret_cast<uint32_t *>(bs + nb);
+    const float *dp = range_dev ? depths : nullptr;
+    isect_count_kernel<<<(unsigned)nb, kThreads, 0, st>>>(total, means2d, radii, dp, ts, tw, th, tpg, bs, br);
     UBS_LAUNCH_CHECK("isect_count_kernel");
-    isect_scan_blocks_kernel<<<1, kThreads, 0, st>>>((int64_t)nb, bs, n_isects_dev);
+    isect_scan_blocks_kernel<<<1, kThreads, 0, st>>>((int64_t)nb, bs, n_isects_dev, br, range_dev);
     UBS_LAUNCH_CHECK("isect_scan_blocks_kernel");
     isect_cum_kernel<<<(unsigned)nb, kThreads, 0, st>>>(total, tpg, bs, cum);
     UBS_LAUNCH_CHECK("isect_cum_kernel");

@@ -270,6 +270,13 @@ size_t ubs_radix_sort_workspace_bytes(int64_t n, int begin_bit, int end_bit) {
 int ubs_radix_sort_pairs(int64_t n, int64_t *keys, int64_t *keys_alt, int32_t *vals, int32_t *vals_alt,
                          int begin_bit, int end_bit, void *workspace, size_t ws_bytes, int *result_in_alt,
                          void *stream) {
+    return ubs_radix_sort_pairs_skip(n, keys, keys_alt, vals, vals_alt, begin_bit, end_bit, 0u, workspace,
+                                     ws_bytes, result_in_alt, stream);
+}
+
+int ubs_radix_sort_pairs_skip(int64_t n, int64_t *keys, int64_t *keys_alt, int32_t *vals, int32_t *vals_alt,
+                              int begin_bit, int end_bit, uint32_t skip_passes, void *workspace, size_t ws_bytes,
+                              int *result_in_alt, void *stream) {
     hipStream_t st = as_stream(stream);
     UBS_REQUIRE(result_in_alt, "result_in_alt is null");
     UBS_REQUIRE(begin_bit >= 0 && end_bit <= 64 && begin_bit <= end_bit, "bad bit range [%d, %d)", begin_bit,
@@ -298,15 +305,20 @@ int ubs_radix_sort_pairs(int64_t n, int64_t *keys, int64_t *keys_alt, int32_t *v
     UBS_LAUNCH_CHECK("sort_hist_scan_kernel");
     int64_t *kb[2] = {keys, keys_alt};
     int32_t *vb[2] = {vals, vals_alt};
+    int run = 0;  // passes launched so far: the data sits in buffer run & 1
     for (int p = 0; p < L.passes; ++p) {
+        // a pass whose digit every key shares is the identity of a stable
+        // sort: leaving it out changes nothing
+        if ((skip_passes >> p) & 1u) continue;
         const int shift = begin_bit + p * kRadixBits;
         const int bits = std::min(kRadixBits, end_bit - shift);
         sort_pass_kernel<<<(unsigned)L.tiles, kSortThreads, 0, st>>>(
-            n, kb[p & 1], kb[(p + 1) & 1], vb[p & 1], vb[(p + 1) & 1], shift, bits, hist + (size_t)p * kRadix,
-            counters + p, lookback + (size_t)p * L.tiles * kRadix, status);
+            n, kb[run & 1], kb[(run + 1) & 1], vb[run & 1], vb[(run + 1) & 1], shift, bits,
+            hist + (size_t)p * kRadix, counters + p, lookback + (size_t)p * L.tiles * kRadix, status);
         UBS_LAUNCH_CHECK("sort_pass_kernel");
+        ++run;
     }
-    *result_in_alt = L.passes & 1;
+    *result_in_alt = run & 1;
     return UBS_OK;
 }
 

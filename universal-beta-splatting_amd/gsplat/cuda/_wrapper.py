@@ -273,12 +273,13 @@ def isect_tiles(
     st = stream_of(means2d)
     tpg = torch.empty((C, N), dtype=torch.int32, device=dev)
     cum = torch.empty((C * N,), dtype=torch.int64, device=dev)
-    n_dev = torch.zeros((1,), dtype=torch.int64, device=dev)
+    # [n_isects, (~min depth bits, max depth bits) as two uint32] read in one sync
+    info = torch.zeros((2,), dtype=torch.int64, device=dev)
     ws_bytes = int(L.ubs_isect_count_workspace_bytes(C, N))
     ws = torch.empty((ws_bytes,), dtype=torch.uint8, device=dev)
-    _call("ubs_isect_tiles_count", C, N, ptr(means2d), ptr(radii), tile_size, tile_width, tile_height,
-          ptr(tpg), ptr(cum), ptr(n_dev), ptr(ws), ws_bytes, st)
-    n_isects = int(n_dev.item())  # host sync, as isect_tiles.cu:181
+    _call("ubs_isect_tiles_count_range", C, N, ptr(means2d), ptr(radii), ptr(depths), tile_size, tile_width,
+          tile_height, ptr(tpg), ptr(cum), ptr(info), _vp_at(info, 1) if sort else None, ptr(ws), ws_bytes, st)
+    n_isects, packed = (int(x) for x in info.cpu())  # host sync, as isect_tiles.cu:181
     isect_ids = torch.empty((n_isects,), dtype=torch.int64, device=dev)
     flatten_ids = torch.empty((n_isects,), dtype=torch.int32, device=dev)
     if n_isects:
@@ -286,15 +287,40 @@ def isect_tiles(
               tile_width, tile_height, ptr(isect_ids), ptr(flatten_ids), st)
     if n_isects and sort:
         end_bit = 32 + int(L.ubs_isect_prefix_bits(C, tile_width, tile_height))
-        isect_ids, flatten_ids = radix_sort_pairs(isect_ids, flatten_ids, 0, end_bit)
+        isect_ids, flatten_ids = radix_sort_pairs(isect_ids, flatten_ids, 0, end_bit,
+                                                  skip_passes=constant_depth_passes(packed))
     return tpg, isect_ids, flatten_ids
 
 
+def constant_depth_passes(packed_range: int) -> int:
+    """Bit mask of the 8-bit sort passes over depth bits (passes 0-3 of the
+    isect-id sort) on which every visible depth agrees, from the packed
+    (~min, max) uint32 pair ubs_isect_tiles_count_range writes: byte p is the same
+    for every depth in [min, max] when min and max agree on bits >= 8p."""
+    lo = ~packed_range & 0xFFFFFFFF
+    hi = (packed_range >> 32) & 0xFFFFFFFF
+    mask = 0
+    if lo <= hi:
+        for p in range(4):
+            if (lo >> (8 * p)) == (hi >> (8 * p)):
+                mask |= 1 << p
+    return mask
+
+
+def _vp_at(t: Tensor, index: int):
+    """Raw pointer to element `index` of a contiguous tensor."""
+    import ctypes
+
+    return ctypes.c_void_p(t.data_ptr() + index * t.element_size())
+
+
 def radix_sort_pairs(keys: Tensor, values: Tensor, begin_bit: int, end_bit: int,
-                     check_status: bool = False) -> Tuple[Tensor, Tensor]:
+                     check_status: bool = False, skip_passes: int = 0) -> Tuple[Tensor, Tensor]:
     """Stable onesweep radix sort of (int64 key, int32 value) on key bits
     [begin_bit, end_bit); returns new sorted tensors (double-buffered; the
-    inputs may be consumed as scratch)."""
+    inputs may be consumed as scratch).  skip_passes: bit p set leaves out
+    the 8-bit pass over bits [begin_bit + 8p, +8), which the caller knows
+    every key shares (the pass would be the identity)."""
     require_cuda(keys, values)
     assert keys.dtype == torch.int64 and values.dtype == torch.int32
     n = keys.numel()
@@ -307,8 +333,12 @@ def radix_sort_pairs(keys: Tensor, values: Tensor, begin_bit: int, end_bit: int,
     import ctypes
 
     sel = ctypes.c_int(0)
-    _call("ubs_radix_sort_pairs", n, ptr(keys), ptr(keys_alt), ptr(values), ptr(vals_alt), begin_bit, end_bit,
-          ptr(ws), ws_bytes, ctypes.byref(sel), st)
+    if skip_passes:
+        _call("ubs_radix_sort_pairs_skip", n, ptr(keys), ptr(keys_alt), ptr(values), ptr(vals_alt), begin_bit,
+              end_bit, int(skip_passes), ptr(ws), ws_bytes, ctypes.byref(sel), st)
+    else:
+        _call("ubs_radix_sort_pairs", n, ptr(keys), ptr(keys_alt), ptr(values), ptr(vals_alt), begin_bit,
+              end_bit, ptr(ws), ws_bytes, ctypes.byref(sel), st)
     if check_status:
         status = ctypes.c_int(0)
         L.check(L.ubs_radix_sort_status(ptr(ws), ctypes.byref(status), st), "radix_sort_status")
