@@ -27,7 +27,10 @@
 namespace ubs {
 
 template <int TS, int CDIM, bool F64T>
-__global__ __launch_bounds__(TS *TS) void raster_bwd_kernel(
+#ifndef UBS_BWD_WAVES_PER_EU
+#define UBS_BWD_WAVES_PER_EU 1
+#endif
+__global__ __launch_bounds__(TS *TS, UBS_BWD_WAVES_PER_EU) void raster_bwd_kernel(
     uint32_t C, uint32_t N, int64_t n_isects, uint32_t channels, uint32_t coff,
     const float *__restrict__ means2d, const float *__restrict__ conics, const float *__restrict__ colors,
     const float *__restrict__ opacities, const float *__restrict__ betas, const float *__restrict__ backgrounds,
@@ -155,9 +158,29 @@ __global__ __launch_bounds__(TS *TS) void raster_bwd_kernel(
         // the same instructions with alpha = 0, (1 - sigma) = 1 and a zero
         // v_alpha, so they produce exact zeros and leave T and bufv unchanged.
         // Returns whether this pixel contributes.
-        auto pixel_grads = [&](const int t, float (&v)[2 * NV], const int o, const bool on) -> bool {
-            const float4 g1 = s_geo[t];
-            const float4 g2 = s_geo2[t];
+        // the LDS record of batch entry t: geometry, opacity, beta, colour
+        struct Rec {
+            float4 g1, g2;
+            float col[CDIM];
+        };
+        auto load_rec = [&](const int t) {
+            Rec r;
+            r.g1 = s_geo[t];
+            r.g2 = s_geo2[t];
+            if constexpr (kColVec) {
+                const float4 cv = s_colv[t];
+                const float c4[4] = {cv.x, cv.y, cv.z, cv.w};
+#pragma unroll
+                for (int q = 0; q < CDIM; ++q) r.col[q] = c4[q];
+            } else {
+#pragma unroll
+                for (int q = 0; q < CDIM; ++q) r.col[q] = s_col[q][t];
+            }
+            return r;
+        };
+        auto pixel_grads = [&](const Rec &r, const int t, float (&v)[2 * NV], const int o, const bool on) -> bool {
+            const float4 g1 = r.g1;
+            const float4 g2 = r.g2;
             const float dx = g1.x - px, dy = g1.y - py;
             const float sigma = __builtin_fmaf(g1.w * dx, dy, __builtin_fmaf(g1.z * dx, dx, (g2.x * dy) * dy));
             const bool valid = on && t >= t_min && sigma >= 0.f && sigma < 1.f;
@@ -180,16 +203,7 @@ __global__ __launch_bounds__(TS *TS) void raster_bwd_kernel(
             const float fac = alpha * Tc;
             // sum_k (c_k T - buf_k ra) v_k = T (c . v) - ra (buf . v), with
             // bufv = buf . v carried as one running sum
-            float col[CDIM];
-            if constexpr (kColVec) {
-                const float4 cv = s_colv[t];
-                const float c4[4] = {cv.x, cv.y, cv.z, cv.w};
-#pragma unroll
-                for (int q = 0; q < CDIM; ++q) col[q] = c4[q];
-            } else {
-#pragma unroll
-                for (int q = 0; q < CDIM; ++q) col[q] = s_col[q][t];
-            }
+            const float(&col)[CDIM] = r.col;
             float cdot = col[0] * vrc[0];
             v[o + 7] = fac * vrc[0];
 #pragma unroll
@@ -219,6 +233,22 @@ __global__ __launch_bounds__(TS *TS) void raster_bwd_kernel(
             v[o + 6] = v[o + 5] * lg;                         // x opac ln 2 (lg = log2(1 - sigma))
             return valid;
         };
+        // The LDS adds of a pair's totals are issued after the next pair's
+        // record reads: LDS operations complete in order, so an add issued
+        // before the reads would make every read wait for it (its latency
+        // was the largest single cost of this loop).
+        bool pend = false;  // wave-uniform: a pair's totals wait to be added
+        int pend_t0 = 0, pend_t1 = 0;
+        float pend_val[NPASS];
+        auto add_pending = [&]() {
+            if (!pend) return;
+#pragma unroll
+            for (int p = 0; p < NPASS; ++p) {
+                const int idx = my_idx[p];
+                if (idx >= 0) atomicAdd(&s_acc[idx % NV][idx < NV ? pend_t0 : pend_t1], pend_val[p]);
+            }
+            pend = false;
+        };
         for (int c0 = 0; c0 < cnt; c0 += 64) {
             const int e = c0 + lane;
             uint64_t todo = __ballot(e < cnt && ((s_mask[e] >> wave) & 1u) && (batch_end - e) <= wave_bin_max);
@@ -230,15 +260,17 @@ __global__ __launch_bounds__(TS *TS) void raster_bwd_kernel(
                 const bool two = todo != 0ull;
                 const int t1 = two ? c0 + __builtin_ctzll(todo) : t0;
                 if (two) todo &= todo - 1ull;
+                const Rec r0 = load_rec(t0), r1 = load_rec(t1);
+                add_pending();
                 float v[2 * NV];
                 // a lone last primitive runs as a pair with a switched-off partner
                 // (exact zeros), which keeps the loop body free of branches
-                const bool c0v = pixel_grads(t0, v, 0, true);
-                const bool c1v = pixel_grads(t1, v, NV, two);
+                const bool c0v = pixel_grads(r0, t0, v, 0, true);
+                const bool c1v = pixel_grads(r1, t1, v, NV, two);
                 const bool any = __ballot(c0v || c1v) != 0ull;
                 if (!any) continue;
-                // wave totals, scattered over the four 16-lane quarters, added into
-                // the primitives' accumulators by one ds_add_f32
+                // wave totals, scattered over the four 16-lane quarters; one
+                // ds_add_f32 per pass adds them into the primitives' accumulators
                 float tot[NG2];
                 wave_totals<2 * NV>(v, tot);
 #pragma unroll
@@ -246,14 +278,54 @@ __global__ __launch_bounds__(TS *TS) void raster_bwd_kernel(
                     float mine = tot[16 * p];
 #pragma unroll
                     for (int r = 16 * p + 1; r < NG2 && r < 16 * p + 16; ++r) mine = my_r == r - 16 * p ? tot[r] : mine;
-                    const int idx = my_idx[p];
-                    if (idx >= 0) atomicAdd(&s_acc[idx % NV][idx < NV ? t0 : t1], mine);
+                    pend_val[p] = mine;
                 }
+                pend_t0 = t0;
+                pend_t1 = t1;
+                pend = true;
             }
         }
+        add_pending();
         __syncthreads();
         // ---- flush this batch's accumulators to the per-primitive gradients
-        if (tid < cnt) {
+        if (vrec) {
+            // packed records: lane -> (entry, value) with 8 consecutive lanes on
+            // one primitive's 32-byte record, so one atomic instruction covers 8
+            // records (8 floats each) instead of 64 scattered floats
+            constexpr int kCv = kColVec ? 4 : CDIM;  // lanes per primitive for the colours
+            for (int k = tid; k < cnt * 8; k += BS) {
+                const int t = k >> 3, q = k & 7;
+                const float4 gp = s_geo2[t];  // c, opacity, beta, id
+                const float k1 = -gp.y * gp.z;
+                float val;
+                if (q < 2) {
+                    const float4 gg = s_geo[t];  // x, y, a, 2b
+                    const float su = s_acc[0][t], sw = s_acc[1][t];
+                    const float bb = 0.5f * gg.w;
+                    val = (q == 0 ? __builtin_fmaf(gg.z, su, bb * sw) : __builtin_fmaf(bb, su, gp.x * sw)) *
+                          (2.f * k1);
+                } else {
+                    const float acc = s_acc[q < 7 ? q : 0][t];
+                    val = q == 2 || q == 4 ? acc * k1
+                          : q == 3        ? acc * (2.f * k1)
+                          : q == 6        ? acc * (gp.y * 0.69314718055994531f)
+                          : q == 5        ? acc
+                                          : 0.f;
+                }
+                if (val != 0.f) atomic_add_f32(vrec + (int64_t)__float_as_int(gp.w) * kRecordFloats + q, val);
+            }
+            for (int k = tid; k < cnt * kCv; k += BS) {
+                const int t = k / kCv, q = k % kCv;
+                if (q < CDIM) {
+                    const float val = s_acc[7 + q][t];
+                    if (val != 0.f)
+                        atomic_add_f32(v_colors + (int64_t)__float_as_int(s_geo2[t].w) * channels + coff + q, val);
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < NV; ++q) s_acc[q][tid] = 0.f;
+        } else if (tid < cnt) {
             const int32_t g = __float_as_int(s_geo2[tid].w);
             float a[NV];
 #pragma unroll
@@ -261,33 +333,24 @@ __global__ __launch_bounds__(TS *TS) void raster_bwd_kernel(
                 a[q] = s_acc[q][tid];
                 s_acc[q][tid] = 0.f;
             }
-            {
-                const float4 gg = s_geo[tid];   // x, y, a, 2b
-                const float4 gp = s_geo2[tid];  // c, opacity, beta, id
-                const float k1 = -gp.y * gp.z, k2 = 2.f * k1;
-                const float su = a[0], sw = a[1];
-                const float b = 0.5f * gg.w;
-                a[0] = __builtin_fmaf(gg.z, su, b * sw) * k2;
-                a[1] = __builtin_fmaf(b, su, gp.x * sw) * k2;
-                a[2] *= k1;
-                a[3] *= k2;
-                a[4] *= k1;
-                a[6] *= gp.y * 0.69314718055994531f;
-            }
-            if (vrec) {  // the primitive's 7 scalar gradients share one 32-byte record
-                float *r = vrec + (int64_t)g * kRecordFloats;
-#pragma unroll
-                for (int q = 0; q < 7; ++q)
-                    if (a[q] != 0.f) atomic_add_f32(r + q, a[q]);
-            } else {
-                if (a[0] != 0.f) atomic_add_f32(v_means2d + (int64_t)g * 2, a[0]);
-                if (a[1] != 0.f) atomic_add_f32(v_means2d + (int64_t)g * 2 + 1, a[1]);
-                if (a[2] != 0.f) atomic_add_f32(v_conics + (int64_t)g * 3, a[2]);
-                if (a[3] != 0.f) atomic_add_f32(v_conics + (int64_t)g * 3 + 1, a[3]);
-                if (a[4] != 0.f) atomic_add_f32(v_conics + (int64_t)g * 3 + 2, a[4]);
-                if (a[5] != 0.f) atomic_add_f32(v_opacities + g, a[5]);
-                if (a[6] != 0.f) atomic_add_f32(v_betas + g, a[6]);
-            }
+            const float4 gg = s_geo[tid];   // x, y, a, 2b
+            const float4 gp = s_geo2[tid];  // c, opacity, beta, id
+            const float k1 = -gp.y * gp.z, k2 = 2.f * k1;
+            const float su = a[0], sw = a[1];
+            const float bb = 0.5f * gg.w;
+            a[0] = __builtin_fmaf(gg.z, su, bb * sw) * k2;
+            a[1] = __builtin_fmaf(bb, su, gp.x * sw) * k2;
+            a[2] *= k1;
+            a[3] *= k2;
+            a[4] *= k1;
+            a[6] *= gp.y * 0.69314718055994531f;
+            if (a[0] != 0.f) atomic_add_f32(v_means2d + (int64_t)g * 2, a[0]);
+            if (a[1] != 0.f) atomic_add_f32(v_means2d + (int64_t)g * 2 + 1, a[1]);
+            if (a[2] != 0.f) atomic_add_f32(v_conics + (int64_t)g * 3, a[2]);
+            if (a[3] != 0.f) atomic_add_f32(v_conics + (int64_t)g * 3 + 1, a[3]);
+            if (a[4] != 0.f) atomic_add_f32(v_conics + (int64_t)g * 3 + 2, a[4]);
+            if (a[5] != 0.f) atomic_add_f32(v_opacities + g, a[5]);
+            if (a[6] != 0.f) atomic_add_f32(v_betas + g, a[6]);
 #pragma unroll
             for (int q = 0; q < CDIM; ++q)
                 if (a[7 + q] != 0.f) atomic_add_f32(v_colors + (int64_t)g * channels + coff + q, a[7 + q]);
