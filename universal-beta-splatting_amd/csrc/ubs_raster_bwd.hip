@@ -13,10 +13,14 @@
 //   * sums them over the 64 lanes with a permlane reduce-scatter that leaves
 //     each 16-lane quarter with different totals (wave_totals), and one lane
 //     per quarter adds them into the primitive's LDS accumulator
-//     (ds_add_f32), so the four waves of a tile share one set of partial sums;
-// and after the batch each thread flushes one primitive's accumulator to the
-// per-primitive gradients with hardware float atomics (one set of atomics per
-// (tile, primitive) instead of one per (wave, primitive)).
+//     (ds_add_f32), so the four waves of a tile share one set of partial sums
+//     (finishing with half-row sums and two adding lanes per total measured
+//     slower: 3.23 vs 3.09 ms, the same-address LDS adds cost more than the
+//     DPP steps they save);
+// and after the batch the accumulators go to the per-primitive gradients
+// with hardware float atomics (one set per (tile, primitive) instead of one
+// per (wave, primitive)), 8 lanes on one primitive's 32-byte packed record so
+// that an atomic instruction touches 8 records, not 64 scattered floats.
 // The T recovery can run in fp64 (template flag) where stability needs it.
 #pragma clang fp contract(off)
 
