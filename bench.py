@@ -36,6 +36,11 @@ CONFIGS = {
                workload="1M-primitive dynamic 7-D Beta scene, one 1352x1014 view, FP32 fwd+bwd"),
     "c1": dict(N=10_000, D=6, W=256, H=256,
                workload="10k-primitive static 6-D Beta scene, one 256x256 view, FP32 fwd+bwd"),
+    # batched multi-view training step (not the headline metric): 8 views,
+    # resolution cycling 960x540, 1280x720, 1920x1080 by view index
+    "c4": dict(N=3_000_000, D=6, views=8, sizes=((960, 540), (1280, 720), (1920, 1080)),
+               workload="3M-primitive static 6-D Beta scene, 8 views at 960x540/1280x720/1920x1080, "
+                        "one batched FP32 fwd+bwd step"),
 }
 
 
@@ -151,7 +156,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS),
+                    help="c2 (default): the headline 1080p / 1M workload; c3: 7-D; c1: small; "
+                         "c4: batched 8-view step at mixed resolutions (reports views/s)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--path", default="fused", choices=("fused", "chain"),
                     help="fused: one-launch Beta projection (render_views); chain: the reference's op chain "
@@ -177,6 +184,8 @@ def main():
     from gsplat.cuda import _wrapper as W
 
     cfg = CONFIGS[args.config]
+    if "views" in cfg:
+        return batch_bench(args, cfg)
     dev = torch.device("cuda:0")
     torch.manual_seed(0)
     sc = scenes.make_scene(cfg["N"], cfg["D"], seed=0)
@@ -294,6 +303,82 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "stages": stages,
+    }
+    print(json.dumps(line))
+
+
+def batch_bench(args, cfg):
+    """The batched multi-view training step: one fused projection for all
+    views (per-view conditioning and image size), isect + sort + blend per
+    resolution group, L1 loss per view, backward to the leaves."""
+    import numpy as np
+    import torch
+
+    from gsplat import beta, scenes
+    from gsplat.cuda import _wrapper as W
+
+    dev = torch.device("cuda:0")
+    C = cfg["views"]
+    sizes = [cfg["sizes"][c % len(cfg["sizes"])] for c in range(C)]
+    sc = scenes.make_scene(cfg["N"], cfg["D"], seed=0)
+    params = beta.BetaParams.from_scene(sc, dev)
+    vms, _, _ = scenes.make_cameras(C, sizes[0][0], sizes[0][1], seed=0)
+    Ks = np.stack([scenes.intrinsics(w, h) for w, h in sizes]).astype(np.float32)
+    vms_t, Ks_t = torch.tensor(vms, device=dev), torch.tensor(Ks, device=dev)
+    g = np.random.Generator(np.random.PCG64(5))
+    tgts = [torch.tensor(g.uniform(0, 1, (h, w, 3)), dtype=torch.float32, device=dev) for w, h in sizes]
+    leaves = list(params.leaves().values())
+
+    def step():
+        for p in leaves:
+            p.grad = None
+        imgs, _ = beta.render_batch(params, vms_t, Ks_t, sizes)
+        loss = sum((im - t).abs().mean() for im, t in zip(imgs, tgts))
+        loss.backward()
+
+    for _ in range(args.warmup):
+        step()
+    timer = W.LaunchTimer()
+    W.set_launch_timer(timer)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    W.set_launch_timer(None)
+    launches = timer.durations_ms()
+    ms = (t1 - t0) * 1e3 / args.steps
+    stages = {k.replace("ubs_", ""): {"ms_per_step": round(sum(v) / args.steps, 4), "calls_per_step": len(v) / args.steps}
+              for k, v in sorted(launches.items(), key=lambda kv: -sum(kv[1]))}
+
+    # the reference's structure for comparison: one view at a time
+    # (BetaModel.render per camera, train.py:111-128), same kernels
+    def step_per_view():
+        for p in leaves:
+            p.grad = None
+        for c, (w, h) in enumerate(sizes):
+            img, _, _ = beta.render_views(params, vms_t[c:c + 1], Ks_t[c:c + 1], w, h)
+            (img[0] - tgts[c]).abs().mean().backward()
+
+    for _ in range(max(1, args.warmup // 2)):
+        step_per_view()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step_per_view()
+    torch.cuda.synchronize()
+    ms_per_view_loop = (time.perf_counter() - t0) * 1e3 / args.steps
+    line = {
+        "metric": "views/s fwd+bwd, batched multi-view step (3M prims, 8 views at 540p/720p/1080p)",
+        "value": round(C * 1e3 / ms, 3), "unit": "views/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32", "data": "synthetic (seeded scene and camera generator, gsplat/scenes.py)",
+        "config": {"workload": cfg["workload"], "primitives": cfg["N"], "dims": cfg["D"], "views": C,
+                   "sizes": sizes, "peak_hbm_gb": round(torch.cuda.max_memory_allocated() / 1e9, 2)},
+        "roofline": None, "cpu_baseline": None, "stages": stages,
+        "per_view_loop_ms_per_step": round(ms_per_view_loop, 3),
+        "speedup_vs_per_view_loop": round(ms_per_view_loop / ms, 3),
     }
     print(json.dumps(line))
 

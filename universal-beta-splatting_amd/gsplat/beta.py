@@ -170,3 +170,51 @@ def render_views(params: BetaParams, viewmats: Tensor, Ks: Tensor, width: int, h
                 flatten_ids=flatten_ids, isect_offsets=isect_offsets, width=width, height=height,
                 tile_size=tile_size, n_cameras=C)
     return render_colors, render_alphas, meta
+
+
+def render_batch(params: BetaParams, viewmats: Tensor, Ks: Tensor, sizes, timestamps: Optional[Tensor] = None,
+                 backgrounds: Optional[Tensor] = None, tile_size: int = 16, near_plane: float = 0.01,
+                 far_plane: float = 1e10, radius_clip: float = 0.0, eps2d: float = 0.3,
+                 cam_pos: Optional[Tensor] = None):
+    """A batch of C views of one scene at per-view image sizes (sizes: C
+    pairs (W, H)), e.g. one training step over several cameras: one fused
+    Beta projection for all views (per-view conditioning and image size),
+    then intersection, sort and blend per group of views sharing a size.
+    Returns (list of render_colors [H_c,W_c,3], list of render_alphas
+    [H_c,W_c,1]) in view order; gradients of every view add up in the
+    leaves.  The reference renders such a batch one view at a time
+    (train.py:111-128, BetaModel.render per view)."""
+    D = params.D
+    if D not in (6, 7):
+        raise NotImplementedError("Only implemented for 6D or 7D query")
+    C = viewmats.shape[0]
+    sizes = [(int(w), int(h)) for w, h in sizes]
+    assert len(sizes) == C, (len(sizes), C)
+    dev = viewmats.device
+    with torch.no_grad():
+        if cam_pos is None:
+            cam_pos = camera_centers(viewmats.detach().to(torch.float32))
+        cam_pos = cam_pos.detach().to(torch.float32).contiguous()
+    wh = torch.tensor(sizes, dtype=torch.int32, device=dev)
+    radii, means2d, depths, conics, opacities, betas = beta_project(
+        params.xyz, params.mean_rest, params.scale_raw, params.l_triangle, params.opacity_raw, params.beta_raw,
+        params.quats, viewmats, Ks, 0, 0, cam_pos=cam_pos, timestamps=timestamps, image_sizes=wh, eps2d=eps2d,
+        near_plane=near_plane, far_plane=far_plane, radius_clip=radius_clip)
+    groups = {}
+    for c, wh_c in enumerate(sizes):
+        groups.setdefault(wh_c, []).append(c)
+    colors_out, alphas_out = [None] * C, [None] * C
+    for (W, H), idx in groups.items():
+        sel = torch.tensor(idx, dtype=torch.long, device=dev)
+        Cg = len(idx)
+        tw, th = math.ceil(W / float(tile_size)), math.ceil(H / float(tile_size))
+        m2, r, d = means2d.index_select(0, sel), radii.index_select(0, sel), depths.index_select(0, sel)
+        _, isect_ids, flatten_ids = isect_tiles(m2, r, d, tile_size, tw, th, n_cameras=Cg)
+        offsets = isect_offset_encode(isect_ids, Cg, tw, th)
+        bg = backgrounds.index_select(0, sel) if backgrounds is not None else None
+        rc, ra = rasterize_to_pixels(m2, conics.index_select(0, sel), params.rgb.expand(Cg, -1, -1),
+                                     opacities.index_select(0, sel), betas.index_select(0, sel), W, H, tile_size,
+                                     offsets, flatten_ids, backgrounds=bg)
+        for k, c in enumerate(idx):
+            colors_out[c], alphas_out[c] = rc[k], ra[k]
+    return colors_out, alphas_out
