@@ -71,6 +71,28 @@ class BetaParams:
         return d
 
     @staticmethod
+    def from_ply(path: str, device, requires_grad: bool = True) -> "BetaParams":
+        """A scene saved by BetaModel.save_ply (scene/beta_model.py:298-320),
+        resident on `device` as one tensor per leaf (rotation from l_triangle)."""
+        from .ply import load_ply
+
+        d = load_ply(path)
+
+        def t(a):
+            return torch.tensor(a, dtype=torch.float32, device=device).requires_grad_(requires_grad)
+        return BetaParams(xyz=t(d["xyz"]), mean_rest=t(d["mean_rest"]), scale_raw=t(d["scale_raw"]),
+                          l_triangle=t(d["l_triangle"]), opacity_raw=t(d["opacity_raw"]), beta_raw=t(d["beta_raw"]),
+                          rgb=t(d["rgb"]))
+
+    def save_ply(self, path: str) -> None:
+        """Write the leaves in the reference's PLY layout (quaternions, which
+        that layout has no property for, are not stored)."""
+        from .ply import save_ply
+
+        save_ply(path, self.xyz, self.mean_rest, self.scale_raw, self.l_triangle, self.opacity_raw, self.beta_raw,
+                 self.rgb)
+
+    @staticmethod
     def from_scene(sc, device, requires_grad: bool = True, use_quats: bool = True) -> "BetaParams":
         def t(a):
             return torch.tensor(a, dtype=torch.float32, device=device).requires_grad_(requires_grad)
