@@ -5,8 +5,9 @@ One step = one training step of the BetaModel render path on one view:
 parameter activations, view-direction query, covariance construction,
 conditioning, projection (fused into one launch each way by default,
 --path chain runs them as the reference's separate ops), tile intersection +
-sort + ranges, blend forward, L1 loss against a target image, and the
-backward of all of it down to the leaf parameters.  Inputs are synthetic (seeded) and resident in HBM before the
+sort + ranges, blend forward, L1 loss against a target image (the fused
+loss kernels; --lambda-dssim adds SSIM as the reference's training loss),
+and the backward of all of it down to the leaf parameters.  Inputs are synthetic (seeded) and resident in HBM before the
 timed region.  Prints one JSON line: throughput, latency percentiles, the
 roofline of the dominant kernel and a CPU baseline from the oracle.
 
@@ -160,6 +161,9 @@ def main():
                     help="c2 (default): the headline 1080p / 1M workload; c3: 7-D; c1: small; "
                          "c4: batched 8-view step at mixed resolutions (reports views/s)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--lambda-dssim", type=float, default=0.0,
+                    help="image loss (1 - l) L1 + l (1 - SSIM) in the fused loss kernels; 0 (default) is the L1 "
+                         "loss of the benchmark definition, 0.2 the reference's training loss")
     ap.add_argument("--path", default="fused", choices=("fused", "chain"),
                     help="fused: one-launch Beta projection (render_views); chain: the reference's op chain "
                          "(render: activations, rotation, covariance, conditioning, projection as separate ops)")
@@ -208,11 +212,16 @@ def main():
             return beta.render_views(params, vms_t, Ks_t, cfg["W"], cfg["H"], timestamps=ts_t)
         return beta.render(params, vm, K, cfg["W"], cfg["H"], timestamp=t_view, rest=rest)
 
+    from gsplat.loss import image_loss
+
     def step():
         for p in leaves:
             p.grad = None
         img, alpha, meta = render()
-        loss = (img[0] - tgt).abs().mean()
+        if args.path == "fused":
+            loss = image_loss(img[0], tgt, args.lambda_dssim)
+        else:  # the reference's torch ops
+            loss = (img[0] - tgt).abs().mean()
         loss.backward()
         return meta
 
@@ -295,6 +304,7 @@ def main():
         "config": {"workload": cfg["workload"], "primitives": cfg["N"], "dims": cfg["D"],
                    "width": cfg["W"], "height": cfg["H"], "views": 1, "tile_size": 16,
                    "n_isects": n_isect, "visible_primitives": n_vis, "projection_path": args.path,
+                   "loss": "L1" if args.lambda_dssim == 0 else f"{1 - args.lambda_dssim:g} L1 + {args.lambda_dssim:g} (1 - SSIM)",
                    "sort_passes_skipped": cfg["sort_skipped"]},
         "fwd_frames_per_s": round(1e3 / fwd_ms, 3),
         "fwd_bwd_mpix_per_s": round(cfg["W"] * cfg["H"] * 1e3 / ms_per_step / 1e6, 2),

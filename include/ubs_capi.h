@@ -331,6 +331,29 @@ int ubs_rasterize_to_pixels_bwd(uint32_t C, uint32_t N, int64_t n_isects, uint32
                                 size_t workspace_bytes, void *stream);
 
 /* ------------------------------------------------------------------------
+ * Image loss of the training step: (1 - lambda) L1 + lambda (1 - SSIM).
+ * Replaces torch's l1_loss (utils/loss_utils.py:18-19) and the external
+ * fused_ssim the reference calls at train.py:117-121, whose arithmetic is
+ * the SSIM of utils/loss_utils.py:45-80 (11x11 Gaussian window, sigma 1.5,
+ * zero padding, C1 = 0.01^2, C2 = 0.03^2).
+ * ------------------------------------------------------------------------ */
+
+/* Workspace bytes for images [B,H,W,C]. */
+size_t ubs_image_loss_workspace_bytes(int64_t B, int64_t H, int64_t W, int64_t C);
+
+/* img, gt float32 [B,H,W,C]; *loss (device float) = (1 - lambda) mean|img - gt|
+ * + lambda (1 - mean SSIM), the means over every element.  Leaves in the
+ * workspace what ubs_image_loss_bwd needs.  lambda_dssim in [0, 1]. */
+int ubs_image_loss_fwd(int64_t B, int64_t H, int64_t W, int64_t C, const float *img, const float *gt,
+                       float lambda_dssim, float *loss, void *workspace, size_t workspace_bytes, void *stream);
+
+/* v_img [B,H,W,C] = *v_loss (device float) x d loss / d img, from the
+ * workspace of the matching forward call. */
+int ubs_image_loss_bwd(int64_t B, int64_t H, int64_t W, int64_t C, const float *img, const float *gt,
+                       float lambda_dssim, const float *v_loss, const void *workspace, size_t workspace_bytes,
+                       float *v_img, void *stream);
+
+/* ------------------------------------------------------------------------
  * Process-wide numerical switches (no counterpart in the reference).
  * ------------------------------------------------------------------------ */
 

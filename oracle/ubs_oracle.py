@@ -442,3 +442,62 @@ def beta_step(leaves: dict, viewmat, K, width, height, target, timestamp=0.0, dt
         beta_raw=np.concatenate([vbe[0][:, None], gb], 1) * beta, rgb=vcol[0],
         quats=_quat_R_vjp(L["quats"], qn, v_rot.astype(np.float64)))
     return img[0], grads, meta
+
+
+# ---------------------------------------------------------------------------
+# Image loss of the training step: (1 - lambda) L1 + lambda (1 - SSIM)
+# (train.py:117-121; SSIM as utils/loss_utils.py:26-80)
+# ---------------------------------------------------------------------------
+def gauss_window_f32(window_size: int = 11, sigma: float = 1.5) -> np.ndarray:
+    """loss_utils.py:26-33: float32 weights exp(-(x - 5)^2 / (2 sigma^2)),
+    divided by their float32 sum."""
+    g = np.array([math.exp(-((x - window_size // 2) ** 2) / float(2 * sigma ** 2)) for x in range(window_size)],
+                 dtype=np.float32)
+    return g / g.sum(dtype=np.float32)
+
+
+def _ssim_map_torch(x, y, g):
+    """loss_utils.py:56-80 on [B, C, H, W] fp64 tensors: 2-D window outer(g, g)
+    correlated per channel with zero padding 5."""
+    import torch
+    import torch.nn.functional as F
+
+    C = x.shape[1]
+    w = torch.outer(g, g)[None, None].expand(C, 1, g.numel(), g.numel()).contiguous()
+    pad = g.numel() // 2
+    mu1 = F.conv2d(x, w, padding=pad, groups=C)
+    mu2 = F.conv2d(y, w, padding=pad, groups=C)
+    mu1_sq, mu2_sq, mu1_mu2 = mu1 * mu1, mu2 * mu2, mu1 * mu2
+    s1 = F.conv2d(x * x, w, padding=pad, groups=C) - mu1_sq
+    s2 = F.conv2d(y * y, w, padding=pad, groups=C) - mu2_sq
+    s12 = F.conv2d(x * y, w, padding=pad, groups=C) - mu1_mu2
+    C1, C2 = 0.01 ** 2, 0.03 ** 2
+    return ((2 * mu1_mu2 + C1) * (2 * s12 + C2)) / ((mu1_sq + mu2_sq + C1) * (s1 + s2 + C2))
+
+
+def ssim_map(x: np.ndarray, y: np.ndarray) -> np.ndarray:
+    """SSIM map (fp64) of images [B, H, W, C] (the rasteriser's layout)."""
+    import torch
+
+    g = torch.tensor(gauss_window_f32(), dtype=torch.float64)
+    tx = torch.tensor(np.asarray(x, np.float64)).permute(0, 3, 1, 2)
+    ty = torch.tensor(np.asarray(y, np.float64)).permute(0, 3, 1, 2)
+    return _ssim_map_torch(tx, ty, g).permute(0, 2, 3, 1).numpy()
+
+
+def image_loss(x: np.ndarray, y: np.ndarray, lambda_dssim: float):
+    """(loss, d loss / d x) in fp64 for images [B, H, W, C]:
+    (1 - lambda) mean|x - y| + lambda (1 - mean SSIM), the mean over every
+    element of the batch (train.py:117-128 averages equal-size views); the
+    gradient by autograd of this restatement (sign(0) = 0, as torch's abs)."""
+    import torch
+
+    g = torch.tensor(gauss_window_f32(), dtype=torch.float64)
+    tx = torch.tensor(np.asarray(x, np.float64), requires_grad=True)
+    ty = torch.tensor(np.asarray(y, np.float64))
+    loss = (1.0 - lambda_dssim) * (tx - ty).abs().mean()
+    if lambda_dssim > 0:
+        s = _ssim_map_torch(tx.permute(0, 3, 1, 2), ty.permute(0, 3, 1, 2), g)
+        loss = loss + lambda_dssim * (1.0 - s.mean())
+    loss.backward()
+    return float(loss), tx.grad.numpy()

@@ -322,3 +322,50 @@ def test_projection_quats_path_matches_covariance_path():
     b = O.proj_fwd(means, None, q, s, vm, K, 640, 480)
     np.testing.assert_array_equal(a[0], b[0])
     np.testing.assert_allclose(a[3], b[3], rtol=1e-9)
+
+
+# ------------------------------------------------------------ image loss oracle
+def test_loss_window_is_the_reference_gaussian():
+    g = O.gauss_window_f32()
+    assert g.dtype == np.float32 and g.shape == (11,)
+    assert abs(float(g.sum(dtype=np.float64)) - 1.0) < 1e-6 and np.allclose(g, g[::-1])
+    assert np.isclose(g[5] / g[4], np.exp(1.0 / 4.5), rtol=1e-6)  # exp(-(k-5)^2 / (2 * 1.5^2))
+
+
+def test_ssim_map_identities():
+    rng = np.random.Generator(np.random.PCG64(0))
+    x = rng.uniform(0, 1, (1, 20, 24, 3))
+    y = rng.uniform(0, 1, (1, 20, 24, 3))
+    assert np.allclose(O.ssim_map(x, x), 1.0, atol=1e-12)          # S(x, x) = 1
+    assert np.allclose(O.ssim_map(x, y), O.ssim_map(y, x), atol=1e-12)  # symmetric
+    # constant images a, b away from the zero-padded border: with m the 2-D
+    # window's mass (1 up to float32 rounding), mu = a m, b m, E[x^2] = a^2 m,
+    # E[xy] = a b m, so the variances are a^2 (m - m^2) etc.; for m = 1 this is
+    # S = (2ab + C1) / (a^2 + b^2 + C1)
+    a, b = 0.3, 0.7
+    s = O.ssim_map(np.full((1, 30, 30, 1), a), np.full((1, 30, 30, 1), b))
+    C1, C2 = 0.01 ** 2, 0.03 ** 2
+    m = float(O.gauss_window_f32().sum(dtype=np.float64)) ** 2
+    v = m - m * m
+    want = ((2 * a * b * m * m + C1) * (2 * a * b * v + C2)) / ((a * a + b * b) * m * m + C1) / ((a * a + b * b) * v + C2)
+    assert np.allclose(s[0, 5:-5, 5:-5, 0], want, rtol=1e-9)
+    assert abs(want - (2 * a * b + C1) / (a * a + b * b + C1)) < 1e-4
+    assert not np.allclose(s[0, 0, 0, 0], s[0, 15, 15, 0])  # the border sees the zero padding
+
+
+@pytest.mark.parametrize("lam", [0.0, 0.2, 1.0])
+def test_image_loss_gradient_finite_differences(lam):
+    rng = np.random.Generator(np.random.PCG64(1))
+    x = rng.uniform(0, 1, (1, 9, 8, 2))
+    y = rng.uniform(0, 1, (1, 9, 8, 2))
+    loss, grad = O.image_loss(x, y, lam)
+    eps = 1e-6
+    for idx in [(0, 0, 0, 0), (0, 4, 3, 1), (0, 8, 7, 0), (0, 2, 6, 1)]:
+        xp, xm = x.copy(), x.copy()
+        xp[idx] += eps
+        xm[idx] -= eps
+        fd = (O.image_loss(xp, y, lam)[0] - O.image_loss(xm, y, lam)[0]) / (2 * eps)
+        assert abs(fd - grad[idx]) < 1e-7 + 1e-5 * abs(fd), (idx, fd, grad[idx])
+    # L1 only: the mean of |x - y|, gradient sign / n
+    l1, g1 = O.image_loss(x, y, 0.0)
+    assert np.isclose(l1, np.abs(x - y).mean()) and np.allclose(g1, np.sign(x - y) / x.size)
