@@ -68,7 +68,7 @@ def algorithmic_bytes(name: str, c: dict, n_isect: int, C: int = 1) -> float:
         return n_isect * (8.0 + 24.0 * passes)
     if name == "ubs_isect_offset_encode":
         return 8.0 * n_isect + 4.0 * C * tiles
-    if name == "ubs_isect_tiles_count":
+    if name in ("ubs_isect_tiles_count", "ubs_isect_tiles_count_range"):
         return 28.0 * C * N
     if name == "ubs_isect_tiles_emit":
         return 24.0 * C * N + 12.0 * n_isect
@@ -89,6 +89,12 @@ def algorithmic_bytes(name: str, c: dict, n_isect: int, C: int = 1) -> float:
         return N * 4.0 * leaf_floats + C * N * 36.0
     if name == "ubs_beta_project_bwd":  # leaves once; radii, conic and 5 gradients per view; leaf grads
         return N * 4.0 * 2 * leaf_floats + C * N * 48.0
+    if name == "ubs_adam_step":  # every leaf element (with rgb): p, g, m, v read; p, m, v written
+        return N * 28.0 * (leaf_floats + 3)
+    if name == "ubs_image_loss_fwd":  # L1 path: image and target read once
+        return 8.0 * pix * 3 if c.get("lambda_dssim", 0.0) == 0.0 else float("nan")
+    if name == "ubs_image_loss_bwd":  # L1 path: image and target read, gradient written
+        return 12.0 * pix * 3 if c.get("lambda_dssim", 0.0) == 0.0 else float("nan")
     if name == "ubs_rot_scale_l_triangle_to_covar_fwd":
         return N * 4.0 * (9 + D + D * (D - 1) // 2 + D * D)
     if name == "ubs_rot_scale_l_triangle_to_covar_bwd":
@@ -164,6 +170,10 @@ def main():
     ap.add_argument("--lambda-dssim", type=float, default=0.0,
                     help="image loss (1 - l) L1 + l (1 - SSIM) in the fused loss kernels; 0 (default) is the L1 "
                          "loss of the benchmark definition, 0.2 the reference's training loss")
+    ap.add_argument("--adam", action="store_true",
+                    help="end each step with the fused Adam update of every leaf (the reference's training "
+                         "iteration: render, loss, backward, optimizer.step); off by default, as the headline "
+                         "metric is render fwd+bwd")
     ap.add_argument("--path", default="fused", choices=("fused", "chain"),
                     help="fused: one-launch Beta projection (render_views); chain: the reference's op chain "
                          "(render: activations, rotation, covariance, conditioning, projection as separate ops)")
@@ -214,6 +224,11 @@ def main():
 
     from gsplat.loss import image_loss
 
+    adam = None
+    if args.adam:
+        from gsplat.optim import BetaAdam
+        adam = BetaAdam.for_model(params)
+
     def step():
         for p in leaves:
             p.grad = None
@@ -223,6 +238,8 @@ def main():
         else:  # the reference's torch ops
             loss = (img[0] - tgt).abs().mean()
         loss.backward()
+        if adam is not None:
+            adam.step()
         return meta
 
     def fwd_only():
@@ -236,7 +253,7 @@ def main():
     n_vis = int((meta["radii"] > 0).sum().item())
     dbits = meta["depths"][meta["radii"] > 0].contiguous().view(torch.int32).to(torch.int64) & 0xFFFFFFFF
     lo, hi = int(dbits.min()), int(dbits.max())
-    cfg = dict(cfg, sort_skipped=bin(W.constant_depth_passes((hi << 32) | (~lo & 0xFFFFFFFF))).count("1"))
+    cfg = dict(cfg, lambda_dssim=args.lambda_dssim, sort_skipped=bin(W.constant_depth_passes((hi << 32) | (~lo & 0xFFFFFFFF))).count("1"))
 
     # ---- timed region: K steps; per-launch HIP events on the launch stream
     timer = W.LaunchTimer()
@@ -280,8 +297,10 @@ def main():
         roofline["traffic_source"] = ("profiles/pmc_traffic.json: rocprofv3 FETCH_SIZE and WRITE_SIZE of this "
                                       "workload, scaled against a 1 GiB device copy")
         roofline["valu_busy"] = valu_busy  # the blend is VALU-bound, not HBM-bound
-    stages = {k.replace("ubs_", ""): {"avg_ms": round(v[0], 4), "per_step": v[1],
-                                      "GB_per_s": round(algorithmic_bytes(k, cfg, n_isect) / (v[0] * 1e-3) / 1e9, 1)}
+    def gbps(k, ms):
+        r = algorithmic_bytes(k, cfg, n_isect) / (ms * 1e-3) / 1e9
+        return round(r, 1) if math.isfinite(r) else None  # JSON has no NaN
+    stages = {k.replace("ubs_", ""): {"avg_ms": round(v[0], 4), "per_step": v[1], "GB_per_s": gbps(k, v[0])}
               for k, v in sorted(per_call.items(), key=lambda kv: -kv[1][0] * kv[1][1])}
 
     cpu = None
@@ -305,7 +324,7 @@ def main():
                    "width": cfg["W"], "height": cfg["H"], "views": 1, "tile_size": 16,
                    "n_isects": n_isect, "visible_primitives": n_vis, "projection_path": args.path,
                    "loss": "L1" if args.lambda_dssim == 0 else f"{1 - args.lambda_dssim:g} L1 + {args.lambda_dssim:g} (1 - SSIM)",
-                   "sort_passes_skipped": cfg["sort_skipped"]},
+                   "sort_passes_skipped": cfg["sort_skipped"], "optimizer_step": "adam" if args.adam else None},
         "fwd_frames_per_s": round(1e3 / fwd_ms, 3),
         "fwd_bwd_mpix_per_s": round(cfg["W"] * cfg["H"] * 1e3 / ms_per_step / 1e6, 2),
         "latency_ms": {"p50": round(step_ms[len(step_ms) // 2], 3),

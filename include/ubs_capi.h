@@ -354,6 +354,41 @@ int ubs_image_loss_bwd(int64_t B, int64_t H, int64_t W, int64_t C, const float *
                        float *v_img, void *stream);
 
 /* ------------------------------------------------------------------------
+ * Optimiser and MCMC step after the backward.
+ * ------------------------------------------------------------------------ */
+
+/* Replaces torch.optim.Adam(groups, lr=0.0, eps=1e-15).step() over the
+ * scene's leaf groups (scene/beta_model.py:239-276, train.py:170): one
+ * launch updates up to 8 float32 tensors in place.  Host arrays of
+ * n_tensors entries: device pointers of each parameter, its gradient and
+ * its two moment buffers, element counts, learning rates and step counts
+ * (the count after this update, >= 1; bias corrections use it).  Tensors
+ * of size 0 are skipped. */
+int ubs_adam_step(int n_tensors, float *const *params, const float *const *grads, float *const *exp_avgs,
+                  float *const *exp_avg_sqs, const int64_t *sizes, const double *lrs, const int64_t *steps,
+                  double beta1, double beta2, double eps, void *stream);
+
+/* Replaces the covariance-shaped position noise of train.py:159-168
+ * (get_xyz_covariance, scene/beta_model.py:143-152, then torch.bmm):
+ * xyz[N,3] += Sigma_xyz (normal * (1 - sigmoid(opacity_raw))^100 * scale),
+ * scale = noise_lr * xyz_lr; normal [N,3] holds the standard normal draws. */
+int ubs_xyz_noise(int64_t N, int64_t D, float *xyz, const float *l_triangle, const float *scale_raw,
+                  const float *opacity_raw, const float *normal, float scale, void *stream);
+
+/* Replaces the row updates of relocate_gs / add_new_gs / _update_params
+ * and replace_tensors_to_optimizer(inds) (scene/beta_model.py:506-658) for
+ * K sampled (src, dst, ratio) triples: each leaf's dst row takes the src
+ * row, the opacity leaf (one column, pre-activation) the split opacity
+ * logit(clamp(1 - (1 - sigmoid(o))^(1/(ratio+1)), min_opacity, 1 - 2^-23));
+ * then each src row takes that opacity and its Adam moments are zeroed
+ * (exp_avgs / exp_avg_sqs may be null, or hold null entries).  dst rows must
+ * not be src rows.  leaves/widths/exp_avgs/exp_avg_sqs are host arrays of
+ * n_leaves (<= 8) entries; rows are contiguous float32 of width columns. */
+int ubs_mcmc_relocate(int64_t K, int n_leaves, float *const *leaves, const int *widths, int opacity_leaf,
+                      float *const *exp_avgs, float *const *exp_avg_sqs, const int64_t *src, const int64_t *dst,
+                      const int64_t *ratio, float min_opacity, void *stream);
+
+/* ------------------------------------------------------------------------
  * Process-wide numerical switches (no counterpart in the reference).
  * ------------------------------------------------------------------------ */
 

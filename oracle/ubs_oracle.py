@@ -501,3 +501,72 @@ def image_loss(x: np.ndarray, y: np.ndarray, lambda_dssim: float):
         loss = loss + lambda_dssim * (1.0 - s.mean())
     loss.backward()
     return float(loss), tx.grad.numpy()
+
+
+# ---------------------------------------------------------------------------
+# Step after the backward: Adam (scene/beta_model.py:239-276 builds
+# torch.optim.Adam(groups, lr=0.0, eps=1e-15)), covariance-shaped position
+# noise (train.py:159-168) and the MCMC row relocation
+# (beta_model.py:553-658).
+# ---------------------------------------------------------------------------
+def adam_step(p, g, m, v, lr, step, beta1=0.9, beta2=0.999, eps=1e-15):
+    """One Adam update of a tensor in fp64 (torch.optim.Adam, no weight
+    decay, no amsgrad): returns (p', m', v') for the step count ``step``
+    (>= 1, the count after this update)."""
+    p, g, m, v = (np.asarray(a, np.float64) for a in (p, g, m, v))
+    m = beta1 * m + (1.0 - beta1) * g
+    v = beta2 * v + (1.0 - beta2) * g * g
+    bc1 = 1.0 - beta1 ** step
+    bc2 = 1.0 - beta2 ** step
+    p = p - (lr / bc1) * m / (np.sqrt(v) / math.sqrt(bc2) + eps)
+    return p, m, v
+
+
+def xyz_covariance(l_triangle, scale_raw):
+    """get_xyz_covariance (beta_model.py:143-152): the spatial 3x3 block of
+    the covariance, through the oracle's rotmat_fwd / covar_fwd."""
+    lt = np.asarray(l_triangle, np.float64)
+    s = _softplus(np.asarray(scale_raw, np.float64))
+    D = s.shape[1]
+    R = rotmat_fwd(np.ascontiguousarray(lt[:, :3]))
+    ti, tj = np.tril_indices(D, -1)
+    keep = (ti >= 3) | (tj >= 3)
+    return covar_fwd(R, s, lt, ti[keep].astype(np.int32), tj[keep].astype(np.int32), spatial_block=True)
+
+
+def xyz_noise(xyz, l_triangle, scale_raw, opacity_raw, normal, noise_lr, xyz_lr):
+    """train.py:159-168 in fp64: xyz + Sigma_xyz (z (1 - o)^100 noise_lr xyz_lr)."""
+    o = _sigmoid(np.asarray(opacity_raw, np.float64))
+    z = np.asarray(normal, np.float64) * (1.0 - o) ** 100 * noise_lr * xyz_lr
+    S = xyz_covariance(l_triangle, scale_raw)
+    return np.asarray(xyz, np.float64) + np.einsum("nij,nj->ni", S, z)
+
+
+def split_opacity_raw(opacity_raw, ratio, min_opacity=0.005):
+    """_update_params (beta_model.py:552-570): the pre-activation opacity of
+    each of ratio + 1 copies, logit(clamp(1 - (1 - o)^(1/(ratio+1))))."""
+    o = _sigmoid(np.asarray(opacity_raw, np.float64))
+    on = 1.0 - (1.0 - o) ** (1.0 / (np.asarray(ratio, np.float64) + 1.0))
+    on = np.clip(on, min_opacity, 1.0 - float(np.finfo(np.float32).eps))
+    return np.log(on / (1.0 - on))
+
+
+def mcmc_relocate(leaves: dict, exp_avg: dict, exp_avg_sq: dict, src, dst, ratio, opacity_key="opacity_raw"):
+    """The row updates of relocate_gs / add_new_gs (beta_model.py:580-658)
+    for sampled (src, dst, ratio): dst rows take src rows (opacity split),
+    then src rows take the split opacity and zeroed moments.  Arrays are
+    copied, fp64."""
+    L = {k: np.array(v, np.float64) for k, v in leaves.items()}
+    M = {k: np.array(v, np.float64) for k, v in exp_avg.items()}
+    V = {k: np.array(v, np.float64) for k, v in exp_avg_sq.items()}
+    src = np.asarray(src, np.int64)
+    dst = np.asarray(dst, np.int64)
+    new_o = split_opacity_raw(L[opacity_key][src, 0], ratio)
+    for k in L:
+        L[k][dst] = L[k][src]
+    L[opacity_key][dst, 0] = new_o
+    L[opacity_key][src, 0] = new_o
+    for k in M:
+        M[k][src] = 0.0
+        V[k][src] = 0.0
+    return L, M, V

@@ -79,6 +79,10 @@ _SIGNATURES = {
     "ubs_image_loss_workspace_bytes": (_c_size, [_c_i64, _c_i64, _c_i64, _c_i64]),
     "ubs_image_loss_fwd": (_c_int, [_c_i64, _c_i64, _c_i64, _c_i64, _vp, _vp, _c_f32, _vp, _vp, _c_size, _vp]),
     "ubs_image_loss_bwd": (_c_int, [_c_i64, _c_i64, _c_i64, _c_i64, _vp, _vp, _c_f32, _vp, _vp, _c_size, _vp, _vp]),
+    "ubs_adam_step": (_c_int, [_c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_double, ctypes.c_double,
+                               ctypes.c_double, _vp]),
+    "ubs_xyz_noise": (_c_int, [_c_i64, _c_i64, _vp, _vp, _vp, _vp, _vp, _c_f32, _vp]),
+    "ubs_mcmc_relocate": (_c_int, [_c_i64, _c_int, _vp, _vp, _c_int, _vp, _vp, _vp, _vp, _vp, _c_f32, _vp]),
     "ubs_set_backward_fp64_transmittance": (_c_int, [_c_int]),
     "ubs_set_cond_backward_exact": (_c_int, [_c_int]),
 }
