@@ -11,7 +11,7 @@ and the backward of all of it down to the leaf parameters.  Inputs are synthetic
 timed region.  Prints one JSON line: throughput, latency percentiles, the
 roofline of the dominant kernel and a CPU baseline from the oracle.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c3|c1]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c3|c1|c4|c5]
 """
 from __future__ import annotations
 
@@ -42,6 +42,14 @@ CONFIGS = {
     "c4": dict(N=3_000_000, D=6, views=8, sizes=((960, 540), (1280, 720), (1920, 1080)),
                workload="3M-primitive static 6-D Beta scene, 8 views at 960x540/1280x720/1920x1080, "
                         "one batched FP32 fwd+bwd step"),
+    # render serving (not the headline metric): 4 resident scenes, views of
+    # mixed resolution arriving as a Poisson process, forward only
+    "c5": dict(N=2_000_000, scenes=(("static6_a", 6, 0), ("static6_b", 6, 1), ("dynamic7_a", 7, 2),
+                                    ("dynamic7_b", 7, 3)),
+               sizes=((640, 480), (1280, 720), (1920, 1080)), rates=(50, 100, 200, 400), duration_s=1.5,
+               saturation_views=64,
+               workload="4 resident scenes of 2M primitives (two 6-D static, two 7-D dynamic), views at "
+                        "640x480/1280x720/1920x1080 on a radius U[3,5] sphere, t U[0,1], FP32 forward"),
 }
 
 
@@ -165,7 +173,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS),
                     help="c2 (default): the headline 1080p / 1M workload; c3: 7-D; c1: small; "
-                         "c4: batched 8-view step at mixed resolutions (reports views/s)")
+                         "c4: batched 8-view step at mixed resolutions (reports views/s); "
+                         "c5: render serving of 4 resident scenes (reports MPix/s, p50/p99)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--lambda-dssim", type=float, default=0.0,
                     help="image loss (1 - l) L1 + l (1 - SSIM) in the fused loss kernels; 0 (default) is the L1 "
@@ -200,6 +209,8 @@ def main():
     cfg = CONFIGS[args.config]
     if "views" in cfg:
         return batch_bench(args, cfg)
+    if "scenes" in cfg:
+        return serve_bench(args, cfg)
     dev = torch.device("cuda:0")
     torch.manual_seed(0)
     sc = scenes.make_scene(cfg["N"], cfg["D"], seed=0)
@@ -283,6 +294,11 @@ def main():
     torch.cuda.synchronize()
     fwd_ms = (time.perf_counter() - f0) * 1e3 / args.steps
 
+    # ---- batched serving of the same scene: 8 views at the same size in one
+    # pass (fused projection, one isect/sort/blend over all views) against
+    # the same 8 views rendered one at a time; forward only
+    serve = serve_views(params, cfg, args.steps)
+
     # ---- roofline of the dominant kernel launch
     per_call = {k: (sum(v) / len(v), len(v) / args.steps) for k, v in launches.items()}
     dominant = max(per_call, key=lambda k: per_call[k][0] * per_call[k][1])
@@ -329,9 +345,145 @@ def main():
         "fwd_bwd_mpix_per_s": round(cfg["W"] * cfg["H"] * 1e3 / ms_per_step / 1e6, 2),
         "latency_ms": {"p50": round(step_ms[len(step_ms) // 2], 3),
                        "p99": round(step_ms[min(len(step_ms) - 1, int(math.ceil(0.99 * len(step_ms))) - 1)], 3)},
+        "batched_serve": serve,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "stages": stages,
+    }
+    print(json.dumps(line))
+
+
+def serve_views(params, cfg, steps: int, views: int = 8) -> dict:
+    """Forward-only rate of `views` cameras of the scene rendered as one
+    batch (beta.render_views over all views) and one view at a time."""
+    import torch
+
+    from gsplat import beta, scenes
+
+    dev = params.xyz.device
+    W, H = cfg["W"], cfg["H"]
+    vms, Ks, ts = scenes.make_cameras(views, W, H, seed=7)
+    vms_t, Ks_t = torch.tensor(vms, device=dev), torch.tensor(Ks, device=dev)
+    ts_t = torch.tensor(ts, device=dev) if cfg["D"] == 7 else None
+
+    def batched():
+        with torch.no_grad():
+            return beta.render_views(params, vms_t, Ks_t, W, H, timestamps=ts_t)
+
+    def one_at_a_time():
+        with torch.no_grad():
+            for c in range(views):
+                beta.render_views(params, vms_t[c:c + 1], Ks_t[c:c + 1], W, H,
+                                  timestamps=None if ts_t is None else ts_t[c:c + 1])
+
+    out = {}
+    for name, fn in (("batched", batched), ("per_view", one_at_a_time)):
+        for _ in range(2):
+            fn()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(steps):
+            ev[k][0].record()
+            fn()
+            ev[k][1].record()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        lat = sorted(a.elapsed_time(b) for a, b in ev)
+        out[name] = {"ms_per_batch": round(ms, 3), "mpix_per_s": round(views * W * H * 1e3 / ms / 1e6, 1),
+                     "frames_per_s": round(views * 1e3 / ms, 1), "p50_ms": round(lat[len(lat) // 2], 3),
+                     "p99_ms": round(lat[min(len(lat) - 1, int(math.ceil(0.99 * len(lat))) - 1)], 3)}
+    return {"views_per_batch": views, "width": W, "height": H, **out,
+            "speedup": round(out["per_view"]["ms_per_batch"] / out["batched"]["ms_per_batch"], 3)}
+
+
+def serve_bench(args, cfg):
+    """Render serving (gsplat.serve): resident scenes, Poisson arrivals of
+    view requests of mixed resolution, continuous batching against one view
+    at a time.  value = MPix/s of the batched server with every request
+    pending at once (saturation); p50/p99 latency per arrival rate."""
+    import numpy as np
+    import torch
+
+    from gsplat import beta, scenes
+    from gsplat.serve import RenderServer, SceneStore, ViewRequest, latency_summary
+
+    dev = torch.device("cuda:0")
+    store = SceneStore(dev)
+    for name, D, seed in cfg["scenes"]:
+        sc = scenes.make_scene(cfg["N"], D, seed=seed)
+        store.add(name, beta.BetaParams.from_scene(sc, dev, requires_grad=False, use_quats=(D == 6)))
+        del sc
+        vm = torch.tensor(scenes.look_at_viewmat(np.array([0.0, 0.4, 4.0])), dtype=torch.float32)
+        store.calibrate(name, vm, torch.tensor(scenes.intrinsics(1920, 1080), dtype=torch.float32), t=0.5)
+        print(f"scene {name}: {cfg['N']} primitives, {D}-D, {store.isects_per_prim[name]:.1f} isects/primitive",
+              file=sys.stderr, flush=True)
+    rng = np.random.Generator(np.random.PCG64(11))
+    names = [n for n, _, _ in cfg["scenes"]]
+
+    def make_request(i):
+        W, H = cfg["sizes"][int(rng.integers(0, len(cfg["sizes"])))]
+        d = rng.normal(0, 1, 3)
+        vm = scenes.look_at_viewmat(rng.uniform(3, 5) * d / np.linalg.norm(d))
+        return ViewRequest(names[int(rng.integers(0, len(names)))], torch.tensor(vm, dtype=torch.float32),
+                           torch.tensor(scenes.intrinsics(W, H), dtype=torch.float32), W, H,
+                           t=float(rng.uniform(0, 1)), rid=i)
+
+    modes = {"batched": dict(max_views=16, streams=2), "per_view": dict(max_views=1, streams=1)}
+
+    def run(reqs, arrivals, mode):
+        srv = RenderServer(store, keep_images=False, **modes[mode])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        i, done = 0, []
+        while i < len(reqs) or srv.busy():
+            now = time.perf_counter() - t0
+            while i < len(reqs) and arrivals[i] <= now:
+                r = reqs[i]
+                srv.submit(ViewRequest(r.scene, r.viewmat, r.K, r.width, r.height, r.t, arrival=t0 + arrivals[i],
+                                       rid=r.rid))
+                i += 1
+            if not srv.poll() and not srv.busy() and i < len(reqs):
+                time.sleep(min(1e-4, max(0.0, arrivals[i] - (time.perf_counter() - t0))))
+            done += srv.results
+            srv.results = []
+        wall = max(r.done for r in done) - t0
+        return latency_summary(done, wall)
+
+    warm = [make_request(-1 - k) for k in range(12)]
+    for mode in modes:
+        run(warm, [0.0] * len(warm), mode)
+
+    n_sat = cfg["saturation_views"]
+    sat_reqs = [make_request(k) for k in range(n_sat)]
+    saturation = {}
+    for mode in modes:
+        saturation[mode] = run(sat_reqs, [0.0] * n_sat, mode)
+        print(f"saturation {mode}: {saturation[mode]}", file=sys.stderr, flush=True)
+    poisson = {}
+    for lam in cfg["rates"]:
+        gaps = rng.exponential(1.0 / lam, size=int(lam * cfg["duration_s"] * 2) + 8)
+        arr = np.cumsum(gaps)
+        arr = arr[arr < cfg["duration_s"]]
+        reqs = [make_request(k) for k in range(arr.size)]
+        poisson[str(lam)] = {}
+        for mode in modes:
+            poisson[str(lam)][mode] = run(reqs, list(arr), mode)
+            print(f"poisson {lam}/s {mode}: {poisson[str(lam)][mode]}", file=sys.stderr, flush=True)
+    b, pv = saturation["batched"], saturation["per_view"]
+    line = {
+        "metric": "batched-serve MPix/s (4 resident 2M-primitive scenes, mixed resolution); p50/p99 latency",
+        "value": b["mpix_per_s"], "unit": "MPix/s", "n_gpus": 1, "steps": n_sat, "warmup": len(warm),
+        "ms_per_step": round(1e3 / b["frames_per_s"], 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (seeded scenes, cameras and arrivals, gsplat/scenes.py)",
+        "config": {"workload": cfg["workload"], "primitives_per_scene": cfg["N"], "scenes": len(names),
+                   "resident_gb": round(store.resident_bytes() / 1e9, 3), "max_views_per_batch": 16,
+                   "streams": 2, "peak_hbm_gb": round(torch.cuda.max_memory_allocated() / 1e9, 2)},
+        "saturation": saturation,
+        "speedup_vs_per_view": round(b["mpix_per_s"] / pv["mpix_per_s"], 3),
+        "poisson": poisson,
+        "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(line))
 

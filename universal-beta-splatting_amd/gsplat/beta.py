@@ -217,26 +217,37 @@ def render_batch(params: BetaParams, viewmats: Tensor, Ks: Tensor, sizes, timest
         if cam_pos is None:
             cam_pos = camera_centers(viewmats.detach().to(torch.float32))
         cam_pos = cam_pos.detach().to(torch.float32).contiguous()
-    wh = torch.tensor(sizes, dtype=torch.int32, device=dev)
+    # project the views ordered by size group, so that each group's
+    # projected rows are one contiguous slice [c0:c1] of the [C,N,*] outputs
+    order = sorted(range(C), key=lambda c: (sizes[c], c))
+    if order != list(range(C)):
+        perm = torch.tensor(order, dtype=torch.long, device=dev)
+        viewmats, Ks, cam_pos = viewmats.index_select(0, perm), Ks.index_select(0, perm), cam_pos.index_select(0, perm)
+        if timestamps is not None:
+            timestamps = timestamps.index_select(0, perm)
+        if backgrounds is not None:
+            backgrounds = backgrounds.index_select(0, perm)
+    wh = torch.tensor([sizes[c] for c in order], dtype=torch.int32, device=dev)
     radii, means2d, depths, conics, opacities, betas = beta_project(
         params.xyz, params.mean_rest, params.scale_raw, params.l_triangle, params.opacity_raw, params.beta_raw,
         params.quats, viewmats, Ks, 0, 0, cam_pos=cam_pos, timestamps=timestamps, image_sizes=wh, eps2d=eps2d,
         near_plane=near_plane, far_plane=far_plane, radius_clip=radius_clip)
-    groups = {}
-    for c, wh_c in enumerate(sizes):
-        groups.setdefault(wh_c, []).append(c)
     colors_out, alphas_out = [None] * C, [None] * C
-    for (W, H), idx in groups.items():
-        sel = torch.tensor(idx, dtype=torch.long, device=dev)
-        Cg = len(idx)
+    c0 = 0
+    while c0 < C:
+        W, H = sizes[order[c0]]
+        c1 = c0
+        while c1 < C and sizes[order[c1]] == (W, H):
+            c1 += 1
+        Cg = c1 - c0
         tw, th = math.ceil(W / float(tile_size)), math.ceil(H / float(tile_size))
-        m2, r, d = means2d.index_select(0, sel), radii.index_select(0, sel), depths.index_select(0, sel)
+        m2, r, d = means2d[c0:c1], radii[c0:c1], depths[c0:c1]
         _, isect_ids, flatten_ids = isect_tiles(m2, r, d, tile_size, tw, th, n_cameras=Cg)
         offsets = isect_offset_encode(isect_ids, Cg, tw, th)
-        bg = backgrounds.index_select(0, sel) if backgrounds is not None else None
-        rc, ra = rasterize_to_pixels(m2, conics.index_select(0, sel), params.rgb.expand(Cg, -1, -1),
-                                     opacities.index_select(0, sel), betas.index_select(0, sel), W, H, tile_size,
-                                     offsets, flatten_ids, backgrounds=bg)
-        for k, c in enumerate(idx):
-            colors_out[c], alphas_out[c] = rc[k], ra[k]
+        bg = backgrounds[c0:c1] if backgrounds is not None else None
+        rc, ra = rasterize_to_pixels(m2, conics[c0:c1], params.rgb.expand(Cg, -1, -1), opacities[c0:c1],
+                                     betas[c0:c1], W, H, tile_size, offsets, flatten_ids, backgrounds=bg)
+        for k in range(Cg):
+            colors_out[order[c0 + k]], alphas_out[order[c0 + k]] = rc[k], ra[k]
+        c0 = c1
     return colors_out, alphas_out
