@@ -310,12 +310,14 @@ TRIU = ([0, 0, 0, 1, 1, 2], [0, 1, 2, 1, 2, 2])
 
 def rasterization(means, covars33, opacities, betas, colors, viewmats, Ks, width, height,
                   near_plane=0.01, far_plane=1e10, radius_clip=0.0, eps2d=0.3, tile_size=16,
-                  backgrounds=None, structure=None, quats=None, scales=None):
+                  backgrounds=None, structure=None, quats=None, scales=None, tile_mask=None):
     """Forward of rendering.py:rasterization for one dtype.
 
     ``structure`` lets an FP64 evaluation reuse the float32 pipeline's tile
     lists (isect_offsets, flatten_ids) so both blend in the same order.
-    Returns (render_colors, render_alphas, meta).
+    ``tile_mask`` [C,th,tw] blends only the tiles it marks (the blend's
+    tile masks, rasterize_to_pixels_fwd.cu:47-56); the others get the
+    background.  Returns (render_colors, render_alphas, meta).
     """
     dt = means.dtype
     C = viewmats.shape[0]
@@ -334,7 +336,7 @@ def rasterization(means, covars33, opacities, betas, colors, viewmats, Ks, width
     else:
         tpg, ids, flat, offs = structure
     out_c, out_a, last = raster_fwd(means2d, conics, cols, opac, bet, width, height, tile_size, offs,
-                                    flat, backgrounds)
+                                    flat, backgrounds, masks=tile_mask)
     meta = dict(radii=radii, means2d=means2d, depths=depths, conics=conics, opacities=opac,
                 betas=bet, colors=cols, tile_width=tw, tile_height=th, tiles_per_gauss=tpg,
                 isect_ids=ids, flatten_ids=flat, isect_offsets=offs, last_ids=last,
@@ -379,13 +381,16 @@ def _quat_R_vjp(q, qn, G):
 
 
 def beta_step(leaves: dict, viewmat, K, width, height, target, timestamp=0.0, dtype=np.float64,
-              tile_size=16, structure=None, want_grads=True, exact_cond=False, fixed_query=None):
+              tile_size=16, structure=None, want_grads=True, exact_cond=False, fixed_query=None,
+              tile_mask=None):
     """One forward+backward of the Beta rasteriser path on the CPU.
 
     leaves: xyz, mean_rest, scale_raw, l_triangle, opacity_raw, beta_raw, rgb,
     quats (numpy).  Loss = mean |image - target|.  Returns (image [H,W,3],
     grads dict, meta).  ``structure`` (from a float32 run's meta) makes an
-    FP64 evaluation blend in the float32 tile lists.
+    FP64 evaluation blend in the float32 tile lists.  ``tile_mask``
+    [th,tw] restricts the blend and the loss to the marked tiles (the loss
+    is then the mean over their pixels).
     """
     dt = dtype
     L = {k: np.asarray(v, dtype=np.float64) for k, v in leaves.items()}
@@ -414,20 +419,25 @@ def beta_step(leaves: dict, viewmat, K, width, height, target, timestamp=0.0, dt
         q = np.asarray(fixed_query, np.float64)
     bcond = np.ascontiguousarray(beta[:, 1:])
     m, V, o = cond_fwd(c_(mean), cov, c_(opac), c_(bcond), c_(q))
+    tmask = None if tile_mask is None else np.ascontiguousarray(np.asarray(tile_mask, bool)[None])
     img, alpha, meta = rasterization(m, V, c_(o[:, 0]), c_(beta[:, 0]), c_(L["rgb"]), c_(vm[None]),
                                      c_(np.asarray(K)[None]), width, height, tile_size=tile_size,
-                                     structure=structure)
+                                     structure=structure, tile_mask=tmask)
     meta["query"] = q
     if not want_grads:
         return img[0], None, meta
     H, W_ = height, width
-    v_img = np.sign(img[0].astype(np.float64) - target) / (H * W_ * 3)
+    if tmask is None:
+        v_img = np.sign(img[0].astype(np.float64) - target) / (H * W_ * 3)
+    else:
+        pm = np.kron(tmask[0], np.ones((tile_size, tile_size), bool))[:H, :W_]
+        v_img = np.sign(img[0].astype(np.float64) - target) * pm[..., None] / (pm.sum() * 3)
     v_img = c_(v_img[None])
     T_final = c_(1.0 - alpha[..., 0])
     vm2, vcn, vcol, vop, vbe = raster_bwd(meta["means2d"], meta["conics"], meta["colors"], meta["opacities"],
                                           meta["betas"], width, height, tile_size, meta["isect_offsets"],
                                           meta["flatten_ids"], alpha, meta["last_ids"], v_img,
-                                          np.zeros_like(alpha), T_final=T_final)
+                                          np.zeros_like(alpha), masks=tmask, T_final=T_final)
     v_means, v_cov6, _, _, _ = proj_bwd(m, meta["covars6"], None, None, c_(vm[None]), c_(np.asarray(K)[None]),
                                         width, height, 0.3, meta["radii"], meta["conics"], None, vm2,
                                         np.zeros_like(meta["depths"]), vcn)
@@ -500,7 +510,7 @@ def image_loss(x: np.ndarray, y: np.ndarray, lambda_dssim: float):
         s = _ssim_map_torch(tx.permute(0, 3, 1, 2), ty.permute(0, 3, 1, 2), g)
         loss = loss + lambda_dssim * (1.0 - s.mean())
     loss.backward()
-    return float(loss), tx.grad.numpy()
+    return float(loss.detach()), tx.grad.numpy()
 
 
 # ---------------------------------------------------------------------------

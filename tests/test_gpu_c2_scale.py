@@ -1,0 +1,130 @@
+"""Parity at the full size of the headline workload: 1M-primitive 6-D scene,
+one 1920x1080 view (the benchmark's scene and camera).
+
+* Structure through the drop-in chain -- covariance, conditioned mean and
+  covariance, projection, tile intersections, sorted keys and values, tile
+  ranges -- is bit-exact against the oracle's float32 restatement at full
+  size (the conditioned opacity, through pow/tanh, to rtol 2e-5 + 1e-9).
+* The image and every leaf gradient on 64 random tiles (tile seed 1) are
+  checked against the FP64 oracle: the L1 loss is taken over those tiles'
+  pixels only, the oracle blends those tiles only (tile masks) in the GPU's
+  tile lists.  Image max-abs <= 1e-4 on pixels away from the T cut-off;
+  gradients relative L2 <= 1e-3.
+* The fused projection path renders the same image bit for bit.
+"""
+import numpy as np
+import pytest
+import torch
+
+from gsplat import beta, rasterization, scenes
+from gsplat.cuda import _wrapper as W
+from oracle import ubs_oracle as O
+
+from util import assert_same, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+N, D, WD, HT = 1_000_000, 6, 1920, 1080
+
+
+def host(t):
+    return t.detach().cpu().numpy()
+
+
+@pytest.fixture(scope="module")
+def c2(device):
+    sc = scenes.make_scene(N, D, seed=0)
+    vms, Ks, ts = scenes.make_cameras(1, WD, HT, seed=0)
+    return sc, vms, Ks
+
+
+def _tile_mask(n_tiles=64, seed=1):
+    tw, th = -(-WD // 16), -(-HT // 16)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    pick = rng.choice(tw * th, n_tiles, replace=False)
+    m = np.zeros(th * tw, bool)
+    m[pick] = True
+    return m.reshape(th, tw)
+
+
+def test_c2_structure_bit_exact(device, c2):
+    sc, vms, Ks = c2
+    print(f"c2 structure: N={N} {WD}x{HT}", flush=True)
+    p = beta.BetaParams.from_scene(sc, device, requires_grad=False)
+    vm, K = torch.tensor(vms[0], device=device), torch.tensor(Ks[0], device=device)
+    scale = torch.nn.functional.softplus(p.scale_raw)
+    opac = torch.sigmoid(p.opacity_raw)
+    bet = 4.0 * torch.exp(p.beta_raw)
+    rot = beta.quat_to_rotmat(p.quats)
+    ri, rj = beta.rest_indices(D, device)
+    cov = W.rot_scale_l_triangle_to_covar(rot, scale, p.l_triangle, ri, rj)
+    ocov = O.covar_fwd(host(rot), host(scale), host(p.l_triangle), host(ri), host(rj))
+    assert_same(host(cov), ocov, "covar")
+    mean = torch.cat([p.xyz, p.mean_rest], -1)
+    vd = p.xyz - beta.camera_center(vm)[None]
+    q = vd / vd.norm(dim=-1, keepdim=True)
+    m, V, o = W.cond_mean_convariance_opacity(mean, cov, opac, bet[:, 1:].contiguous(), q)
+    om, oV, oo = O.cond_fwd(host(mean), ocov, host(opac), host(bet[:, 1:]), host(q))
+    assert_same(host(m), om, "m_cond")
+    assert_same(host(V), oV, "V_cond")
+    # the conditioned opacity goes through pow/tanh (device vs host libm): not part of the structure
+    np.testing.assert_allclose(host(o), oo, rtol=2e-5, atol=1e-9)
+    img, alpha, meta = rasterization(m, p.l_triangle, scale, o[:, 0], bet[:, 0].contiguous(), p.rgb, vm[None],
+                                     K[None], WD, HT, covars=V)
+    tm = _tile_mask()
+    ref_img, _, rm = O.rasterization(om, oV, host(o[:, 0]), host(bet[:, 0]), host(p.rgb), vms[:1], Ks[:1], WD, HT,
+                                     tile_mask=tm[None])
+    for k in ("radii", "tiles_per_gauss", "isect_ids", "flatten_ids", "isect_offsets"):
+        assert_same(host(meta[k]), rm[k], k)
+    vis = rm["radii"] > 0
+    for k in ("means2d", "depths", "conics"):
+        assert_same(host(meta[k])[vis], rm[k][vis], k)
+    assert rm["isect_ids"].size > 10_000_000
+    # the float32 blend of the same lists on the 64 tiles, away from the T cut-off
+    _, _, _, mT, mS = O.raster_fwd(rm["means2d"], rm["conics"], rm["colors"], rm["opacities"], rm["betas"], WD, HT,
+                                   16, rm["isect_offsets"], rm["flatten_ids"], masks=tm[None], margins=True)
+    pm = np.kron(tm, np.ones((16, 16), bool))[:HT, :WD]
+    safe = pm & (mT[0] > 1e-3) & (mS[0] > 1e-6)
+    assert safe.sum() > 0.95 * pm.sum()
+    assert np.abs(host(img[0]) - ref_img[0])[safe].max() <= 5e-5
+
+
+def test_c2_tiles_image_and_gradients_vs_fp64(device, c2):
+    sc, vms, Ks = c2
+    print(f"c2 fp64 tiles: N={N} {WD}x{HT}", flush=True)
+    tgt = np.random.Generator(np.random.PCG64(5)).uniform(0, 1, (HT, WD, 3))
+    tm = _tile_mask()
+    pm = np.kron(tm, np.ones((16, 16), bool))[:HT, :WD]
+    p = beta.BetaParams.from_scene(sc, device)
+    img, alpha, meta = beta.render(p, torch.tensor(vms[0], device=device), torch.tensor(Ks[0], device=device), WD, HT)
+    pm_t = torch.tensor(pm, device=device)[..., None]
+    loss = ((img[0] - torch.tensor(tgt, dtype=torch.float32, device=device)).abs() * pm_t).sum() / (pm.sum() * 3)
+    loss.backward()
+    struct = tuple(host(meta[k]) for k in ("tiles_per_gauss", "isect_ids", "flatten_ids", "isect_offsets"))
+    leaves = dict(xyz=sc.xyz, mean_rest=sc.mean_rest, scale_raw=sc.scale_raw, l_triangle=sc.l_triangle,
+                  opacity_raw=sc.opacity_raw, beta_raw=sc.beta_raw, rgb=sc.rgb, quats=sc.quats)
+    ref_img, ref_g, rm = O.beta_step(leaves, vms[0], Ks[0], WD, HT, tgt, dtype=np.float64, structure=struct,
+                                     tile_mask=tm)
+    _, _, _, mT, mS = O.raster_fwd(rm["means2d"], rm["conics"], rm["colors"], rm["opacities"], rm["betas"], WD, HT,
+                                   16, rm["isect_offsets"], rm["flatten_ids"], masks=tm[None], margins=True)
+    safe = pm & (mT[0] > 1e-3) & (mS[0] > 1e-6)
+    assert safe.sum() > 0.95 * pm.sum()
+    err = np.abs(host(img[0]) - ref_img)[safe].max()
+    assert err <= 1e-4, err
+    touched = 0
+    for k, t in p.leaves().items():
+        g = host(t.grad)
+        touched = max(touched, int((np.abs(ref_g[k]) > 0).any(axis=-1).sum()))
+        r = rel_l2(g, ref_g[k])
+        assert r < 1e-3, (k, r)
+    assert touched > 1000  # the 64 tiles reach many primitives
+
+
+def test_c2_fused_path_renders_the_same_image(device, c2):
+    sc, vms, Ks = c2
+    p = beta.BetaParams.from_scene(sc, device, requires_grad=False)
+    vm, K = torch.tensor(vms[0], device=device), torch.tensor(Ks[0], device=device)
+    with torch.no_grad():
+        a, aa, _ = beta.render(p, vm, K, WD, HT)
+        b, bb, _ = beta.render_views(p, vm[None].contiguous(), K[None].contiguous(), WD, HT)
+    assert torch.equal(a, b) and torch.equal(aa, bb)

@@ -216,3 +216,35 @@ def test_beta_step_central_differences():
         fd = (f(leaves[name] + eps * d) - f(leaves[name] - eps * d)) / (2 * eps)
         an = (grads[name] * d).sum()
         assert abs(an - fd) <= 1e-4 * max(abs(fd), 1e-3 * np.abs(w).sum()), (name, an, fd)
+
+
+def test_beta_step_tile_mask():
+    """A tile mask restricts the blend and the loss to its tiles: an all-true
+    mask changes nothing; a partial mask reproduces the unmasked image on its
+    tiles, the background elsewhere, and the gradient of the loss taken over
+    its pixels only (checked against the unmasked run with that loss)."""
+    sc = scenes.make_scene(300, 6, seed=12)
+    W, H = 64, 48
+    vms, Ks, _ = scenes.make_cameras(1, W, H, seed=12, radius=2.5)
+    leaves = {k: getattr(sc, k).astype(np.float64) for k in
+              ("xyz", "mean_rest", "scale_raw", "l_triangle", "opacity_raw", "beta_raw", "rgb", "quats")}
+    leaves["scale_raw"][:, :3] += 2.0
+    tgt = np.random.Generator(np.random.PCG64(3)).uniform(0, 1, (H, W, 3))
+    img, grads, _ = O.beta_step(leaves, vms[0], Ks[0], W, H, tgt)
+    full = np.ones((3, 4), bool)
+    img_f, grads_f, _ = O.beta_step(leaves, vms[0], Ks[0], W, H, tgt, tile_mask=full)
+    np.testing.assert_array_equal(img_f, img)
+    for k in grads:
+        np.testing.assert_array_equal(grads_f[k], grads[k])
+    tm = np.zeros((3, 4), bool)
+    tm[0, 1] = tm[2, 3] = tm[1, 0] = True
+    pm = np.kron(tm, np.ones((16, 16), bool))
+    img_m, grads_m, _ = O.beta_step(leaves, vms[0], Ks[0], W, H, tgt, tile_mask=tm)
+    np.testing.assert_array_equal(img_m[pm], img[pm])
+    assert np.all(img_m[~pm] == 0.0)  # black background
+    # unmasked run with the loss over the masked pixels: the target equals the image elsewhere
+    tgt2 = np.where(pm[..., None], tgt, img)
+    _, grads2, _ = O.beta_step(leaves, vms[0], Ks[0], W, H, tgt2)
+    scale = (H * W) / pm.sum()  # mean over all pixels vs over the masked ones
+    for k in grads:
+        np.testing.assert_allclose(grads_m[k], grads2[k] * scale, rtol=1e-9, atol=1e-15)
