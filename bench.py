@@ -186,6 +186,9 @@ def main():
     ap.add_argument("--path", default="fused", choices=("fused", "chain"),
                     help="fused: one-launch Beta projection (render_views); chain: the reference's op chain "
                          "(render: activations, rotation, covariance, conditioning, projection as separate ops)")
+    ap.add_argument("--transmittance", default="recompute", choices=("recompute", "checkpoint"),
+                    help="per-pixel transmittance in the blend backward: recomputed by division from the final T "
+                         "(default, as the reference) or stored by the forward after every batch of 256 entries")
     ap.add_argument("--fp64-transmittance", action="store_true",
                     help="recover transmittance in fp64 in the blend backward")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -224,6 +227,7 @@ def main():
     rest = beta.rest_indices(cfg["D"], dev)
     leaves = list(params.leaves().values())
     W.set_backward_fp64_transmittance(args.fp64_transmittance)
+    W.set_backward_transmittance_checkpoint(args.transmittance == "checkpoint")
 
     vms_t, Ks_t = vm[None].contiguous(), K[None].contiguous()
     ts_t = torch.tensor([t_view], dtype=torch.float32, device=dev)
@@ -340,7 +344,8 @@ def main():
                    "width": cfg["W"], "height": cfg["H"], "views": 1, "tile_size": 16,
                    "n_isects": n_isect, "visible_primitives": n_vis, "projection_path": args.path,
                    "loss": "L1" if args.lambda_dssim == 0 else f"{1 - args.lambda_dssim:g} L1 + {args.lambda_dssim:g} (1 - SSIM)",
-                   "sort_passes_skipped": cfg["sort_skipped"], "optimizer_step": "adam" if args.adam else None},
+                   "sort_passes_skipped": cfg["sort_skipped"], "optimizer_step": "adam" if args.adam else None,
+                   "transmittance": args.transmittance},
         "fwd_frames_per_s": round(1e3 / fwd_ms, 3),
         "fwd_bwd_mpix_per_s": round(cfg["W"] * cfg["H"] * 1e3 / ms_per_step / 1e6, 2),
         "latency_ms": {"p50": round(step_ms[len(step_ms) // 2], 3),
@@ -517,19 +522,29 @@ def batch_bench(args, cfg):
         loss = sum((im - t).abs().mean() for im, t in zip(imgs, tgts))
         loss.backward()
 
-    for _ in range(args.warmup):
-        step()
-    timer = W.LaunchTimer()
-    W.set_launch_timer(timer)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    W.set_launch_timer(None)
-    launches = timer.durations_ms()
-    ms = (t1 - t0) * 1e3 / args.steps
+    # the blend backward's per-pixel transmittance, both ways: recomputed by
+    # division (the default, as the reference) and stored per batch
+    modes = {}
+    for mode in ("checkpoint", "recompute"):
+        W.set_backward_transmittance_checkpoint(mode == "checkpoint")
+        torch.cuda.reset_peak_memory_stats()
+        for _ in range(args.warmup):
+            step()
+        timer = W.LaunchTimer()
+        W.set_launch_timer(timer)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        W.set_launch_timer(None)
+        launches = timer.durations_ms()
+        modes[mode] = {"ms_per_step": round((t1 - t0) * 1e3 / args.steps, 3),
+                       "views_per_s": round(C * 1e3 * args.steps / ((t1 - t0) * 1e3), 3),
+                       "peak_hbm_gb": round(torch.cuda.max_memory_allocated() / 1e9, 2)}
+    W.set_backward_transmittance_checkpoint(False)
+    ms = modes["recompute"]["ms_per_step"]
     stages = {k.replace("ubs_", ""): {"ms_per_step": round(sum(v) / args.steps, 4), "calls_per_step": len(v) / args.steps}
               for k, v in sorted(launches.items(), key=lambda kv: -sum(kv[1]))}
 
@@ -556,8 +571,8 @@ def batch_bench(args, cfg):
         "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "fp32", "data": "synthetic (seeded scene and camera generator, gsplat/scenes.py)",
         "config": {"workload": cfg["workload"], "primitives": cfg["N"], "dims": cfg["D"], "views": C,
-                   "sizes": sizes, "peak_hbm_gb": round(torch.cuda.max_memory_allocated() / 1e9, 2)},
-        "roofline": None, "cpu_baseline": None, "stages": stages,
+                   "sizes": sizes, "transmittance": "recompute", "peak_hbm_gb": modes["recompute"]["peak_hbm_gb"]},
+        "roofline": None, "cpu_baseline": None, "stages": stages, "transmittance_modes": modes,
         "per_view_loop_ms_per_step": round(ms_per_view_loop, 3),
         "speedup_vs_per_view_loop": round(ms_per_view_loop / ms, 3),
     }

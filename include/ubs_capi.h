@@ -330,6 +330,42 @@ int ubs_rasterize_to_pixels_bwd(uint32_t C, uint32_t N, int64_t n_isects, uint32
                                 float *v_opacities, float *v_betas, void *workspace,
                                 size_t workspace_bytes, void *stream);
 
+/* Stored-transmittance variant of the blend pair (the recompute-vs-store
+ * choice for per-pixel transmittance; no counterpart in the reference,
+ * whose backward only recovers T by division, rasterize_to_pixels_bwd.cu:
+ * 202-203).  The forward also writes each pixel's T after every batch of
+ * tile_size^2 entries of its tile's list into T_ckpt; the backward restarts
+ * each batch from it, so the division recovery runs over one batch at a
+ * time.  T_ckpt holds ubs_transmittance_checkpoint_floats(...) floats and
+ * needs no initialisation; the pair must see the same tile lists. */
+size_t ubs_transmittance_checkpoint_floats(uint32_t C, int64_t n_isects, uint32_t tile_width,
+                                           uint32_t tile_height, uint32_t tile_size);
+
+int ubs_rasterize_to_pixels_fwd_ckpt(uint32_t C, uint32_t N, int64_t n_isects, uint32_t channels,
+                                     const float *means2d, const float *conics, const float *colors,
+                                     const float *opacities, const float *betas,
+                                     const float *backgrounds, const uint8_t *masks, uint32_t width,
+                                     uint32_t height, uint32_t tile_size, uint32_t tile_width,
+                                     uint32_t tile_height, const int32_t *tile_offsets,
+                                     const int32_t *flatten_ids, float *render_colors,
+                                     float *render_alphas, int32_t *last_ids, float *T_final,
+                                     float *T_ckpt, size_t ckpt_floats, void *workspace,
+                                     size_t workspace_bytes, void *stream);
+
+int ubs_rasterize_to_pixels_bwd_ckpt(uint32_t C, uint32_t N, int64_t n_isects, uint32_t channels,
+                                     const float *means2d, const float *conics, const float *colors,
+                                     const float *opacities, const float *betas,
+                                     const float *backgrounds, const uint8_t *masks, uint32_t width,
+                                     uint32_t height, uint32_t tile_size, uint32_t tile_width,
+                                     uint32_t tile_height, const int32_t *tile_offsets,
+                                     const int32_t *flatten_ids, const float *render_alphas,
+                                     const int32_t *last_ids, const float *T_final,
+                                     const float *T_ckpt, size_t ckpt_floats,
+                                     const float *v_render_colors, const float *v_render_alphas,
+                                     float *v_means2d, float *v_conics, float *v_colors,
+                                     float *v_opacities, float *v_betas, void *workspace,
+                                     size_t workspace_bytes, void *stream);
+
 /* ------------------------------------------------------------------------
  * Image loss of the training step: (1 - lambda) L1 + lambda (1 - SSIM).
  * Replaces torch's l1_loss (utils/loss_utils.py:18-19) and the external

@@ -184,6 +184,25 @@ static __global__ void raster_unpack_grads_kernel(uint64_t CN, const float4 *__r
     v_betas[i] = r1.z;
 }
 
+// Transmittance checkpoints (optional): the forward stores every pixel's T
+// after each batch of BS = TS^2 entries of its tile's list, the backward
+// starts each batch from that value instead of carrying the division
+// recovery T / (1 - alpha) across batches.  Batch k of the tile with global
+// index L (camera * th * tw + tile) whose list starts at isect index s has
+// slot floor(s / BS) + L + k, BS floats wide (one per pixel of the tile).
+// Slots are unique: the last slot of a tile [s, e) is at most
+// floor(e / BS) + L, one below the next tile's first, floor(e / BS) + L + 1,
+// since floor(s / BS) + ceil((e - s) / BS) <= floor(e / BS) + 1.  So
+// floor(n_isects / BS) + C th tw slots cover every tile.
+__host__ __device__ inline int64_t ckpt_slot(int32_t range_start, uint32_t tile_global, int32_t k, int BS) {
+    return (int64_t)(range_start / BS) + (int64_t)tile_global + k;
+}
+
+inline size_t ckpt_floats(uint32_t C, int64_t n_isects, uint32_t tw, uint32_t th, uint32_t tile_size) {
+    const size_t BS = (size_t)tile_size * tile_size;
+    return ((size_t)n_isects / BS + (size_t)C * tw * th + 1) * BS;
+}
+
 // Channel chunks the kernels are instantiated for; wider colour vectors are
 // blended chunk by chunk (the transmittance does not depend on colour).  The
 // widest chunk is bounded by LDS: 32 channels for 8x8 and 16x16 tiles, 4 for

@@ -44,7 +44,7 @@ __global__ __launch_bounds__(TS *TS, UBS_BWD_WAVES_PER_EU) void raster_bwd_kerne
     const float *__restrict__ v_render_colors, const float *__restrict__ v_render_alphas,
     float *__restrict__ v_means2d, float *__restrict__ v_conics, float *__restrict__ v_colors,
     float *__restrict__ v_opacities, float *__restrict__ v_betas, const float4 *__restrict__ rec,
-    float *__restrict__ vrec) {
+    float *__restrict__ vrec, const float *__restrict__ T_ckpt) {
     constexpr int BS = TS * TS;
     constexpr int SB = TS / 8;
     constexpr int NW = BS / 64;
@@ -119,10 +119,17 @@ __global__ __launch_bounds__(TS *TS, UBS_BWD_WAVES_PER_EU) void raster_bwd_kerne
     for (int p = 0; p < NPASS; ++p)
         my_idx[p] = 16 * p + my_r < NG2 ? wave_total_index<2 * NV>(16 * p + my_r, lane >> 4) : -1;
 
+    // batches are the forward's, [range_start + k BS, ...), taken last first
     const int32_t nb = range_end > range_start ? (range_end - range_start + BS - 1) / BS : 0;
     for (int32_t b = 0; b < nb; ++b) {
-        const int32_t batch_end = range_end - 1 - b * BS;  // isect index of entry t is batch_end - t
-        const int32_t cnt = min(BS, batch_end + 1 - range_start);
+        const int32_t kb = nb - 1 - b;
+        const int32_t batch_lo = range_start + kb * BS;
+        const int32_t batch_end = min(range_end, batch_lo + BS) - 1;  // isect index of entry t is batch_end - t
+        const int32_t cnt = batch_end + 1 - batch_lo;
+        // with checkpoints, a pixel that has entries here restarts from the
+        // forward's T after this batch (the forward ran this batch for it)
+        if (T_ckpt && inside && bin_final >= batch_lo)
+            T = (TT)T_ckpt[ckpt_slot(range_start, logical, kb, BS) * BS + tid];
         if (tid < cnt) {
             const int32_t g = flatten_ids[batch_end - tid];
             float2 xy;
@@ -370,15 +377,15 @@ static int launch_bwd(bool f64t, uint32_t C, uint32_t N, int64_t n_isects, uint3
                       uint32_t tw, uint32_t th, const int32_t *offs, const int32_t *flat, const float *ra,
                       const int32_t *last, const float *Tf, const float *vrc, const float *vra, float *vm2,
                       float *vcon, float *vcol, float *vop, float *vbe, const float4 *rec, float *vrec,
-                      hipStream_t st) {
+                      const float *ck, hipStream_t st) {
     if (f64t)
         raster_bwd_kernel<TS, CDIM, true><<<xcd_grid(C, tw, th), TS * TS, 0, st>>>(
             C, N, n_isects, channels, coff, means2d, conics, colors, opac, betas, bg, masks, W, H, tw, th, offs, flat,
-            ra, last, Tf, vrc, vra, vm2, vcon, vcol, vop, vbe, rec, vrec);
+            ra, last, Tf, vrc, vra, vm2, vcon, vcol, vop, vbe, rec, vrec, ck);
     else
         raster_bwd_kernel<TS, CDIM, false><<<xcd_grid(C, tw, th), TS * TS, 0, st>>>(
             C, N, n_isects, channels, coff, means2d, conics, colors, opac, betas, bg, masks, W, H, tw, th, offs, flat,
-            ra, last, Tf, vrc, vra, vm2, vcon, vcol, vop, vbe, rec, vrec);
+            ra, last, Tf, vrc, vra, vm2, vcon, vcol, vop, vbe, rec, vrec, ck);
     UBS_LAUNCH_CHECK("raster_bwd_kernel");
     return UBS_OK;
 }
@@ -390,10 +397,10 @@ static int launch_bwd_chunk(int cd, bool f64t, uint32_t C, uint32_t N, int64_t n
                             uint32_t H, uint32_t tw, uint32_t th, const int32_t *offs, const int32_t *flat,
                             const float *ra, const int32_t *last, const float *Tf, const float *vrc,
                             const float *vra, float *vm2, float *vcon, float *vcol, float *vop, float *vbe,
-                            const float4 *rec, float *vrec, hipStream_t st) {
+                            const float4 *rec, float *vrec, const float *ck, hipStream_t st) {
 #define UBS_BWD_ARGS                                                                                            \
     f64t, C, N, n_isects, channels, coff, means2d, conics, colors, opac, betas, bg, masks, W, H, tw, th, offs,  \
-        flat, ra, last, Tf, vrc, vra, vm2, vcon, vcol, vop, vbe, rec, vrec, st
+        flat, ra, last, Tf, vrc, vra, vm2, vcon, vcol, vop, vbe, rec, vrec, ck, st
     switch (cd) {
     case 1: return launch_bwd<TS, 1>(UBS_BWD_ARGS);
     case 2: return launch_bwd<TS, 2>(UBS_BWD_ARGS);
@@ -432,15 +439,15 @@ int ubs_set_backward_fp64_transmittance(int enable) {
     return UBS_OK;
 }
 
-int ubs_rasterize_to_pixels_bwd(uint32_t C, uint32_t N, int64_t n_isects, uint32_t channels, const float *means2d,
-                                const float *conics, const float *colors, const float *opacities,
-                                const float *betas, const float *backgrounds, const uint8_t *masks, uint32_t W,
-                                uint32_t H, uint32_t tile_size, uint32_t tw, uint32_t th,
-                                const int32_t *tile_offsets, const int32_t *flatten_ids, const float *render_alphas,
-                                const int32_t *last_ids, const float *T_final, const float *v_render_colors,
-                                const float *v_render_alphas, float *v_means2d, float *v_conics, float *v_colors,
-                                float *v_opacities, float *v_betas, void *workspace, size_t workspace_bytes,
-                                void *stream) {
+static int raster_bwd_impl(uint32_t C, uint32_t N, int64_t n_isects, uint32_t channels, const float *means2d,
+                           const float *conics, const float *colors, const float *opacities, const float *betas,
+                           const float *backgrounds, const uint8_t *masks, uint32_t W, uint32_t H,
+                           uint32_t tile_size, uint32_t tw, uint32_t th, const int32_t *tile_offsets,
+                           const int32_t *flatten_ids, const float *render_alphas, const int32_t *last_ids,
+                           const float *T_final, const float *v_render_colors, const float *v_render_alphas,
+                           float *v_means2d, float *v_conics, float *v_colors, float *v_opacities, float *v_betas,
+                           const float *T_ckpt, size_t ckpt_n, void *workspace, size_t workspace_bytes,
+                           void *stream) {
     hipStream_t st = as_stream(stream);
     UBS_REQUIRE(tile_size == 8 || tile_size == 16 || tile_size == 32, "tile_size must be 8, 16 or 32 (got %u)",
                 tile_size);
@@ -483,6 +490,13 @@ int ubs_rasterize_to_pixels_bwd(uint32_t C, uint32_t N, int64_t n_isects, uint32
     UBS_REQUIRE(means2d && conics && colors && opacities && betas && tile_offsets && flatten_ids && last_ids &&
                     (render_alphas || T_final) && v_render_colors && v_render_alphas,
                 "null pointer");
+    if (T_ckpt) {
+        const size_t need = ckpt_floats(C, n_isects, tw, th, tile_size);
+        if (ckpt_n < need) {
+            set_error("transmittance checkpoint buffer of %zu floats, %zu needed", ckpt_n, need);
+            return UBS_ERR_WORKSPACE;
+        }
+    }
     const bool f64t = g_f64_transmittance != 0;
     uint32_t coff = 0;
     while (coff < channels) {
@@ -491,17 +505,17 @@ int ubs_rasterize_to_pixels_bwd(uint32_t C, uint32_t N, int64_t n_isects, uint32
             e = launch_bwd_chunk<16>(cd, f64t, C, N, n_isects, channels, coff, means2d, conics, colors, opacities,
                                      betas, backgrounds, masks, W, H, tw, th, tile_offsets, flatten_ids,
                                      render_alphas, last_ids, T_final, v_render_colors, v_render_alphas, v_means2d,
-                                     v_conics, v_colors, v_opacities, v_betas, rec, vrec, st);
+                                     v_conics, v_colors, v_opacities, v_betas, rec, vrec, T_ckpt, st);
         else if (tile_size == 8)
             e = launch_bwd_chunk<8>(cd, f64t, C, N, n_isects, channels, coff, means2d, conics, colors, opacities,
                                     betas, backgrounds, masks, W, H, tw, th, tile_offsets, flatten_ids,
                                     render_alphas, last_ids, T_final, v_render_colors, v_render_alphas, v_means2d,
-                                    v_conics, v_colors, v_opacities, v_betas, rec, vrec, st);
+                                    v_conics, v_colors, v_opacities, v_betas, rec, vrec, T_ckpt, st);
         else
             e = launch_bwd_chunk<32>(cd, f64t, C, N, n_isects, channels, coff, means2d, conics, colors, opacities,
                                      betas, backgrounds, masks, W, H, tw, th, tile_offsets, flatten_ids,
                                      render_alphas, last_ids, T_final, v_render_colors, v_render_alphas, v_means2d,
-                                     v_conics, v_colors, v_opacities, v_betas, rec, vrec, st);
+                                     v_conics, v_colors, v_opacities, v_betas, rec, vrec, T_ckpt, st);
         if (e) return e;
         coff += (uint32_t)cd;
     }
@@ -511,6 +525,38 @@ int ubs_rasterize_to_pixels_bwd(uint32_t C, uint32_t N, int64_t n_isects, uint32
         UBS_LAUNCH_CHECK("raster_unpack_grads_kernel");
     }
     return UBS_OK;
+}
+
+int ubs_rasterize_to_pixels_bwd(uint32_t C, uint32_t N, int64_t n_isects, uint32_t channels, const float *means2d,
+                                const float *conics, const float *colors, const float *opacities,
+                                const float *betas, const float *backgrounds, const uint8_t *masks, uint32_t W,
+                                uint32_t H, uint32_t tile_size, uint32_t tw, uint32_t th,
+                                const int32_t *tile_offsets, const int32_t *flatten_ids, const float *render_alphas,
+                                const int32_t *last_ids, const float *T_final, const float *v_render_colors,
+                                const float *v_render_alphas, float *v_means2d, float *v_conics, float *v_colors,
+                                float *v_opacities, float *v_betas, void *workspace, size_t workspace_bytes,
+                                void *stream) {
+    return raster_bwd_impl(C, N, n_isects, channels, means2d, conics, colors, opacities, betas, backgrounds, masks, W,
+                           H, tile_size, tw, th, tile_offsets, flatten_ids, render_alphas, last_ids, T_final,
+                           v_render_colors, v_render_alphas, v_means2d, v_conics, v_colors, v_opacities, v_betas,
+                           nullptr, 0, workspace, workspace_bytes, stream);
+}
+
+int ubs_rasterize_to_pixels_bwd_ckpt(uint32_t C, uint32_t N, int64_t n_isects, uint32_t channels,
+                                     const float *means2d, const float *conics, const float *colors,
+                                     const float *opacities, const float *betas, const float *backgrounds,
+                                     const uint8_t *masks, uint32_t W, uint32_t H, uint32_t tile_size, uint32_t tw,
+                                     uint32_t th, const int32_t *tile_offsets, const int32_t *flatten_ids,
+                                     const float *render_alphas, const int32_t *last_ids, const float *T_final,
+                                     const float *T_ckpt, size_t ckpt_n, const float *v_render_colors,
+                                     const float *v_render_alphas, float *v_means2d, float *v_conics,
+                                     float *v_colors, float *v_opacities, float *v_betas, void *workspace,
+                                     size_t workspace_bytes, void *stream) {
+    UBS_REQUIRE(T_ckpt || n_isects == 0, "null checkpoint buffer");
+    return raster_bwd_impl(C, N, n_isects, channels, means2d, conics, colors, opacities, betas, backgrounds, masks, W,
+                           H, tile_size, tw, th, tile_offsets, flatten_ids, render_alphas, last_ids, T_final,
+                           v_render_colors, v_render_alphas, v_means2d, v_conics, v_colors, v_opacities, v_betas,
+                           T_ckpt, ckpt_n, workspace, workspace_bytes, stream);
 }
 
 }  // extern "C"

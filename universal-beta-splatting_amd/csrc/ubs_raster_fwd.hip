@@ -25,7 +25,7 @@ __global__ __launch_bounds__(TS *TS) void raster_fwd_kernel(
     const uint8_t *__restrict__ masks, uint32_t W, uint32_t H, uint32_t tw, uint32_t th,
     const int32_t *__restrict__ tile_offsets, const int32_t *__restrict__ flatten_ids,
     float *__restrict__ render_colors, float *__restrict__ render_alphas, int32_t *__restrict__ last_ids,
-    float *__restrict__ T_final, const float4 *__restrict__ rec) {
+    float *__restrict__ T_final, const float4 *__restrict__ rec, float *__restrict__ T_ckpt) {
     constexpr int BS = TS * TS;        // threads = batch size
     constexpr int SB = TS / 8;         // sub-blocks per tile side
     constexpr int NW = BS / 64;        // waves
@@ -168,6 +168,8 @@ __global__ __launch_bounds__(TS *TS) void raster_fwd_kernel(
             }
         }
         if (cur_off >= 0) cur = batch_start + (cur_off >> 4);
+        // T after this batch, for a backward that restarts each batch from it
+        if (T_ckpt && write_meta) T_ckpt[ckpt_slot(range_start, logical, b, BS) * BS + tid] = T;
     }
 
     if (inside) {
@@ -188,10 +190,10 @@ static int launch_fwd(uint32_t C, uint32_t N, int64_t n_isects, uint32_t channel
                       const float *means2d, const float *conics, const float *colors, const float *opac,
                       const float *betas, const float *bg, const uint8_t *masks, uint32_t W, uint32_t H,
                       uint32_t tw, uint32_t th, const int32_t *offs, const int32_t *flat, float *rc, float *ra,
-                      int32_t *last, float *Tf, const float4 *rec, hipStream_t st) {
+                      int32_t *last, float *Tf, const float4 *rec, float *ck, hipStream_t st) {
     raster_fwd_kernel<TS, CDIM><<<xcd_grid(C, tw, th), TS * TS, 0, st>>>(
         C, N, n_isects, channels, coff, means2d, conics, colors, opac, betas, bg, masks, W, H, tw, th, offs, flat,
-        rc, ra, last, Tf, rec);
+        rc, ra, last, Tf, rec, ck);
     UBS_LAUNCH_CHECK("raster_fwd_kernel");
     return UBS_OK;
 }
@@ -201,11 +203,11 @@ static int launch_fwd_chunk(int cd, uint32_t C, uint32_t N, int64_t n_isects, ui
                             const float *means2d, const float *conics, const float *colors, const float *opac,
                             const float *betas, const float *bg, const uint8_t *masks, uint32_t W, uint32_t H,
                             uint32_t tw, uint32_t th, const int32_t *offs, const int32_t *flat, float *rc,
-                            float *ra, int32_t *last, float *Tf, const float4 *rec, hipStream_t st) {
+                            float *ra, int32_t *last, float *Tf, const float4 *rec, float *ck, hipStream_t st) {
 #define UBS_FWD_CASE(K)                                                                                     \
     case K:                                                                                                 \
         return launch_fwd<TS, K>(C, N, n_isects, channels, coff, means2d, conics, colors, opac, betas, bg,  \
-                                 masks, W, H, tw, th, offs, flat, rc, ra, last, Tf, rec, st);
+                                 masks, W, H, tw, th, offs, flat, rc, ra, last, Tf, rec, ck, st);
     switch (cd) {
         UBS_FWD_CASE(1)
         UBS_FWD_CASE(2)
@@ -214,17 +216,17 @@ static int launch_fwd_chunk(int cd, uint32_t C, uint32_t N, int64_t n_isects, ui
     case 8:
         if constexpr (TS <= 16)
             return launch_fwd<TS, 8>(C, N, n_isects, channels, coff, means2d, conics, colors, opac, betas, bg,
-                                     masks, W, H, tw, th, offs, flat, rc, ra, last, Tf, rec, st);
+                                     masks, W, H, tw, th, offs, flat, rc, ra, last, Tf, rec, ck, st);
         break;
     case 16:
         if constexpr (TS <= 16)
             return launch_fwd<TS, 16>(C, N, n_isects, channels, coff, means2d, conics, colors, opac, betas, bg,
-                                      masks, W, H, tw, th, offs, flat, rc, ra, last, Tf, rec, st);
+                                      masks, W, H, tw, th, offs, flat, rc, ra, last, Tf, rec, ck, st);
         break;
     case 32:
         if constexpr (TS <= 16)
             return launch_fwd<TS, 32>(C, N, n_isects, channels, coff, means2d, conics, colors, opac, betas, bg,
-                                      masks, W, H, tw, th, offs, flat, rc, ra, last, Tf, rec, st);
+                                      masks, W, H, tw, th, offs, flat, rc, ra, last, Tf, rec, ck, st);
         break;
     default:
         break;
@@ -238,14 +240,13 @@ static int launch_fwd_chunk(int cd, uint32_t C, uint32_t N, int64_t n_isects, ui
 
 using namespace ubs;
 
-extern "C" int ubs_rasterize_to_pixels_fwd(uint32_t C, uint32_t N, int64_t n_isects, uint32_t channels,
-                                           const float *means2d, const float *conics, const float *colors,
-                                           const float *opacities, const float *betas, const float *backgrounds,
-                                           const uint8_t *masks, uint32_t W, uint32_t H, uint32_t tile_size,
-                                           uint32_t tw, uint32_t th, const int32_t *tile_offsets,
-                                           const int32_t *flatten_ids, float *render_colors, float *render_alphas,
-                                           int32_t *last_ids, float *T_final, void *workspace,
-                                           size_t workspace_bytes, void *stream) {
+static int raster_fwd_impl(uint32_t C, uint32_t N, int64_t n_isects, uint32_t channels, const float *means2d,
+                           const float *conics, const float *colors, const float *opacities, const float *betas,
+                           const float *backgrounds, const uint8_t *masks, uint32_t W, uint32_t H,
+                           uint32_t tile_size, uint32_t tw, uint32_t th, const int32_t *tile_offsets,
+                           const int32_t *flatten_ids, float *render_colors, float *render_alphas,
+                           int32_t *last_ids, float *T_final, float *T_ckpt, size_t ckpt_n, void *workspace,
+                           size_t workspace_bytes, void *stream) {
     hipStream_t st = as_stream(stream);
     UBS_REQUIRE(tile_size == 8 || tile_size == 16 || tile_size == 32, "tile_size must be 8, 16 or 32 (got %u)",
                 tile_size);
@@ -257,6 +258,13 @@ extern "C" int ubs_rasterize_to_pixels_fwd(uint32_t C, uint32_t N, int64_t n_ise
     UBS_REQUIRE(means2d && conics && colors && opacities && betas && tile_offsets && render_colors && render_alphas &&
                     last_ids && (flatten_ids || n_isects == 0),
                 "null pointer");
+    if (T_ckpt) {
+        const size_t need = ckpt_floats(C, n_isects, tw, th, tile_size);
+        if (ckpt_n < need) {
+            set_error("transmittance checkpoint buffer of %zu floats, %zu needed", ckpt_n, need);
+            return UBS_ERR_WORKSPACE;
+        }
+    }
     const float4 *rec = nullptr;
     if (workspace) {
         const size_t need = raster_workspace_bytes(C, N, 0);
@@ -279,19 +287,52 @@ extern "C" int ubs_rasterize_to_pixels_fwd(uint32_t C, uint32_t N, int64_t n_ise
         if (tile_size == 16)
             e = launch_fwd_chunk<16>(cd, C, N, n_isects, channels, coff, means2d, conics, colors, opacities, betas,
                                      backgrounds, masks, W, H, tw, th, tile_offsets, flatten_ids, render_colors,
-                                     render_alphas, last_ids, T_final, rec, st);
+                                     render_alphas, last_ids, T_final, rec, T_ckpt, st);
         else if (tile_size == 8)
             e = launch_fwd_chunk<8>(cd, C, N, n_isects, channels, coff, means2d, conics, colors, opacities, betas,
                                     backgrounds, masks, W, H, tw, th, tile_offsets, flatten_ids, render_colors,
-                                    render_alphas, last_ids, T_final, rec, st);
+                                    render_alphas, last_ids, T_final, rec, T_ckpt, st);
         else
             e = launch_fwd_chunk<32>(cd, C, N, n_isects, channels, coff, means2d, conics, colors, opacities, betas,
                                      backgrounds, masks, W, H, tw, th, tile_offsets, flatten_ids, render_colors,
-                                     render_alphas, last_ids, T_final, rec, st);
+                                     render_alphas, last_ids, T_final, rec, T_ckpt, st);
         if (e) return e;
         coff += (uint32_t)cd;
     }
     return UBS_OK;
+}
+
+extern "C" int ubs_rasterize_to_pixels_fwd(uint32_t C, uint32_t N, int64_t n_isects, uint32_t channels,
+                                           const float *means2d, const float *conics, const float *colors,
+                                           const float *opacities, const float *betas, const float *backgrounds,
+                                           const uint8_t *masks, uint32_t W, uint32_t H, uint32_t tile_size,
+                                           uint32_t tw, uint32_t th, const int32_t *tile_offsets,
+                                           const int32_t *flatten_ids, float *render_colors, float *render_alphas,
+                                           int32_t *last_ids, float *T_final, void *workspace,
+                                           size_t workspace_bytes, void *stream) {
+    return raster_fwd_impl(C, N, n_isects, channels, means2d, conics, colors, opacities, betas, backgrounds, masks, W,
+                           H, tile_size, tw, th, tile_offsets, flatten_ids, render_colors, render_alphas, last_ids,
+                           T_final, nullptr, 0, workspace, workspace_bytes, stream);
+}
+
+extern "C" int ubs_rasterize_to_pixels_fwd_ckpt(uint32_t C, uint32_t N, int64_t n_isects, uint32_t channels,
+                                                const float *means2d, const float *conics, const float *colors,
+                                                const float *opacities, const float *betas,
+                                                const float *backgrounds, const uint8_t *masks, uint32_t W,
+                                                uint32_t H, uint32_t tile_size, uint32_t tw, uint32_t th,
+                                                const int32_t *tile_offsets, const int32_t *flatten_ids,
+                                                float *render_colors, float *render_alphas, int32_t *last_ids,
+                                                float *T_final, float *T_ckpt, size_t ckpt_n, void *workspace,
+                                                size_t workspace_bytes, void *stream) {
+    UBS_REQUIRE(T_ckpt || n_isects == 0, "null checkpoint buffer");
+    return raster_fwd_impl(C, N, n_isects, channels, means2d, conics, colors, opacities, betas, backgrounds, masks, W,
+                           H, tile_size, tw, th, tile_offsets, flatten_ids, render_colors, render_alphas, last_ids,
+                           T_final, T_ckpt, ckpt_n, workspace, workspace_bytes, stream);
+}
+
+extern "C" size_t ubs_transmittance_checkpoint_floats(uint32_t C, int64_t n_isects, uint32_t tile_width,
+                                                      uint32_t tile_height, uint32_t tile_size) {
+    return ckpt_floats(C, n_isects < 0 ? 0 : n_isects, tile_width, tile_height, tile_size);
 }
 
 extern "C" size_t ubs_rasterize_workspace_bytes(uint32_t C, uint32_t N, int backward) {

@@ -410,6 +410,19 @@ def set_backward_fp64_transmittance(enable: bool) -> None:
     lib().ubs_set_backward_fp64_transmittance(1 if enable else 0)
 
 
+_t_checkpoint = False
+
+
+def set_backward_transmittance_checkpoint(enable: bool) -> None:
+    """Per-pixel transmittance in the blend backward: False (default)
+    recomputes it by division from the final T, as the reference does
+    (rasterize_to_pixels_bwd.cu:202-203); True stores it in the forward after
+    every batch of tile_size^2 entries (4 B per pixel per batch) and restarts
+    each batch of the backward from the stored value."""
+    global _t_checkpoint
+    _t_checkpoint = bool(enable)
+
+
 def set_cond_backward_exact(enable: bool) -> None:
     """Conditioning backward: False (default) reproduces the reference's
     gradient, whose Cholesky backward solves with L where L^T is needed
@@ -449,6 +462,21 @@ def _blend_workspace(C: int, N: int, backward: bool, device):
 
 def raster_fwd_raw(means2d, conics, colors, opacities, betas, backgrounds, masks, width, height, tile_size,
                    isect_offsets, flatten_ids):
+    """Blend forward: (render_colors, render_alphas, last_ids, T_final)."""
+    return _raster_fwd(means2d, conics, colors, opacities, betas, backgrounds, masks, width, height, tile_size,
+                       isect_offsets, flatten_ids, False)[:4]
+
+
+def raster_fwd_ckpt_raw(means2d, conics, colors, opacities, betas, backgrounds, masks, width, height, tile_size,
+                        isect_offsets, flatten_ids):
+    """Blend forward that also stores the transmittance checkpoints:
+    (render_colors, render_alphas, last_ids, T_final, T_ckpt)."""
+    return _raster_fwd(means2d, conics, colors, opacities, betas, backgrounds, masks, width, height, tile_size,
+                       isect_offsets, flatten_ids, True)
+
+
+def _raster_fwd(means2d, conics, colors, opacities, betas, backgrounds, masks, width, height, tile_size,
+                isect_offsets, flatten_ids, checkpoint: bool):
     require_cuda(means2d, conics, colors, opacities, betas, backgrounds, isect_offsets, flatten_ids)
     require_dtype(_F32, means2d, conics, colors, opacities, betas, backgrounds)
     require_dtype(_I32, isect_offsets, flatten_ids)
@@ -462,16 +490,24 @@ def raster_fwd_raw(means2d, conics, colors, opacities, betas, backgrounds, masks
     T_final = torch.empty((C, height, width), dtype=torch.float32, device=dev)
     mp = _mask_ptr(masks)
     ws, ws_bytes = _blend_workspace(C, N, False, dev)
-    _call("ubs_rasterize_to_pixels_fwd", C, N, flatten_ids.numel(), channels, ptr(means2d), ptr(conics),
+    if not checkpoint:
+        _call("ubs_rasterize_to_pixels_fwd", C, N, flatten_ids.numel(), channels, ptr(means2d), ptr(conics),
+              ptr(colors), ptr(opacities), ptr(betas), ptr(backgrounds), mp[0] if mp else None, width, height,
+              tile_size, tw, th, ptr(isect_offsets), ptr(flatten_ids), ptr(render_colors), ptr(render_alphas),
+              ptr(last_ids), ptr(T_final), ptr(ws), ws_bytes, stream_of(means2d))
+        return render_colors, render_alphas, last_ids, T_final, None
+    n_ck = int(lib().ubs_transmittance_checkpoint_floats(C, flatten_ids.numel(), tw, th, tile_size))
+    T_ckpt = torch.empty((n_ck,), dtype=torch.float32, device=dev)
+    _call("ubs_rasterize_to_pixels_fwd_ckpt", C, N, flatten_ids.numel(), channels, ptr(means2d), ptr(conics),
           ptr(colors), ptr(opacities), ptr(betas), ptr(backgrounds), mp[0] if mp else None, width, height,
           tile_size, tw, th, ptr(isect_offsets), ptr(flatten_ids), ptr(render_colors), ptr(render_alphas),
-          ptr(last_ids), ptr(T_final), ptr(ws), ws_bytes, stream_of(means2d))
-    return render_colors, render_alphas, last_ids, T_final
+          ptr(last_ids), ptr(T_final), ptr(T_ckpt), n_ck, ptr(ws), ws_bytes, stream_of(means2d))
+    return render_colors, render_alphas, last_ids, T_final, T_ckpt
 
 
 def raster_bwd_raw(means2d, conics, colors, opacities, betas, backgrounds, masks, width, height, tile_size,
                    isect_offsets, flatten_ids, render_alphas, last_ids, T_final, v_render_colors,
-                   v_render_alphas):
+                   v_render_alphas, T_ckpt=None):
     require_cuda(means2d, conics, colors, opacities, betas, backgrounds, isect_offsets, flatten_ids,
                  render_alphas, last_ids, T_final, v_render_colors, v_render_alphas)
     require_dtype(_F32, means2d, conics, colors, opacities, betas, backgrounds, render_alphas, T_final,
@@ -487,11 +523,21 @@ def raster_bwd_raw(means2d, conics, colors, opacities, betas, backgrounds, masks
     v_betas = torch.empty_like(betas)
     mp = _mask_ptr(masks)
     ws, ws_bytes = _blend_workspace(C, N, True, means2d.device)
-    _call("ubs_rasterize_to_pixels_bwd", C, N, flatten_ids.numel(), channels, ptr(means2d), ptr(conics),
-          ptr(colors), ptr(opacities), ptr(betas), ptr(backgrounds), mp[0] if mp else None, width, height,
-          tile_size, tw, th, ptr(isect_offsets), ptr(flatten_ids), ptr(render_alphas), ptr(last_ids),
-          ptr(T_final), ptr(v_render_colors), ptr(v_render_alphas), ptr(v_means2d), ptr(v_conics),
-          ptr(v_colors), ptr(v_opacities), ptr(v_betas), ptr(ws), ws_bytes, stream_of(means2d))
+    if T_ckpt is None:
+        _call("ubs_rasterize_to_pixels_bwd", C, N, flatten_ids.numel(), channels, ptr(means2d), ptr(conics),
+              ptr(colors), ptr(opacities), ptr(betas), ptr(backgrounds), mp[0] if mp else None, width, height,
+              tile_size, tw, th, ptr(isect_offsets), ptr(flatten_ids), ptr(render_alphas), ptr(last_ids),
+              ptr(T_final), ptr(v_render_colors), ptr(v_render_alphas), ptr(v_means2d), ptr(v_conics),
+              ptr(v_colors), ptr(v_opacities), ptr(v_betas), ptr(ws), ws_bytes, stream_of(means2d))
+    else:
+        require_cuda(T_ckpt)
+        require_dtype(_F32, T_ckpt)
+        _call("ubs_rasterize_to_pixels_bwd_ckpt", C, N, flatten_ids.numel(), channels, ptr(means2d), ptr(conics),
+              ptr(colors), ptr(opacities), ptr(betas), ptr(backgrounds), mp[0] if mp else None, width, height,
+              tile_size, tw, th, ptr(isect_offsets), ptr(flatten_ids), ptr(render_alphas), ptr(last_ids),
+              ptr(T_final), ptr(T_ckpt), T_ckpt.numel(), ptr(v_render_colors), ptr(v_render_alphas),
+              ptr(v_means2d), ptr(v_conics), ptr(v_colors), ptr(v_opacities), ptr(v_betas), ptr(ws), ws_bytes,
+              stream_of(means2d))
     return v_means2d, v_conics, v_colors, v_opacities, v_betas
 
 
@@ -861,22 +907,23 @@ class _RasterizeToPixels(torch.autograd.Function):
     @staticmethod
     def forward(ctx, means2d, conics, colors, opacities, betas, backgrounds, masks, width, height, tile_size,
                 isect_offsets, flatten_ids):
-        render_colors, render_alphas, last_ids, T_final = raster_fwd_raw(
+        ckpt = _t_checkpoint and any(ctx.needs_input_grad[:5])
+        render_colors, render_alphas, last_ids, T_final, T_ckpt = _raster_fwd(
             means2d, conics, colors, opacities, betas, backgrounds, masks, width, height, tile_size,
-            isect_offsets, flatten_ids)
+            isect_offsets, flatten_ids, ckpt)
         ctx.save_for_backward(means2d, conics, colors, opacities, betas, backgrounds, masks, isect_offsets,
-                              flatten_ids, render_alphas, last_ids, T_final)
+                              flatten_ids, render_alphas, last_ids, T_final, T_ckpt)
         ctx.width, ctx.height, ctx.tile_size = width, height, tile_size
         return render_colors, render_alphas
 
     @staticmethod
     def backward(ctx, v_render_colors, v_render_alphas):
         (means2d, conics, colors, opacities, betas, backgrounds, masks, isect_offsets, flatten_ids,
-         render_alphas, last_ids, T_final) = ctx.saved_tensors
+         render_alphas, last_ids, T_final, T_ckpt) = ctx.saved_tensors
         v_means2d, v_conics, v_colors, v_opacities, v_betas = raster_bwd_raw(
             means2d, conics, colors, opacities, betas, backgrounds, masks, ctx.width, ctx.height,
             ctx.tile_size, isect_offsets, flatten_ids, render_alphas, last_ids, T_final,
-            v_render_colors.contiguous(), v_render_alphas.contiguous())
+            v_render_colors.contiguous(), v_render_alphas.contiguous(), T_ckpt=T_ckpt)
         if ctx.needs_input_grad[5]:
             v_backgrounds = (v_render_colors * (1.0 - render_alphas).float()).sum(dim=(1, 2))
         else:
