@@ -16,6 +16,8 @@ roofline of the dominant kernel and a CPU baseline from the oracle.
 from __future__ import annotations
 
 import argparse
+import contextlib
+import gc
 import json
 import math
 import os
@@ -51,6 +53,19 @@ CONFIGS = {
                workload="4 resident scenes of 2M primitives (two 6-D static, two 7-D dynamic), views at "
                         "640x480/1280x720/1920x1080 on a radius U[3,5] sphere, t U[0,1], FP32 forward"),
 }
+
+
+@contextlib.contextmanager
+def no_gc():
+    """Python's cyclic garbage collector off inside a timed region: a full
+    collection over torch's objects pauses the host for ~40 ms at random
+    steps, which is not part of the workload."""
+    gc.collect()
+    gc.disable()
+    try:
+        yield
+    finally:
+        gc.enable()
 
 
 def algorithmic_bytes(name: str, c: dict, n_isect: int, C: int = 1) -> float:
@@ -186,6 +201,9 @@ def main():
     ap.add_argument("--path", default="fused", choices=("fused", "chain"),
                     help="fused: one-launch Beta projection (render_views); chain: the reference's op chain "
                          "(render: activations, rotation, covariance, conditioning, projection as separate ops)")
+    ap.add_argument("--sort", default="segmented", choices=("segmented", "onesweep"),
+                    help="isect-id sort: onesweep passes over the camera/tile bits then a per-tile depth sort "
+                         "in LDS (default), or onesweep passes over every byte (same order)")
     ap.add_argument("--transmittance", default="recompute", choices=("recompute", "checkpoint"),
                     help="per-pixel transmittance in the blend backward: recomputed by division from the final T "
                          "(default, as the reference) or stored by the forward after every batch of 256 entries")
@@ -228,6 +246,7 @@ def main():
     leaves = list(params.leaves().values())
     W.set_backward_fp64_transmittance(args.fp64_transmittance)
     W.set_backward_transmittance_checkpoint(args.transmittance == "checkpoint")
+    W.set_segmented_depth_sort(args.sort == "segmented")
 
     vms_t, Ks_t = vm[None].contiguous(), K[None].contiguous()
     ts_t = torch.tensor([t_view], dtype=torch.float32, device=dev)
@@ -275,14 +294,15 @@ def main():
     W.set_launch_timer(timer)
     step_ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
                for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        step_ev[k][0].record()
-        step()
-        step_ev[k][1].record()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
+    with no_gc():
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            step_ev[k][0].record()
+            step()
+            step_ev[k][1].record()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
     W.set_launch_timer(None)
     launches = timer.durations_ms()
     step_ms = sorted(s.elapsed_time(e) for s, e in step_ev)
@@ -291,12 +311,13 @@ def main():
     # ---- forward-only rate (not the headline value)
     for _ in range(3):
         fwd_only()
-    torch.cuda.synchronize()
-    f0 = time.perf_counter()
-    for _ in range(args.steps):
-        fwd_only()
-    torch.cuda.synchronize()
-    fwd_ms = (time.perf_counter() - f0) * 1e3 / args.steps
+    with no_gc():
+        torch.cuda.synchronize()
+        f0 = time.perf_counter()
+        for _ in range(args.steps):
+            fwd_only()
+        torch.cuda.synchronize()
+        fwd_ms = (time.perf_counter() - f0) * 1e3 / args.steps
 
     # ---- batched serving of the same scene: 8 views at the same size in one
     # pass (fused projection, one isect/sort/blend over all views) against
@@ -345,7 +366,7 @@ def main():
                    "n_isects": n_isect, "visible_primitives": n_vis, "projection_path": args.path,
                    "loss": "L1" if args.lambda_dssim == 0 else f"{1 - args.lambda_dssim:g} L1 + {args.lambda_dssim:g} (1 - SSIM)",
                    "sort_passes_skipped": cfg["sort_skipped"], "optimizer_step": "adam" if args.adam else None,
-                   "transmittance": args.transmittance},
+                   "transmittance": args.transmittance, "sort": args.sort},
         "fwd_frames_per_s": round(1e3 / fwd_ms, 3),
         "fwd_bwd_mpix_per_s": round(cfg["W"] * cfg["H"] * 1e3 / ms_per_step / 1e6, 2),
         "latency_ms": {"p50": round(step_ms[len(step_ms) // 2], 3),
@@ -386,14 +407,15 @@ def serve_views(params, cfg, steps: int, views: int = 8) -> dict:
         for _ in range(2):
             fn()
         ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for k in range(steps):
-            ev[k][0].record()
-            fn()
-            ev[k][1].record()
-        torch.cuda.synchronize()
-        ms = (time.perf_counter() - t0) * 1e3 / steps
+        with no_gc():
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for k in range(steps):
+                ev[k][0].record()
+                fn()
+                ev[k][1].record()
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t0) * 1e3 / steps
         lat = sorted(a.elapsed_time(b) for a, b in ev)
         out[name] = {"ms_per_batch": round(ms, 3), "mpix_per_s": round(views * W * H * 1e3 / ms / 1e6, 1),
                      "frames_per_s": round(views * 1e3 / ms, 1), "p50_ms": round(lat[len(lat) // 2], 3),
@@ -437,6 +459,10 @@ def serve_bench(args, cfg):
     modes = {"batched": dict(max_views=16, streams=2), "per_view": dict(max_views=1, streams=1)}
 
     def run(reqs, arrivals, mode):
+        with no_gc():
+            return _run(reqs, arrivals, mode)
+
+    def _run(reqs, arrivals, mode):
         srv = RenderServer(store, keep_images=False, **modes[mode])
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -532,12 +558,13 @@ def batch_bench(args, cfg):
             step()
         timer = W.LaunchTimer()
         W.set_launch_timer(timer)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            step()
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
+        with no_gc():
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                step()
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
         W.set_launch_timer(None)
         launches = timer.durations_ms()
         modes[mode] = {"ms_per_step": round((t1 - t0) * 1e3 / args.steps, 3),
@@ -559,12 +586,13 @@ def batch_bench(args, cfg):
 
     for _ in range(max(1, args.warmup // 2)):
         step_per_view()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step_per_view()
-    torch.cuda.synchronize()
-    ms_per_view_loop = (time.perf_counter() - t0) * 1e3 / args.steps
+    with no_gc():
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step_per_view()
+        torch.cuda.synchronize()
+        ms_per_view_loop = (time.perf_counter() - t0) * 1e3 / args.steps
     line = {
         "metric": "views/s fwd+bwd, batched multi-view step (3M prims, 8 views at 540p/720p/1080p)",
         "value": round(C * 1e3 / ms, 3), "unit": "views/s", "n_gpus": 1, "steps": args.steps,

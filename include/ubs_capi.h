@@ -272,6 +272,21 @@ int ubs_radix_sort_pairs(int64_t n, int64_t *keys, int64_t *keys_alt, int32_t *v
 int ubs_radix_sort_pairs_skip(int64_t n, int64_t *keys, int64_t *keys_alt, int32_t *vals,
                               int32_t *vals_alt, int begin_bit, int end_bit, uint32_t skip_passes,
                               void *workspace, size_t workspace_bytes, int *result_in_alt, void *stream);
+
+/* Depth sort inside each (camera, tile) segment of isect ids that are
+ * already sorted stably on their camera/tile bits (ubs_radix_sort_pairs_skip
+ * with begin_bit 32): keys_in/vals_in [n] -> keys_out/vals_out [n], each
+ * segment [tile_offsets[L], tile_offsets[L+1]) (the last to n) sorted stably
+ * on the 32 depth bits.  The two steps together give the order of the full
+ * stable sort on bits [0, 32 + tile_bits + cam_bits), bit for bit.
+ * tile_offsets: ubs_isect_offset_encode of keys_in (the same for keys_out).
+ * skip_passes: bit p leaves out the pass over depth byte p (every key shares
+ * it).  keys_in/vals_in are scratch for segments longer than 8192.
+ * Replaces the depth passes of the SortPairs call of isect_tiles.cu:224-278. */
+int ubs_sort_tile_segments(int64_t n, int64_t *keys_in, int32_t *vals_in, int64_t *keys_out, int32_t *vals_out,
+                           const int32_t *tile_offsets, uint32_t C, uint32_t tile_width, uint32_t tile_height,
+                           uint32_t skip_passes, void *stream);
+
 /* Reads the sort's device-side status word (0 = fine, nonzero = a look-back
  * wait hit its spin limit and the result is invalid).  Synchronises stream. */
 int ubs_radix_sort_status(const void *workspace, int *status, void *stream);

@@ -474,3 +474,48 @@ def test_blend_packed_records_match_unpacked(device, C, channels, ts):
         assert torch.equal(a, b)
     for a, b, n in zip(res[False][1], res[True][1], ("v_means2d", "v_conics", "v_colors", "v_opac", "v_betas")):
         assert rel_l2(host(b), host(a).astype(np.float64)) < 1e-5, n
+
+
+@pytest.mark.parametrize("case", ["random", "dense_ties", "dense_4_bytes", "two_cams"])
+def test_segmented_depth_sort_matches_full_sort(device, case):
+    """The two-step sort (onesweep on the camera/tile bits, then the per-tile
+    depth sort) gives the full stable sort's ids and values bit for bit,
+    against the oracle and against the all-bytes onesweep: segments that fit
+    LDS and segments longer than 8192 (HBM chunks, odd and even pass counts),
+    equal depths (stability), skipped constant depth bytes."""
+    g = np.random.Generator(np.random.PCG64({"random": 1, "dense_ties": 2, "dense_4_bytes": 3, "two_cams": 4}[case]))
+    C = 2 if case == "two_cams" else 1
+    W_, H_, ts = 160, 128, 16
+    tw, th = W_ // ts, H_ // ts
+    N = 30_000
+    if case.startswith("dense"):  # most primitives cover the centre tiles: segments of ~10k entries
+        m2 = g.normal((80.0, 64.0), 6.0, (C, N, 2))
+        radii = g.integers(8, 24, (C, N))
+    else:
+        m2 = g.uniform(0, (W_, H_), (C, N, 2))
+        radii = g.integers(0, 20, (C, N))
+    if case == "dense_ties":  # 40 distinct depths in [2, 4): long runs of equal keys, 3 depth passes
+        depths = g.choice(np.linspace(2.0, 3.9, 40), (C, N))
+    elif case == "dense_4_bytes":  # depths spanning exponents: no constant depth byte
+        depths = np.exp(g.uniform(-8, 8, (C, N)))
+    else:
+        depths = g.uniform(2.0, 6.0, (C, N))
+    m2, radii, depths = m2.astype(np.float32), radii.astype(np.int32), depths.astype(np.float32)
+    _, want_ids, want_flat = O.isect_tiles(m2, radii, depths, ts, tw, th)
+    if case.startswith("dense"):
+        off = O.isect_offset_encode(want_ids, C, tw, th).reshape(-1)
+        assert np.diff(np.append(off, want_ids.size)).max() > 8192  # the chunked HBM path
+    got = {}
+    for seg in (True, False):
+        W.set_segmented_depth_sort(seg)
+        try:
+            _, ids, flat = W.isect_tiles(cu(m2), cu(radii), cu(depths), ts, tw, th)
+        finally:
+            W.set_segmented_depth_sort(True)
+        got[seg] = (host(ids), host(flat))
+    for seg in (True, False):
+        assert_same(got[seg][0], want_ids, f"ids seg={seg}")
+        assert_same(got[seg][1], want_flat, f"flat seg={seg}")
+    # the offsets handed out with the segmented sort are those of the final ids
+    _, ids, flat, offs = W.isect_tiles_offsets(cu(m2), cu(radii), cu(depths), ts, tw, th)
+    assert_same(host(offs), O.isect_offset_encode(want_ids, C, tw, th), "offsets")

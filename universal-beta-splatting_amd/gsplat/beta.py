@@ -26,7 +26,7 @@ from .cuda._wrapper import (
     camera_centers,
     cond_mean_convariance_opacity,
     isect_offset_encode,
-    isect_tiles,
+    isect_tiles_offsets,
     l_triangle_to_rotmat,
     rasterize_to_pixels,
     rot_scale_l_triangle_to_covar,
@@ -181,8 +181,10 @@ def render_views(params: BetaParams, viewmats: Tensor, Ks: Tensor, width: int, h
         near_plane=near_plane, far_plane=far_plane, radius_clip=radius_clip)
     tw = math.ceil(width / float(tile_size))
     th = math.ceil(height / float(tile_size))
-    tiles_per_gauss, isect_ids, flatten_ids = isect_tiles(means2d, radii, depths, tile_size, tw, th, n_cameras=C)
-    isect_offsets = isect_offset_encode(isect_ids, C, tw, th)
+    tiles_per_gauss, isect_ids, flatten_ids, isect_offsets = isect_tiles_offsets(means2d, radii, depths, tile_size,
+                                                                                tw, th)
+    if isect_offsets is None:
+        isect_offsets = isect_offset_encode(isect_ids, C, tw, th)
     colors = params.rgb.expand(C, -1, -1)
     render_colors, render_alphas = rasterize_to_pixels(
         means2d, conics, colors, opacities, betas, width, height, tile_size, isect_offsets, flatten_ids,
@@ -242,8 +244,9 @@ def render_batch(params: BetaParams, viewmats: Tensor, Ks: Tensor, sizes, timest
         Cg = c1 - c0
         tw, th = math.ceil(W / float(tile_size)), math.ceil(H / float(tile_size))
         m2, r, d = means2d[c0:c1], radii[c0:c1], depths[c0:c1]
-        _, isect_ids, flatten_ids = isect_tiles(m2, r, d, tile_size, tw, th, n_cameras=Cg)
-        offsets = isect_offset_encode(isect_ids, Cg, tw, th)
+        _, isect_ids, flatten_ids, offsets = isect_tiles_offsets(m2, r, d, tile_size, tw, th)
+        if offsets is None:
+            offsets = isect_offset_encode(isect_ids, Cg, tw, th)
         bg = backgrounds[c0:c1] if backgrounds is not None else None
         rc, ra = rasterize_to_pixels(m2, conics[c0:c1], params.rgb.expand(Cg, -1, -1), opacities[c0:c1],
                                      betas[c0:c1], W, H, tile_size, offsets, flatten_ids, backgrounds=bg)

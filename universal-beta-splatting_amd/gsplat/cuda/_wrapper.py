@@ -260,6 +260,25 @@ def isect_tiles(
     """Tiles per primitive [C,N] int32, isect ids int64 [n] (cam | tile |
     depth bits) sorted stably on their low 32 + tile_bits + cam_bits bits, and
     flatten ids int32 [n].  reference: _wrapper.py:285-343, isect_tiles.cu:99-285"""
+    return isect_tiles_offsets(means2d, radii, depths, tile_size, tile_width, tile_height, sort)[:3]
+
+
+_segmented_sort = True
+
+
+def set_segmented_depth_sort(enable: bool) -> None:
+    """Sort isect ids as two onesweep passes over the camera/tile bits plus a
+    per-tile depth sort in LDS (default), or as onesweep passes over every
+    byte.  Both give the same order, bit for bit."""
+    global _segmented_sort
+    _segmented_sort = bool(enable)
+
+
+def isect_tiles_offsets(means2d: Tensor, radii: Tensor, depths: Tensor, tile_size: int, tile_width: int,
+                        tile_height: int, sort: bool = True):
+    """isect_tiles, plus the tile offsets of the sorted ids when the sort
+    computed them on the way (else None): (tiles_per_gauss, isect_ids,
+    flatten_ids, isect_offsets or None)."""
     C, N, _ = means2d.shape
     assert means2d.shape == (C, N, 2), means2d.size()
     assert radii.shape == (C, N), radii.size()
@@ -285,11 +304,20 @@ def isect_tiles(
     if n_isects:
         _call("ubs_isect_tiles_emit", C, N, ptr(means2d), ptr(radii), ptr(depths), ptr(cum), tile_size,
               tile_width, tile_height, ptr(isect_ids), ptr(flatten_ids), st)
+    offsets = None
     if n_isects and sort:
         end_bit = 32 + int(L.ubs_isect_prefix_bits(C, tile_width, tile_height))
-        isect_ids, flatten_ids = radix_sort_pairs(isect_ids, flatten_ids, 0, end_bit,
-                                                  skip_passes=constant_depth_passes(packed))
-    return tpg, isect_ids, flatten_ids
+        if _segmented_sort:
+            # stable on the camera/tile bits, then per tile on the depth bits
+            ids1, flat1 = radix_sort_pairs(isect_ids, flatten_ids, 32, end_bit)
+            offsets = isect_offset_encode(ids1, C, tile_width, tile_height)
+            isect_ids, flatten_ids = torch.empty_like(ids1), torch.empty_like(flat1)
+            _call("ubs_sort_tile_segments", n_isects, ptr(ids1), ptr(flat1), ptr(isect_ids), ptr(flatten_ids),
+                  ptr(offsets), C, tile_width, tile_height, constant_depth_passes(packed), st)
+        else:
+            isect_ids, flatten_ids = radix_sort_pairs(isect_ids, flatten_ids, 0, end_bit,
+                                                      skip_passes=constant_depth_passes(packed))
+    return tpg, isect_ids, flatten_ids, offsets
 
 
 def constant_depth_passes(packed_range: int) -> int:
