@@ -166,28 +166,67 @@ __global__ void isect_cum_kernel(uint64_t total, const int32_t *__restrict__ tpg
 }
 
 // Pass 2: keys cam | tile | depth bits.  reference: isect_tiles.cu:76-96
-__global__ void isect_emit_kernel(uint32_t C, uint32_t N, const float *__restrict__ means2d,
-                                  const int32_t *__restrict__ radii, const float *__restrict__ depths,
-                                  const int64_t *__restrict__ cum, uint32_t ts, uint32_t tw, uint32_t th,
-                                  uint32_t tile_bits, int64_t *__restrict__ isect_ids,
-                                  int32_t *__restrict__ flatten_ids) {
+// The 64 primitives of a wave own one contiguous run of the output (their
+// counts' prefix from pass 1); the wave writes that run with consecutive
+// lanes on consecutive entries: entry k of the run belongs to the last lane
+// whose exclusive count prefix is <= k (a 6-step search over the lanes'
+// prefixes by lane shuffles) and is that primitive's tile number
+// k - prefix in row-major order of its box.  Same entries, same order as
+// one thread per primitive writing its own box, with coalesced stores.
+__global__ __launch_bounds__(256) void isect_emit_kernel(uint32_t C, uint32_t N, const float *__restrict__ means2d,
+                                                         const int32_t *__restrict__ radii,
+                                                         const float *__restrict__ depths,
+                                                         const int64_t *__restrict__ cum, uint32_t ts, uint32_t tw,
+                                                         uint32_t th, uint32_t tile_bits,
+                                                         int64_t *__restrict__ isect_ids,
+                                                         int32_t *__restrict__ flatten_ids) {
+    const uint64_t CN = (uint64_t)C * N;
     const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (uint64_t)C * N) return;
-    const int32_t r = radii[idx];
-    if (r <= 0) return;
-    uint32_t x0, y0, x1, y1;
-    tile_box(means2d[idx * 2], means2d[idx * 2 + 1], r, ts, tw, th, x0, y0, x1, y1);
-    const int64_t cid = (int64_t)(idx / N);
-    const int64_t cid_enc = cid << (32 + tile_bits);
-    const int64_t depth_enc = (int64_t)__float_as_int(depths[idx]);  // sign-extended, as the reference
-    int64_t cur = idx == 0 ? 0 : cum[idx - 1];
-    for (uint32_t i = y0; i < y1; ++i)
-        for (uint32_t j = x0; j < x1; ++j) {
-            const int64_t tile = (int64_t)i * tw + j;
-            isect_ids[cur] = cid_enc | (tile << 32) | depth_enc;
-            flatten_ids[cur] = (int32_t)idx;
-            ++cur;
+    const int lane = (int)(threadIdx.x & 63);
+    const uint64_t idx0 = idx - (uint64_t)lane;
+    if (idx0 >= CN) return;  // wave-uniform
+    const bool valid = idx < CN;
+    const int32_t r = valid ? radii[idx] : 0;
+    uint32_t x0 = 0, y0 = 0, x1 = 0, y1 = 0;
+    if (r > 0) tile_box(means2d[idx * 2], means2d[idx * 2 + 1], r, ts, tw, th, x0, y0, x1, y1);
+    const uint32_t bw = x1 - x0;
+    const uint32_t cnt = r > 0 ? (y1 - y0) * bw : 0u;
+    // exclusive prefix of the counts over the wave
+    uint32_t inc = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t u = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += u;
+    }
+    const uint32_t excl = inc - cnt;
+    const uint32_t total = __shfl(inc, 63, 64);
+    const int64_t wave_start = idx0 == 0 ? 0 : cum[idx0 - 1];
+    const int64_t cid = valid ? (int64_t)(idx / N) : 0;
+    const int64_t key_hi = (cid << (32 + tile_bits)) | (int64_t)__float_as_int(valid ? depths[idx] : 0.f);
+    const int32_t fid = (int32_t)idx;
+    const uint32_t rounds = (total + 63u) / 64u;
+    for (uint32_t it = 0; it < rounds; ++it) {  // every lane runs every round: the shuffles see all lanes
+        const uint32_t k = it * 64u + (uint32_t)lane;
+        int p = 0;
+#pragma unroll
+        for (int step = 32; step >= 1; step >>= 1) {
+            const uint32_t v = __shfl(excl, p + step, 64);
+            p = v <= k ? p + step : p;
         }
+        const uint32_t pe = __shfl(excl, p, 64);
+        const uint32_t pbw = __shfl(bw, p, 64);
+        const uint32_t px0 = __shfl(x0, p, 64);
+        const uint32_t py0 = __shfl(y0, p, 64);
+        const int32_t pfid = __shfl(fid, p, 64);
+        const int64_t pkey = __shfl(key_hi, p, 64);
+        if (k < total) {
+            const uint32_t local = k - pe;
+            const uint32_t q = local / pbw;
+            const int64_t tile = (int64_t)(py0 + q) * tw + (px0 + (local - q * pbw));
+            isect_ids[wave_start + k] = pkey | (tile << 32);
+            flatten_ids[wave_start + k] = pfid;
+        }
+    }
 }
 
 // offsets[k] = number of sorted isects whose (cam, tile) index is < k.
