@@ -85,6 +85,13 @@ def algorithmic_bytes(name: str, c: dict, n_isect: int, C: int = 1) -> float:
         return 44.0 * n_isect + 24.0 * pix + 40.0 * n_isect
     if name in ("ubs_radix_sort_pairs", "ubs_radix_sort_pairs_skip"):
         end_bit = 32 + int(tiles).bit_length() + int(C).bit_length()
+        tile_passes = math.ceil((end_bit - 32) / 8)
+        mode = c.get("sort_mode", "onesweep")
+        if name == "ubs_radix_sort_pairs" and mode == "depth_first":
+            # two calls per step: the C*N depth keys (4 passes), the entries on the camera/tile bits
+            return (C * N * (8.0 + 24.0 * 4) + n_isect * (8.0 + 24.0 * tile_passes)) / 2
+        if name == "ubs_radix_sort_pairs" and mode == "segmented":
+            return n_isect * (8.0 + 24.0 * tile_passes)
         passes = math.ceil(end_bit / 8)
         if name.endswith("_skip"):  # passes over depth bytes every key shares are not run
             passes -= c.get("sort_skipped", 0)
@@ -95,6 +102,14 @@ def algorithmic_bytes(name: str, c: dict, n_isect: int, C: int = 1) -> float:
         return 28.0 * C * N
     if name == "ubs_isect_tiles_emit":
         return 24.0 * C * N + 12.0 * n_isect
+    if name == "ubs_isect_tiles_emit_ordered":  # + the order and prefix reads
+        return 36.0 * C * N + 12.0 * n_isect
+    if name == "ubs_isect_depth_keys":
+        return 20.0 * C * N
+    if name == "ubs_isect_scan_ordered":
+        return 24.0 * C * N
+    if name == "ubs_sort_tile_segments":
+        return 24.0 * n_isect
     if name == "ubs_fully_fused_projection_fwd":
         return 36.0 * N + 28.0 * C * N
     if name == "ubs_fully_fused_projection_bwd":
@@ -201,9 +216,10 @@ def main():
     ap.add_argument("--path", default="fused", choices=("fused", "chain"),
                     help="fused: one-launch Beta projection (render_views); chain: the reference's op chain "
                          "(render: activations, rotation, covariance, conditioning, projection as separate ops)")
-    ap.add_argument("--sort", default="segmented", choices=("segmented", "onesweep"),
-                    help="isect-id sort: onesweep passes over the camera/tile bits then a per-tile depth sort "
-                         "in LDS (default), or onesweep passes over every byte (same order)")
+    ap.add_argument("--sort", default="depth_first", choices=("depth_first", "segmented", "onesweep"),
+                    help="isect-id sort (same order every way): depth_first (default) sorts the primitives by depth "
+                         "once and emits in that order, then onesweep on the camera/tile bits; segmented: onesweep "
+                         "on the camera/tile bits then a per-tile depth sort in LDS; onesweep: every byte")
     ap.add_argument("--transmittance", default="recompute", choices=("recompute", "checkpoint"),
                     help="per-pixel transmittance in the blend backward: recomputed by division from the final T "
                          "(default, as the reference) or stored by the forward after every batch of 256 entries")
@@ -227,6 +243,8 @@ def main():
     from gsplat import beta, scenes
     from gsplat.cuda import _wrapper as W
 
+    W.set_backward_fp64_transmittance(args.fp64_transmittance)
+    W.set_isect_sort_mode(args.sort)
     cfg = CONFIGS[args.config]
     if "views" in cfg:
         return batch_bench(args, cfg)
@@ -244,9 +262,7 @@ def main():
                        dtype=torch.float32, device=dev)
     rest = beta.rest_indices(cfg["D"], dev)
     leaves = list(params.leaves().values())
-    W.set_backward_fp64_transmittance(args.fp64_transmittance)
     W.set_backward_transmittance_checkpoint(args.transmittance == "checkpoint")
-    W.set_segmented_depth_sort(args.sort == "segmented")
 
     vms_t, Ks_t = vm[None].contiguous(), K[None].contiguous()
     ts_t = torch.tensor([t_view], dtype=torch.float32, device=dev)
@@ -287,7 +303,7 @@ def main():
     n_vis = int((meta["radii"] > 0).sum().item())
     dbits = meta["depths"][meta["radii"] > 0].contiguous().view(torch.int32).to(torch.int64) & 0xFFFFFFFF
     lo, hi = int(dbits.min()), int(dbits.max())
-    cfg = dict(cfg, lambda_dssim=args.lambda_dssim, sort_skipped=bin(W.constant_depth_passes((hi << 32) | (~lo & 0xFFFFFFFF))).count("1"))
+    cfg = dict(cfg, lambda_dssim=args.lambda_dssim, sort_mode=args.sort, sort_skipped=bin(W.constant_depth_passes((hi << 32) | (~lo & 0xFFFFFFFF))).count("1"))
 
     # ---- timed region: K steps; per-launch HIP events on the launch stream
     timer = W.LaunchTimer()
@@ -509,7 +525,7 @@ def serve_bench(args, cfg):
         "vs_baseline": None, "dtype": "fp32",
         "data": "synthetic (seeded scenes, cameras and arrivals, gsplat/scenes.py)",
         "config": {"workload": cfg["workload"], "primitives_per_scene": cfg["N"], "scenes": len(names),
-                   "resident_gb": round(store.resident_bytes() / 1e9, 3), "max_views_per_batch": 16,
+                   "resident_gb": round(store.resident_bytes() / 1e9, 3), "max_views_per_batch": 16, "sort": args.sort,
                    "streams": 2, "peak_hbm_gb": round(torch.cuda.max_memory_allocated() / 1e9, 2)},
         "saturation": saturation,
         "speedup_vs_per_view": round(b["mpix_per_s"] / pv["mpix_per_s"], 3),
@@ -599,7 +615,7 @@ def batch_bench(args, cfg):
         "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "fp32", "data": "synthetic (seeded scene and camera generator, gsplat/scenes.py)",
         "config": {"workload": cfg["workload"], "primitives": cfg["N"], "dims": cfg["D"], "views": C,
-                   "sizes": sizes, "transmittance": "recompute", "peak_hbm_gb": modes["recompute"]["peak_hbm_gb"]},
+                   "sizes": sizes, "transmittance": "recompute", "sort": args.sort, "peak_hbm_gb": modes["recompute"]["peak_hbm_gb"]},
         "roofline": None, "cpu_baseline": None, "stages": stages, "transmittance_modes": modes,
         "per_view_loop_ms_per_step": round(ms_per_view_loop, 3),
         "speedup_vs_per_view_loop": round(ms_per_view_loop / ms, 3),

@@ -273,6 +273,30 @@ int ubs_radix_sort_pairs_skip(int64_t n, int64_t *keys, int64_t *keys_alt, int32
                               int32_t *vals_alt, int begin_bit, int end_bit, uint32_t skip_passes,
                               void *workspace, size_t workspace_bytes, int *result_in_alt, void *stream);
 
+/* Depth-first emission: the same intersection entries in an order that a
+ * stable sort on the camera/tile bits alone ([32, 32 + tile_bits +
+ * cam_bits)) turns into the full stable sort's order, bit for bit.  The
+ * (camera, primitive) pairs are sorted stably by depth once (C*N pairs
+ * instead of n_isects entries), then each emits its tiles in that order.
+ *   ubs_isect_depth_keys: keys int64 [C*N] = depth bit pattern of each
+ *     visible pair (0xFFFFFFFF when culled), vals int32 [C*N] = flat id;
+ *     sort them on bits [0, 32) -> order (the sorted vals).
+ *   ubs_isect_scan_ordered: cum int64 [C*N] = inclusive prefix of
+ *     tiles_per_gauss[order[k]]; workspace ubs_isect_ordered_workspace_bytes.
+ *   ubs_isect_tiles_emit_ordered: the entries of pair order[k] at
+ *     [cum[k-1], cum[k]), its tiles in row-major order.
+ * Replaces the emission and the depth passes of the sort of
+ * isect_tiles.cu:76-96 and :224-278. */
+int ubs_isect_depth_keys(uint32_t C, uint32_t N, const int32_t *radii, const float *depths, int64_t *keys,
+                         int32_t *vals, void *stream);
+size_t ubs_isect_ordered_workspace_bytes(uint32_t C, uint32_t N);
+int ubs_isect_scan_ordered(uint32_t C, uint32_t N, const int32_t *tiles_per_gauss, const int32_t *order,
+                           int64_t *cum, void *workspace, size_t workspace_bytes, void *stream);
+int ubs_isect_tiles_emit_ordered(uint32_t C, uint32_t N, const float *means2d, const int32_t *radii,
+                                 const float *depths, const int32_t *order, const int64_t *cum, uint32_t tile_size,
+                                 uint32_t tile_width, uint32_t tile_height, int64_t *isect_ids, int32_t *flatten_ids,
+                                 void *stream);
+
 /* Depth sort inside each (camera, tile) segment of isect ids that are
  * already sorted stably on their camera/tile bits (ubs_radix_sort_pairs_skip
  * with begin_bit 32): keys_in/vals_in [n] -> keys_out/vals_out [n], each

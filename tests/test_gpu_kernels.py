@@ -478,11 +478,12 @@ def test_blend_packed_records_match_unpacked(device, C, channels, ts):
 
 @pytest.mark.parametrize("case", ["random", "dense_ties", "dense_4_bytes", "two_cams"])
 def test_segmented_depth_sort_matches_full_sort(device, case):
-    """The two-step sort (onesweep on the camera/tile bits, then the per-tile
-    depth sort) gives the full stable sort's ids and values bit for bit,
-    against the oracle and against the all-bytes onesweep: segments that fit
-    LDS and segments longer than 8192 (HBM chunks, odd and even pass counts),
-    equal depths (stability), skipped constant depth bytes."""
+    """Every sort mode -- depth-first emission + onesweep on the camera/tile
+    bits, onesweep on the camera/tile bits + per-tile depth sort, onesweep on
+    every byte -- gives the oracle's ids and values bit for bit: segments
+    that fit LDS and segments longer than 8192 (HBM chunks, odd and even
+    pass counts), equal depths (stability), skipped constant depth bytes,
+    culled primitives, two cameras."""
     g = np.random.Generator(np.random.PCG64({"random": 1, "dense_ties": 2, "dense_4_bytes": 3, "two_cams": 4}[case]))
     C = 2 if case == "two_cams" else 1
     W_, H_, ts = 160, 128, 16
@@ -506,16 +507,15 @@ def test_segmented_depth_sort_matches_full_sort(device, case):
         off = O.isect_offset_encode(want_ids, C, tw, th).reshape(-1)
         assert np.diff(np.append(off, want_ids.size)).max() > 8192  # the chunked HBM path
     got = {}
-    for seg in (True, False):
-        W.set_segmented_depth_sort(seg)
+    before = W.isect_sort_mode()
+    for mode in ("depth_first", "segmented", "onesweep"):
+        W.set_isect_sort_mode(mode)
         try:
             _, ids, flat = W.isect_tiles(cu(m2), cu(radii), cu(depths), ts, tw, th)
+            _, _, _, offs = W.isect_tiles_offsets(cu(m2), cu(radii), cu(depths), ts, tw, th)
         finally:
-            W.set_segmented_depth_sort(True)
-        got[seg] = (host(ids), host(flat))
-    for seg in (True, False):
-        assert_same(got[seg][0], want_ids, f"ids seg={seg}")
-        assert_same(got[seg][1], want_flat, f"flat seg={seg}")
-    # the offsets handed out with the segmented sort are those of the final ids
-    _, ids, flat, offs = W.isect_tiles_offsets(cu(m2), cu(radii), cu(depths), ts, tw, th)
-    assert_same(host(offs), O.isect_offset_encode(want_ids, C, tw, th), "offsets")
+            W.set_isect_sort_mode(before)
+        assert_same(host(ids), want_ids, f"ids {mode}")
+        assert_same(host(flat), want_flat, f"flat {mode}")
+        if offs is not None:  # offsets handed out on the way are those of the final ids
+            assert_same(host(offs), O.isect_offset_encode(want_ids, C, tw, th), f"offsets {mode}")

@@ -146,14 +146,15 @@ __global__ void isect_scan_blocks_kernel(int64_t nb, int64_t *__restrict__ block
 // Pass 1c: int64 inclusive cumulative sum (blocked: each thread owns 4
 // consecutive elements).
 __global__ void isect_cum_kernel(uint64_t total, const int32_t *__restrict__ tpg,
-                                 const int64_t *__restrict__ block_offsets, int64_t *__restrict__ cum) {
+                                 const int64_t *__restrict__ block_offsets, int64_t *__restrict__ cum,
+                                 const int32_t *__restrict__ order) {
     __shared__ int64_t lds_wave[kThreads / 64];
     const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
     int32_t v[kScanItems];
     int64_t s = 0;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
-        v[k] = (base + k < total) ? tpg[base + k] : 0;
+        v[k] = (base + k < total) ? tpg[order ? (uint64_t)order[base + k] : base + k] : 0;
         s += v[k];
     }
     int64_t tot;
@@ -165,7 +166,40 @@ __global__ void isect_cum_kernel(uint64_t total, const int32_t *__restrict__ tpg
     }
 }
 
+// Depth-first emission (an order of the same entries that a stable sort on
+// the camera/tile bits alone turns into the full sort order): the
+// (camera, primitive) pairs sorted stably by depth, then emitted in that
+// order.  Keys: the visible depths' bit patterns, 0xFFFFFFFF for culled
+// pairs (they emit nothing); values: the flat ids.
+__global__ void isect_depth_keys_kernel(uint64_t total, const int32_t *__restrict__ radii,
+                                        const float *__restrict__ depths, int64_t *__restrict__ keys,
+                                        int32_t *__restrict__ vals) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    keys[i] = radii[i] > 0 ? (int64_t)__float_as_uint(depths[i]) : (int64_t)0xFFFFFFFFu;
+    vals[i] = (int32_t)i;
+}
+
+// Block sums of the tile counts taken in `order` (the scan's pass 1 for the
+// depth-first emission).
+__global__ void isect_ordered_sums_kernel(uint64_t total, const int32_t *__restrict__ tpg,
+                                          const int32_t *__restrict__ order, int64_t *__restrict__ block_sums) {
+    __shared__ int64_t lds_wave[kThreads / 64];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+    int64_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const uint64_t idx = base + (uint64_t)k * kThreads + threadIdx.x;
+        if (idx < total) s += tpg[order[idx]];
+    }
+    int64_t tot;
+    block_exclusive_scan(s, lds_wave, tot);
+    if (threadIdx.x == 0) block_sums[blockIdx.x] = tot;
+}
+
 // Pass 2: keys cam | tile | depth bits.  reference: isect_tiles.cu:76-96
+// Primitives are taken in flat-id order, or in `order` (depth-first
+// emission, with cum the prefix of the counts in that order).
 // The 64 primitives of a wave own one contiguous run of the output (their
 // counts' prefix from pass 1); the wave writes that run with consecutive
 // lanes on consecutive entries: entry k of the run belongs to the last lane
@@ -179,13 +213,15 @@ __global__ __launch_bounds__(256) void isect_emit_kernel(uint32_t C, uint32_t N,
                                                          const int64_t *__restrict__ cum, uint32_t ts, uint32_t tw,
                                                          uint32_t th, uint32_t tile_bits,
                                                          int64_t *__restrict__ isect_ids,
-                                                         int32_t *__restrict__ flatten_ids) {
+                                                         int32_t *__restrict__ flatten_ids,
+                                                         const int32_t *__restrict__ order) {
     const uint64_t CN = (uint64_t)C * N;
-    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t pos = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;  // place in emission order
     const int lane = (int)(threadIdx.x & 63);
-    const uint64_t idx0 = idx - (uint64_t)lane;
-    if (idx0 >= CN) return;  // wave-uniform
-    const bool valid = idx < CN;
+    const uint64_t pos0 = pos - (uint64_t)lane;
+    if (pos0 >= CN) return;  // wave-uniform
+    const bool valid = pos < CN;
+    const uint64_t idx = !valid ? 0 : order ? (uint64_t)order[pos] : pos;  // the flat (camera, primitive) id
     const int32_t r = valid ? radii[idx] : 0;
     uint32_t x0 = 0, y0 = 0, x1 = 0, y1 = 0;
     if (r > 0) tile_box(means2d[idx * 2], means2d[idx * 2 + 1], r, ts, tw, th, x0, y0, x1, y1);
@@ -200,7 +236,7 @@ __global__ __launch_bounds__(256) void isect_emit_kernel(uint32_t C, uint32_t N,
     }
     const uint32_t excl = inc - cnt;
     const uint32_t total = __shfl(inc, 63, 64);
-    const int64_t wave_start = idx0 == 0 ? 0 : cum[idx0 - 1];
+    const int64_t wave_start = pos0 == 0 ? 0 : cum[pos0 - 1];
     const int64_t cid = valid ? (int64_t)(idx / N) : 0;
     const int64_t key_hi = (cid << (32 + tile_bits)) | (int64_t)__float_as_int(valid ? depths[idx] : 0.f);
     const int32_t fid = (int32_t)idx;
@@ -305,7 +341,7 @@ int ubs_isect_tiles_count_range(uint32_t C, uint32_t N, const float *means2d, co
     UBS_LAUNCH_CHECK("isect_count_kernel");
     isect_scan_blocks_kernel<<<1, kThreads, 0, st>>>((int64_t)nb, bs, n_isects_dev, br, range_dev);
     UBS_LAUNCH_CHECK("isect_scan_blocks_kernel");
-    isect_cum_kernel<<<(unsigned)nb, kThreads, 0, st>>>(total, tpg, bs, cum);
+    isect_cum_kernel<<<(unsigned)nb, kThreads, 0, st>>>(total, tpg, bs, cum, nullptr);
     UBS_LAUNCH_CHECK("isect_cum_kernel");
     return UBS_OK;
 }
@@ -319,7 +355,61 @@ int ubs_isect_tiles_emit(uint32_t C, uint32_t N, const float *means2d, const int
     UBS_REQUIRE(means2d && radii && depths && cum && isect_ids && flatten_ids, "null pointer");
     const uint32_t tile_bits = bit_width_u32(tw * th);
     isect_emit_kernel<<<grid_for((int64_t)total), kThreads, 0, as_stream(stream)>>>(
-        C, N, means2d, radii, depths, cum, ts, tw, th, tile_bits, isect_ids, flatten_ids);
+        C, N, means2d, radii, depths, cum, ts, tw, th, tile_bits, isect_ids, flatten_ids, nullptr);
+    UBS_LAUNCH_CHECK("isect_emit_kernel");
+    return UBS_OK;
+}
+
+int ubs_isect_depth_keys(uint32_t C, uint32_t N, const int32_t *radii, const float *depths, int64_t *keys,
+                         int32_t *vals, void *stream) {
+    const uint64_t total = (uint64_t)C * N;
+    if (total == 0) return UBS_OK;
+    UBS_REQUIRE(total < (1ull << 31), "C*N must be < 2^31");
+    UBS_REQUIRE(radii && depths && keys && vals, "null pointer");
+    isect_depth_keys_kernel<<<grid_for((int64_t)total), kThreads, 0, as_stream(stream)>>>(total, radii, depths, keys,
+                                                                                         vals);
+    UBS_LAUNCH_CHECK("isect_depth_keys_kernel");
+    return UBS_OK;
+}
+
+size_t ubs_isect_ordered_workspace_bytes(uint32_t C, uint32_t N) {
+    const uint64_t total = (uint64_t)C * N;
+    const uint64_t nb = (total + kScanTile - 1) / kScanTile;
+    return (size_t)(std::max<uint64_t>(nb, 1) + 1) * sizeof(int64_t);
+}
+
+int ubs_isect_scan_ordered(uint32_t C, uint32_t N, const int32_t *tpg, const int32_t *order, int64_t *cum,
+                           void *workspace, size_t ws_bytes, void *stream) {
+    hipStream_t st = as_stream(stream);
+    const uint64_t total = (uint64_t)C * N;
+    if (total == 0) return UBS_OK;
+    UBS_REQUIRE(total < (1ull << 31), "C*N must be < 2^31");
+    UBS_REQUIRE(tpg && order && cum, "null pointer");
+    if (!workspace || ws_bytes < ubs_isect_ordered_workspace_bytes(C, N)) {
+        set_error("isect workspace too small (%zu < %zu)", ws_bytes, ubs_isect_ordered_workspace_bytes(C, N));
+        return UBS_ERR_WORKSPACE;
+    }
+    const uint64_t nb = (total + kScanTile - 1) / kScanTile;
+    int64_t *bs = reinterpret_cast<int64_t *>(workspace);
+    isect_ordered_sums_kernel<<<(unsigned)nb, kThreads, 0, st>>>(total, tpg, order, bs);
+    UBS_LAUNCH_CHECK("isect_ordered_sums_kernel");
+    isect_scan_blocks_kernel<<<1, kThreads, 0, st>>>((int64_t)nb, bs, bs + nb, nullptr, nullptr);
+    UBS_LAUNCH_CHECK("isect_scan_blocks_kernel");
+    isect_cum_kernel<<<(unsigned)nb, kThreads, 0, st>>>(total, tpg, bs, cum, order);
+    UBS_LAUNCH_CHECK("isect_cum_kernel");
+    return UBS_OK;
+}
+
+int ubs_isect_tiles_emit_ordered(uint32_t C, uint32_t N, const float *means2d, const int32_t *radii,
+                                 const float *depths, const int32_t *order, const int64_t *cum, uint32_t ts,
+                                 uint32_t tw, uint32_t th, int64_t *isect_ids, int32_t *flatten_ids, void *stream) {
+    const uint64_t total = (uint64_t)C * N;
+    if (total == 0) return UBS_OK;
+    UBS_REQUIRE(total < (1ull << 31), "C*N must be < 2^31");
+    UBS_REQUIRE(means2d && radii && depths && order && cum && isect_ids && flatten_ids, "null pointer");
+    const uint32_t tile_bits = bit_width_u32(tw * th);
+    isect_emit_kernel<<<grid_for((int64_t)total), kThreads, 0, as_stream(stream)>>>(
+        C, N, means2d, radii, depths, cum, ts, tw, th, tile_bits, isect_ids, flatten_ids, order);
     UBS_LAUNCH_CHECK("isect_emit_kernel");
     return UBS_OK;
 }

@@ -66,6 +66,11 @@ _SIGNATURES = {
     "ubs_radix_sort_pairs_skip": (
         _c_int, [_c_i64, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_u32, _vp, _c_size, ctypes.POINTER(_c_int), _vp]),
     "ubs_radix_sort_status": (_c_int, [_vp, ctypes.POINTER(_c_int), _vp]),
+    "ubs_isect_depth_keys": (_c_int, [_c_u32, _c_u32, _vp, _vp, _vp, _vp, _vp]),
+    "ubs_isect_ordered_workspace_bytes": (_c_size, [_c_u32, _c_u32]),
+    "ubs_isect_scan_ordered": (_c_int, [_c_u32, _c_u32, _vp, _vp, _vp, _vp, _c_size, _vp]),
+    "ubs_isect_tiles_emit_ordered": (
+        _c_int, [_c_u32, _c_u32, _vp, _vp, _vp, _vp, _vp, _c_u32, _c_u32, _c_u32, _vp, _vp, _vp]),
     "ubs_sort_tile_segments": (_c_int, [_c_i64, _vp, _vp, _vp, _vp, _vp, _c_u32, _c_u32, _c_u32, _c_u32, _vp]),
     "ubs_isect_tiles_count_range": (
         _c_int, [_c_u32, _c_u32, _vp, _vp, _vp, _c_u32, _c_u32, _c_u32, _vp, _vp, _vp, _vp, _vp, _c_size, _vp]),

@@ -263,15 +263,32 @@ def isect_tiles(
     return isect_tiles_offsets(means2d, radii, depths, tile_size, tile_width, tile_height, sort)[:3]
 
 
-_segmented_sort = True
+_SORT_MODES = ("depth_first", "segmented", "onesweep")
+_sort_mode = "depth_first"
+
+
+def set_isect_sort_mode(mode: str) -> None:
+    """How isect ids reach their sorted order; every mode gives the same
+    ids and values, bit for bit:
+      depth_first (default): the (camera, primitive) pairs sorted by depth
+        once, entries emitted in that order, then two onesweep passes over
+        the camera/tile bits;
+      segmented: entries in primitive order, onesweep over the camera/tile
+        bits, then a per-tile depth sort in LDS;
+      onesweep: onesweep passes over every byte of the ids."""
+    global _sort_mode
+    if mode not in _SORT_MODES:
+        raise ValueError(f"sort mode must be one of {_SORT_MODES}")
+    _sort_mode = mode
+
+
+def isect_sort_mode() -> str:
+    return _sort_mode
 
 
 def set_segmented_depth_sort(enable: bool) -> None:
-    """Sort isect ids as two onesweep passes over the camera/tile bits plus a
-    per-tile depth sort in LDS (default), or as onesweep passes over every
-    byte.  Both give the same order, bit for bit."""
-    global _segmented_sort
-    _segmented_sort = bool(enable)
+    """segmented (True) or onesweep (False) sort mode."""
+    set_isect_sort_mode("segmented" if enable else "onesweep")
 
 
 def isect_tiles_offsets(means2d: Tensor, radii: Tensor, depths: Tensor, tile_size: int, tile_width: int,
@@ -298,16 +315,32 @@ def isect_tiles_offsets(means2d: Tensor, radii: Tensor, depths: Tensor, tile_siz
     ws = torch.empty((ws_bytes,), dtype=torch.uint8, device=dev)
     _call("ubs_isect_tiles_count_range", C, N, ptr(means2d), ptr(radii), ptr(depths), tile_size, tile_width,
           tile_height, ptr(tpg), ptr(cum), ptr(info), _vp_at(info, 1) if sort else None, ptr(ws), ws_bytes, st)
+    depth_first = sort and _sort_mode == "depth_first" and C * N > 0
+    if depth_first:
+        # queued before the host read below, so it overlaps the round trip
+        dkeys = torch.empty((C * N,), dtype=torch.int64, device=dev)
+        dvals = torch.empty((C * N,), dtype=torch.int32, device=dev)
+        _call("ubs_isect_depth_keys", C, N, ptr(radii), ptr(depths), ptr(dkeys), ptr(dvals), st)
+        _, order = radix_sort_pairs(dkeys, dvals, 0, 32)
+        cum = torch.empty((C * N,), dtype=torch.int64, device=dev)
+        ows = int(L.ubs_isect_ordered_workspace_bytes(C, N))
+        ow = torch.empty((ows,), dtype=torch.uint8, device=dev)
+        _call("ubs_isect_scan_ordered", C, N, ptr(tpg), ptr(order), ptr(cum), ptr(ow), ows, st)
     n_isects, packed = (int(x) for x in info.cpu())  # host sync, as isect_tiles.cu:181
     isect_ids = torch.empty((n_isects,), dtype=torch.int64, device=dev)
     flatten_ids = torch.empty((n_isects,), dtype=torch.int32, device=dev)
-    if n_isects:
+    if n_isects and depth_first:
+        _call("ubs_isect_tiles_emit_ordered", C, N, ptr(means2d), ptr(radii), ptr(depths), ptr(order), ptr(cum),
+              tile_size, tile_width, tile_height, ptr(isect_ids), ptr(flatten_ids), st)
+    elif n_isects:
         _call("ubs_isect_tiles_emit", C, N, ptr(means2d), ptr(radii), ptr(depths), ptr(cum), tile_size,
               tile_width, tile_height, ptr(isect_ids), ptr(flatten_ids), st)
     offsets = None
     if n_isects and sort:
         end_bit = 32 + int(L.ubs_isect_prefix_bits(C, tile_width, tile_height))
-        if _segmented_sort:
+        if depth_first:  # entries already in depth order: stable on the camera/tile bits finishes it
+            isect_ids, flatten_ids = radix_sort_pairs(isect_ids, flatten_ids, 32, end_bit)
+        elif _sort_mode == "segmented":
             # stable on the camera/tile bits, then per tile on the depth bits
             ids1, flat1 = radix_sort_pairs(isect_ids, flatten_ids, 32, end_bit)
             offsets = isect_offset_encode(ids1, C, tile_width, tile_height)
