@@ -33,6 +33,7 @@ constexpr uint32_t kFlagAgg = 1u << 30;
 constexpr uint32_t kFlagPrefix = 2u << 30;
 constexpr uint32_t kValueMask = (1u << 30) - 1u;
 constexpr uint32_t kSpinLimit = 1u << 22;
+constexpr int kLookback = 8;  // predecessor slots read per look-back round trip
 
 struct SortLayout {
     size_t status_off, hist_off, counter_off, lookback_off, total;
@@ -219,23 +220,41 @@ __global__ __launch_bounds__(kSortThreads) void sort_pass_kernel(
         const int d = tid;
         uint32_t excl = 0;
         if (tile > 0) {
+            // look back kLookback tiles per round trip: their slots are read
+            // together, then taken in order until an inclusive prefix (done)
+            // or a tile that has not published yet (read again from there)
             int64_t j = tile - 1;
             uint32_t spins = 0;
             while (j >= 0) {
-                const uint32_t s = __hip_atomic_load(lookback + (size_t)j * kRadix + d, __ATOMIC_RELAXED,
-                                                     __HIP_MEMORY_SCOPE_AGENT);
-                const uint32_t flag = s & ~kValueMask;
-                if (flag == 0u) {
+                uint32_t v[kLookback];
+#pragma unroll
+                for (int u = 0; u < kLookback; ++u)
+                    v[u] = j - u >= 0 ? __hip_atomic_load(lookback + (size_t)(j - u) * kRadix + d, __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_AGENT)
+                                      : kFlagPrefix;  // (tile 0 publishes a prefix: never reached)
+                bool done = false, wait = false;
+                int taken = 0;
+#pragma unroll
+                for (int u = 0; u < kLookback; ++u) {
+                    if (done || wait) continue;
+                    const uint32_t flag = v[u] & ~kValueMask;
+                    if (flag == 0u) {
+                        wait = true;
+                        continue;
+                    }
+                    excl += v[u] & kValueMask;
+                    ++taken;
+                    if (flag == kFlagPrefix) done = true;
+                }
+                if (done) break;
+                j -= taken;
+                if (wait) {
                     if (++spins > kSpinLimit) {
                         __hip_atomic_store(status, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         break;
                     }
                     __builtin_amdgcn_s_sleep(1);
-                    continue;
                 }
-                excl += s & kValueMask;
-                if (flag == kFlagPrefix) break;
-                --j;
             }
             __hip_atomic_store(lookback + (size_t)tile * kRadix + d, kFlagPrefix | ((excl + tot) & kValueMask),
                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
