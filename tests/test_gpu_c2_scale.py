@@ -1,5 +1,6 @@
-"""Parity at the full size of the headline workload: 1M-primitive 6-D scene,
-one 1920x1080 view (the benchmark's scene and camera).
+"""Parity at full size: the headline workload (1M-primitive 6-D scene, one
+1920x1080 view, the benchmark's scene and camera) and the dynamic one
+(1M-primitive 7-D scene, one 1352x1014 view at its time).
 
 * Structure through the drop-in chain -- covariance, conditioned mean and
   covariance, projection, tile intersections, sorted keys and values, tile
@@ -24,21 +25,27 @@ from util import assert_same, rel_l2
 
 pytestmark = pytest.mark.gpu
 
-N, D, WD, HT = 1_000_000, 6, 1920, 1080
+N = 1_000_000
+CASES = {"c2": (6, 1920, 1080), "c3": (7, 1352, 1014)}
 
 
 def host(t):
     return t.detach().cpu().numpy()
 
 
-@pytest.fixture(scope="module")
-def c2(device):
-    sc = scenes.make_scene(N, D, seed=0)
-    vms, Ks, ts = scenes.make_cameras(1, WD, HT, seed=0)
-    return sc, vms, Ks
+_scenes = {}
 
 
-def _tile_mask(n_tiles=64, seed=1):
+def _case(name):
+    if name not in _scenes:
+        D, WD, HT = CASES[name]
+        sc = scenes.make_scene(N, D, seed=0)
+        vms, Ks, ts = scenes.make_cameras(1, WD, HT, seed=0)
+        _scenes[name] = (sc, vms, Ks, float(ts[0]) if D == 7 else 0.0)
+    return _scenes[name]
+
+
+def _tile_mask(WD, HT, n_tiles=64, seed=1):
     tw, th = -(-WD // 16), -(-HT // 16)
     rng = np.random.Generator(np.random.PCG64(seed))
     pick = rng.choice(tw * th, n_tiles, replace=False)
@@ -47,9 +54,11 @@ def _tile_mask(n_tiles=64, seed=1):
     return m.reshape(th, tw)
 
 
-def test_c2_structure_bit_exact(device, c2):
-    sc, vms, Ks = c2
-    print(f"c2 structure: N={N} {WD}x{HT}", flush=True)
+@pytest.mark.parametrize("case", ["c2", "c3"])
+def test_full_size_structure_bit_exact(device, case):
+    sc, vms, Ks, t = _case(case)
+    D, WD, HT = CASES[case]
+    print(f"{case} structure: N={N} {D}-D {WD}x{HT}", flush=True)
     p = beta.BetaParams.from_scene(sc, device, requires_grad=False)
     vm, K = torch.tensor(vms[0], device=device), torch.tensor(Ks[0], device=device)
     scale = torch.nn.functional.softplus(p.scale_raw)
@@ -63,6 +72,8 @@ def test_c2_structure_bit_exact(device, c2):
     mean = torch.cat([p.xyz, p.mean_rest], -1)
     vd = p.xyz - beta.camera_center(vm)[None]
     q = vd / vd.norm(dim=-1, keepdim=True)
+    if D == 7:
+        q = torch.cat([q, torch.full_like(q[:, :1], t)], -1)
     m, V, o = W.cond_mean_convariance_opacity(mean, cov, opac, bet[:, 1:].contiguous(), q)
     om, oV, oo = O.cond_fwd(host(mean), ocov, host(opac), host(bet[:, 1:]), host(q))
     assert_same(host(m), om, "m_cond")
@@ -71,7 +82,7 @@ def test_c2_structure_bit_exact(device, c2):
     np.testing.assert_allclose(host(o), oo, rtol=2e-5, atol=1e-9)
     img, alpha, meta = rasterization(m, p.l_triangle, scale, o[:, 0], bet[:, 0].contiguous(), p.rgb, vm[None],
                                      K[None], WD, HT, covars=V)
-    tm = _tile_mask()
+    tm = _tile_mask(WD, HT)
     ref_img, _, rm = O.rasterization(om, oV, host(o[:, 0]), host(bet[:, 0]), host(p.rgb), vms[:1], Ks[:1], WD, HT,
                                      tile_mask=tm[None])
     for k in ("radii", "tiles_per_gauss", "isect_ids", "flatten_ids", "isect_offsets"):
@@ -79,7 +90,7 @@ def test_c2_structure_bit_exact(device, c2):
     vis = rm["radii"] > 0
     for k in ("means2d", "depths", "conics"):
         assert_same(host(meta[k])[vis], rm[k][vis], k)
-    assert rm["isect_ids"].size > 10_000_000
+    assert rm["isect_ids"].size > 5_000_000
     # the float32 blend of the same lists on the 64 tiles, away from the T cut-off
     _, _, _, mT, mS = O.raster_fwd(rm["means2d"], rm["conics"], rm["colors"], rm["opacities"], rm["betas"], WD, HT,
                                    16, rm["isect_offsets"], rm["flatten_ids"], masks=tm[None], margins=True)
@@ -89,22 +100,25 @@ def test_c2_structure_bit_exact(device, c2):
     assert np.abs(host(img[0]) - ref_img[0])[safe].max() <= 5e-5
 
 
-def test_c2_tiles_image_and_gradients_vs_fp64(device, c2):
-    sc, vms, Ks = c2
-    print(f"c2 fp64 tiles: N={N} {WD}x{HT}", flush=True)
+@pytest.mark.parametrize("case", ["c2", "c3"])
+def test_full_size_tiles_image_and_gradients_vs_fp64(device, case):
+    sc, vms, Ks, t = _case(case)
+    D, WD, HT = CASES[case]
+    print(f"{case} fp64 tiles: N={N} {D}-D {WD}x{HT}", flush=True)
     tgt = np.random.Generator(np.random.PCG64(5)).uniform(0, 1, (HT, WD, 3))
-    tm = _tile_mask()
+    tm = _tile_mask(WD, HT)
     pm = np.kron(tm, np.ones((16, 16), bool))[:HT, :WD]
     p = beta.BetaParams.from_scene(sc, device)
-    img, alpha, meta = beta.render(p, torch.tensor(vms[0], device=device), torch.tensor(Ks[0], device=device), WD, HT)
+    img, alpha, meta = beta.render(p, torch.tensor(vms[0], device=device), torch.tensor(Ks[0], device=device), WD, HT,
+                                   timestamp=t)
     pm_t = torch.tensor(pm, device=device)[..., None]
     loss = ((img[0] - torch.tensor(tgt, dtype=torch.float32, device=device)).abs() * pm_t).sum() / (pm.sum() * 3)
     loss.backward()
     struct = tuple(host(meta[k]) for k in ("tiles_per_gauss", "isect_ids", "flatten_ids", "isect_offsets"))
     leaves = dict(xyz=sc.xyz, mean_rest=sc.mean_rest, scale_raw=sc.scale_raw, l_triangle=sc.l_triangle,
                   opacity_raw=sc.opacity_raw, beta_raw=sc.beta_raw, rgb=sc.rgb, quats=sc.quats)
-    ref_img, ref_g, rm = O.beta_step(leaves, vms[0], Ks[0], WD, HT, tgt, dtype=np.float64, structure=struct,
-                                     tile_mask=tm)
+    ref_img, ref_g, rm = O.beta_step(leaves, vms[0], Ks[0], WD, HT, tgt, timestamp=t, dtype=np.float64,
+                                     structure=struct, tile_mask=tm)
     _, _, _, mT, mS = O.raster_fwd(rm["means2d"], rm["conics"], rm["colors"], rm["opacities"], rm["betas"], WD, HT,
                                    16, rm["isect_offsets"], rm["flatten_ids"], masks=tm[None], margins=True)
     safe = pm & (mT[0] > 1e-3) & (mS[0] > 1e-6)
@@ -120,11 +134,14 @@ def test_c2_tiles_image_and_gradients_vs_fp64(device, c2):
     assert touched > 1000  # the 64 tiles reach many primitives
 
 
-def test_c2_fused_path_renders_the_same_image(device, c2):
-    sc, vms, Ks = c2
+@pytest.mark.parametrize("case", ["c2", "c3"])
+def test_full_size_fused_path_renders_the_same_image(device, case):
+    sc, vms, Ks, t = _case(case)
+    D, WD, HT = CASES[case]
     p = beta.BetaParams.from_scene(sc, device, requires_grad=False)
     vm, K = torch.tensor(vms[0], device=device), torch.tensor(Ks[0], device=device)
+    ts = torch.tensor([t], dtype=torch.float32, device=device) if D == 7 else None
     with torch.no_grad():
-        a, aa, _ = beta.render(p, vm, K, WD, HT)
-        b, bb, _ = beta.render_views(p, vm[None].contiguous(), K[None].contiguous(), WD, HT)
+        a, aa, _ = beta.render(p, vm, K, WD, HT, timestamp=t)
+        b, bb, _ = beta.render_views(p, vm[None].contiguous(), K[None].contiguous(), WD, HT, timestamps=ts)
     assert torch.equal(a, b) and torch.equal(aa, bb)
