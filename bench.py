@@ -216,6 +216,8 @@ def main():
     ap.add_argument("--path", default="fused", choices=("fused", "chain"),
                     help="fused: one-launch Beta projection (render_views); chain: the reference's op chain "
                          "(render: activations, rotation, covariance, conditioning, projection as separate ops)")
+    ap.add_argument("--serve-streams", type=int, default=2, help="c5: HIP streams of the batched server")
+    ap.add_argument("--serve-max-views", type=int, default=16, help="c5: most views per batch")
     ap.add_argument("--sort", default="depth_first", choices=("depth_first", "segmented", "onesweep"),
                     help="isect-id sort (same order every way): depth_first (default) sorts the primitives by depth "
                          "once and emits in that order, then onesweep on the camera/tile bits; segmented: onesweep "
@@ -472,7 +474,8 @@ def serve_bench(args, cfg):
                            torch.tensor(scenes.intrinsics(W, H), dtype=torch.float32), W, H,
                            t=float(rng.uniform(0, 1)), rid=i)
 
-    modes = {"batched": dict(max_views=16, streams=2), "per_view": dict(max_views=1, streams=1)}
+    modes = {"batched": dict(max_views=args.serve_max_views, streams=args.serve_streams),
+             "per_view": dict(max_views=1, streams=1)}
 
     def run(reqs, arrivals, mode):
         with no_gc():
@@ -525,8 +528,8 @@ def serve_bench(args, cfg):
         "vs_baseline": None, "dtype": "fp32",
         "data": "synthetic (seeded scenes, cameras and arrivals, gsplat/scenes.py)",
         "config": {"workload": cfg["workload"], "primitives_per_scene": cfg["N"], "scenes": len(names),
-                   "resident_gb": round(store.resident_bytes() / 1e9, 3), "max_views_per_batch": 16, "sort": args.sort,
-                   "streams": 2, "peak_hbm_gb": round(torch.cuda.max_memory_allocated() / 1e9, 2)},
+                   "resident_gb": round(store.resident_bytes() / 1e9, 3), "max_views_per_batch": args.serve_max_views,
+                   "sort": args.sort, "streams": args.serve_streams, "peak_hbm_gb": round(torch.cuda.max_memory_allocated() / 1e9, 2)},
         "saturation": saturation,
         "speedup_vs_per_view": round(b["mpix_per_s"] / pv["mpix_per_s"], 3),
         "poisson": poisson,
