@@ -141,19 +141,19 @@ def algorithmic_bytes(name: str, c: dict, n_isect: int, C: int = 1) -> float:
 
 
 def measured_counters(entry: str, workload: str, n_isect: int):
-    """(HBM bytes per call, VALU-busy share of the entry's main kernel) from the
-    committed rocprofv3 --pmc summary of this same workload, or (None, None)
-    when there is none for it."""
+    """(HBM bytes per call, VALU issue share and LDS busy share of the entry's
+    main kernel) from the committed rocprofv3 --pmc summary of this same
+    workload, or Nones when there is none for it."""
     try:
         with open(os.path.join(ROOT, "profiles", "pmc_traffic.json")) as f:
             d = json.load(f)
     except (OSError, ValueError):
-        return None, None
+        return None, None, None
     if d.get("workload") != workload or d.get("n_isects") != n_isect or entry not in d.get("entries", {}):
-        return None, None
+        return None, None, None
     e = d["entries"][entry]
-    busy = d.get("valu", {}).get(e["kernels"][0], {}).get("valu_busy") if e.get("kernels") else None
-    return e["traffic_bytes"], busy
+    util = d.get("valu", {}).get(e["kernels"][0], {}) if e.get("kernels") else {}
+    return e["traffic_bytes"], util.get("valu_busy"), util.get("lds_util")
 
 
 def cpu_baseline(cfg: dict, seed: int = 0):
@@ -206,6 +206,8 @@ def main():
                          "c4: batched 8-view step at mixed resolutions (reports views/s); "
                          "c5: render serving of 4 resident scenes (reports MPix/s, p50/p99)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="leave out the batched-serve measurement (for counter runs that average per kernel)")
     ap.add_argument("--lambda-dssim", type=float, default=0.0,
                     help="image loss (1 - l) L1 + l (1 - SSIM) in the fused loss kernels; 0 (default) is the L1 "
                          "loss of the benchmark definition, 0.2 the reference's training loss")
@@ -340,7 +342,7 @@ def main():
     # ---- batched serving of the same scene: 8 views at the same size in one
     # pass (fused projection, one isect/sort/blend over all views) against
     # the same 8 views rendered one at a time; forward only
-    serve = serve_views(params, cfg, args.steps)
+    serve = None if args.no_serve else serve_views(params, cfg, args.steps)
 
     # ---- roofline of the dominant kernel launch
     per_call = {k: (sum(v) / len(v), len(v) / args.steps) for k, v in launches.items()}
@@ -348,7 +350,7 @@ def main():
     avg_ms = per_call[dominant][0]
     alg = algorithmic_bytes(dominant, cfg, n_isect)
     achieved = alg / (avg_ms * 1e-3) / 1e9
-    traffic, valu_busy = measured_counters(dominant, cfg["workload"], n_isect)
+    traffic, valu_busy, lds_busy = measured_counters(dominant, cfg["workload"], n_isect)
     roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": alg}
@@ -356,6 +358,7 @@ def main():
         roofline["traffic_source"] = ("profiles/pmc_traffic.json: rocprofv3 FETCH_SIZE and WRITE_SIZE of this "
                                       "workload, scaled against a 1 GiB device copy")
         roofline["valu_busy"] = valu_busy  # the blend is VALU-bound, not HBM-bound
+        roofline["lds_busy"] = lds_busy
     def gbps(k, ms):
         r = algorithmic_bytes(k, cfg, n_isect) / (ms * 1e-3) / 1e9
         return round(r, 1) if math.isfinite(r) else None  # JSON has no NaN
