@@ -25,9 +25,14 @@ constexpr int kRadix = 1 << kRadixBits;
 constexpr int kSortThreads = 256;
 constexpr int kSortWaves = kSortThreads / 64;
 // 24 keys per thread: fewer, larger tiles amortise the per-tile digit scan
-// and look-back (measured 0.86 ms per 12.3M-pair sort vs 0.93 ms at 16 and
-// 1.2 ms at 28-32, where the LDS staging leaves one workgroup per CU)
-constexpr int kItems = 24;
+// and look-back (measured 0.86 ms per 12.3M-pair 5-pass sort vs 0.93 ms at
+// 16 and 1.2 ms at 28-32, where the LDS staging leaves one workgroup per
+// CU; for the 2-pass camera/tile sort 12 to 24 measure alike).  Build with
+// -DUBS_SORT_ITEMS=n to try another size.
+#ifndef UBS_SORT_ITEMS
+#define UBS_SORT_ITEMS 24
+#endif
+constexpr int kItems = UBS_SORT_ITEMS;
 constexpr int kTile = kSortThreads * kItems;  // 6144 keys per workgroup
 constexpr uint32_t kFlagAgg = 1u << 30;
 constexpr uint32_t kFlagPrefix = 2u << 30;
