@@ -198,7 +198,10 @@ __global__ __launch_bounds__(TS *TS, UBS_BWD_WAVES_PER_EU) void raster_bwd_kerne
             const float opac = g2.y, beta = g2.z;
             const float oms = valid ? 1.f - sigma : 1.f;
             const float lg = __builtin_amdgcn_logf(oms);  // log2(1 - sigma)
-            const float vis = __builtin_amdgcn_exp2f(beta * lg);
+            // (1-sigma)^(beta-1) by one exp2, (1-sigma)^beta from it by one
+            // multiply (one transcendental fewer than two exp2)
+            const float pm1 = __builtin_amdgcn_exp2f((beta - 1.f) * lg);
+            const float vis = pm1 * oms;
             const float ov = opac * vis;
             const float alpha = valid ? fminf(0.999f, ov) : 0.f;
             float ra;
@@ -227,7 +230,6 @@ __global__ __launch_bounds__(TS *TS, UBS_BWD_WAVES_PER_EU) void raster_bwd_kerne
             bufv = __builtin_fmaf(cdot, fac, bufv);
             // geometry, opacity and beta get gradient only below the 0.999 clamp
             const float va = (valid && ov <= 0.999f) ? v_alpha : 0.f;
-            const float pm1 = __builtin_amdgcn_exp2f((beta - 1.f) * lg);  // (1-sigma)^(beta-1)
             // v_sigma = -opac beta va pm1; the per-primitive factors (-opac beta,
             // the 2 of the symmetric terms, opac ln 2) are applied once per
             // (tile, primitive) at the flush, not per pixel
