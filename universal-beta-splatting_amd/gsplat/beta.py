@@ -25,6 +25,8 @@ from .cuda._wrapper import (
     beta_project,
     camera_centers,
     cond_mean_convariance_opacity,
+    isect_begin,
+    isect_finish,
     isect_offset_encode,
     isect_tiles_offsets,
     l_triangle_to_rotmat,
@@ -235,22 +237,30 @@ def render_batch(params: BetaParams, viewmats: Tensor, Ks: Tensor, sizes, timest
         params.quats, viewmats, Ks, 0, 0, cam_pos=cam_pos, timestamps=timestamps, image_sizes=wh, eps2d=eps2d,
         near_plane=near_plane, far_plane=far_plane, radius_clip=radius_clip)
     colors_out, alphas_out = [None] * C, [None] * C
-    c0 = 0
+    groups, c0 = [], 0
     while c0 < C:
-        W, H = sizes[order[c0]]
         c1 = c0
-        while c1 < C and sizes[order[c1]] == (W, H):
+        while c1 < C and sizes[order[c1]] == sizes[order[c0]]:
             c1 += 1
+        groups.append((c0, c1))
+        c0 = c1
+    # every group's counts first, then one host read of all their n_isects
+    states = []
+    for c0, c1 in groups:
+        W, H = sizes[order[c0]]
+        tw, th = math.ceil(W / float(tile_size)), math.ceil(H / float(tile_size))
+        states.append(isect_begin(means2d[c0:c1], radii[c0:c1], depths[c0:c1], tile_size, tw, th))
+    infos = torch.stack([st.info for st in states]).cpu() if states else None
+    for g, (c0, c1) in enumerate(groups):
+        W, H = sizes[order[c0]]
         Cg = c1 - c0
         tw, th = math.ceil(W / float(tile_size)), math.ceil(H / float(tile_size))
-        m2, r, d = means2d[c0:c1], radii[c0:c1], depths[c0:c1]
-        _, isect_ids, flatten_ids, offsets = isect_tiles_offsets(m2, r, d, tile_size, tw, th)
+        _, isect_ids, flatten_ids, offsets = isect_finish(states[g], int(infos[g, 0]), int(infos[g, 1]))
         if offsets is None:
             offsets = isect_offset_encode(isect_ids, Cg, tw, th)
         bg = backgrounds[c0:c1] if backgrounds is not None else None
-        rc, ra = rasterize_to_pixels(m2, conics[c0:c1], params.rgb.expand(Cg, -1, -1), opacities[c0:c1],
+        rc, ra = rasterize_to_pixels(means2d[c0:c1], conics[c0:c1], params.rgb.expand(Cg, -1, -1), opacities[c0:c1],
                                      betas[c0:c1], W, H, tile_size, offsets, flatten_ids, backgrounds=bg)
         for k in range(Cg):
             colors_out[order[c0 + k]], alphas_out[order[c0 + k]] = rc[k], ra[k]
-        c0 = c1
     return colors_out, alphas_out

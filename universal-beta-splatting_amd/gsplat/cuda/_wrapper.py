@@ -296,6 +296,24 @@ def isect_tiles_offsets(means2d: Tensor, radii: Tensor, depths: Tensor, tile_siz
     """isect_tiles, plus the tile offsets of the sorted ids when the sort
     computed them on the way (else None): (tiles_per_gauss, isect_ids,
     flatten_ids, isect_offsets or None)."""
+    state = isect_begin(means2d, radii, depths, tile_size, tile_width, tile_height, sort)
+    n_isects, packed = (int(x) for x in state.info.cpu())  # host sync, as isect_tiles.cu:181
+    return isect_finish(state, n_isects, packed)
+
+
+class IsectState:
+    """The intersection stage up to the host read of n_isects (see isect_begin)."""
+    __slots__ = ("means2d", "radii", "depths", "tile_size", "tile_width", "tile_height", "sort", "tpg", "cum",
+                 "info", "order", "C", "N")
+
+
+def isect_begin(means2d: Tensor, radii: Tensor, depths: Tensor, tile_size: int, tile_width: int, tile_height: int,
+                sort: bool = True) -> IsectState:
+    """First half of isect_tiles: tile counts, their prefix, n_isects and the
+    depth range into ``state.info`` ([n_isects, packed range] int64, on the
+    device) and, for the depth-first sort, the depth order of the pairs.
+    Nothing waits for the host; several states can be read in one sync
+    before their isect_finish calls."""
     C, N, _ = means2d.shape
     assert means2d.shape == (C, N, 2), means2d.size()
     assert radii.shape == (C, N), radii.size()
@@ -307,50 +325,62 @@ def isect_tiles_offsets(means2d: Tensor, radii: Tensor, depths: Tensor, tile_siz
     dev = means2d.device
     L = lib()
     st = stream_of(means2d)
-    tpg = torch.empty((C, N), dtype=torch.int32, device=dev)
-    cum = torch.empty((C * N,), dtype=torch.int64, device=dev)
-    # [n_isects, (~min depth bits, max depth bits) as two uint32] read in one sync
-    info = torch.zeros((2,), dtype=torch.int64, device=dev)
+    S = IsectState()
+    S.means2d, S.radii, S.depths, S.C, S.N = means2d, radii, depths, C, N
+    S.tile_size, S.tile_width, S.tile_height, S.sort = tile_size, tile_width, tile_height, sort
+    S.tpg = torch.empty((C, N), dtype=torch.int32, device=dev)
+    S.cum = torch.empty((C * N,), dtype=torch.int64, device=dev)
+    # [n_isects, (~min depth bits, max depth bits) as two uint32], read in one sync
+    S.info = torch.zeros((2,), dtype=torch.int64, device=dev)
     ws_bytes = int(L.ubs_isect_count_workspace_bytes(C, N))
     ws = torch.empty((ws_bytes,), dtype=torch.uint8, device=dev)
     _call("ubs_isect_tiles_count_range", C, N, ptr(means2d), ptr(radii), ptr(depths), tile_size, tile_width,
-          tile_height, ptr(tpg), ptr(cum), ptr(info), _vp_at(info, 1) if sort else None, ptr(ws), ws_bytes, st)
-    depth_first = sort and _sort_mode == "depth_first" and C * N > 0
-    if depth_first:
-        # queued before the host read below, so it overlaps the round trip
+          tile_height, ptr(S.tpg), ptr(S.cum), ptr(S.info), _vp_at(S.info, 1) if sort else None, ptr(ws), ws_bytes,
+          st)
+    S.order = None
+    if sort and _sort_mode == "depth_first" and C * N > 0:
+        # queued before the host read of n_isects, so it overlaps the round trip
         dkeys = torch.empty((C * N,), dtype=torch.int64, device=dev)
         dvals = torch.empty((C * N,), dtype=torch.int32, device=dev)
         _call("ubs_isect_depth_keys", C, N, ptr(radii), ptr(depths), ptr(dkeys), ptr(dvals), st)
-        _, order = radix_sort_pairs(dkeys, dvals, 0, 32)
-        cum = torch.empty((C * N,), dtype=torch.int64, device=dev)
+        _, S.order = radix_sort_pairs(dkeys, dvals, 0, 32)
+        S.cum = torch.empty((C * N,), dtype=torch.int64, device=dev)
         ows = int(L.ubs_isect_ordered_workspace_bytes(C, N))
         ow = torch.empty((ows,), dtype=torch.uint8, device=dev)
-        _call("ubs_isect_scan_ordered", C, N, ptr(tpg), ptr(order), ptr(cum), ptr(ow), ows, st)
-    n_isects, packed = (int(x) for x in info.cpu())  # host sync, as isect_tiles.cu:181
+        _call("ubs_isect_scan_ordered", C, N, ptr(S.tpg), ptr(S.order), ptr(S.cum), ptr(ow), ows, st)
+    return S
+
+
+def isect_finish(S: IsectState, n_isects: int, packed: int):
+    """Second half of isect_tiles, given the state's host-read info:
+    (tiles_per_gauss, isect_ids, flatten_ids, isect_offsets or None)."""
+    L = lib()
+    C, dev, st = S.C, S.means2d.device, stream_of(S.means2d)
+    tw, th, ts = S.tile_width, S.tile_height, S.tile_size
     isect_ids = torch.empty((n_isects,), dtype=torch.int64, device=dev)
     flatten_ids = torch.empty((n_isects,), dtype=torch.int32, device=dev)
-    if n_isects and depth_first:
-        _call("ubs_isect_tiles_emit_ordered", C, N, ptr(means2d), ptr(radii), ptr(depths), ptr(order), ptr(cum),
-              tile_size, tile_width, tile_height, ptr(isect_ids), ptr(flatten_ids), st)
+    if n_isects and S.order is not None:
+        _call("ubs_isect_tiles_emit_ordered", C, S.N, ptr(S.means2d), ptr(S.radii), ptr(S.depths), ptr(S.order),
+              ptr(S.cum), ts, tw, th, ptr(isect_ids), ptr(flatten_ids), st)
     elif n_isects:
-        _call("ubs_isect_tiles_emit", C, N, ptr(means2d), ptr(radii), ptr(depths), ptr(cum), tile_size,
-              tile_width, tile_height, ptr(isect_ids), ptr(flatten_ids), st)
+        _call("ubs_isect_tiles_emit", C, S.N, ptr(S.means2d), ptr(S.radii), ptr(S.depths), ptr(S.cum), ts, tw, th,
+              ptr(isect_ids), ptr(flatten_ids), st)
     offsets = None
-    if n_isects and sort:
-        end_bit = 32 + int(L.ubs_isect_prefix_bits(C, tile_width, tile_height))
-        if depth_first:  # entries already in depth order: stable on the camera/tile bits finishes it
+    if n_isects and S.sort:
+        end_bit = 32 + int(L.ubs_isect_prefix_bits(C, tw, th))
+        if S.order is not None:  # entries already in depth order: stable on the camera/tile bits finishes it
             isect_ids, flatten_ids = radix_sort_pairs(isect_ids, flatten_ids, 32, end_bit)
         elif _sort_mode == "segmented":
             # stable on the camera/tile bits, then per tile on the depth bits
             ids1, flat1 = radix_sort_pairs(isect_ids, flatten_ids, 32, end_bit)
-            offsets = isect_offset_encode(ids1, C, tile_width, tile_height)
+            offsets = isect_offset_encode(ids1, C, tw, th)
             isect_ids, flatten_ids = torch.empty_like(ids1), torch.empty_like(flat1)
             _call("ubs_sort_tile_segments", n_isects, ptr(ids1), ptr(flat1), ptr(isect_ids), ptr(flatten_ids),
-                  ptr(offsets), C, tile_width, tile_height, constant_depth_passes(packed), st)
+                  ptr(offsets), C, tw, th, constant_depth_passes(packed), st)
         else:
             isect_ids, flatten_ids = radix_sort_pairs(isect_ids, flatten_ids, 0, end_bit,
                                                       skip_passes=constant_depth_passes(packed))
-    return tpg, isect_ids, flatten_ids, offsets
+    return S.tpg, isect_ids, flatten_ids, offsets
 
 
 def constant_depth_passes(packed_range: int) -> int:
