@@ -191,10 +191,11 @@ class RenderServer:
         params = self.store.scenes[batch[0].scene]
         s = self.streams[slot]
         dev = self.store.device
+        caller = torch.cuda.current_stream(dev) if s is not None else None
         ctx = torch.cuda.stream(s) if s is not None else _nullctx()
         with torch.no_grad(), ctx:
             if s is not None:  # inputs made on the caller's stream are ready before this batch reads them
-                s.wait_stream(torch.cuda.current_stream(dev))
+                s.wait_stream(caller)
             vms = torch.stack([r.viewmat.to(dev, torch.float32) for r in batch]).contiguous()
             Ks = torch.stack([r.K.to(dev, torch.float32) for r in batch]).contiguous()
             ts = None
