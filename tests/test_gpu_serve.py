@@ -81,3 +81,60 @@ def test_calibrate_measures_isects(store):
     vm = torch.tensor(scenes.look_at_viewmat(np.array([0.0, 0.5, 4.0])), dtype=torch.float32)
     k = store.calibrate("s6", vm, torch.tensor(scenes.intrinsics(1920, 1080), dtype=torch.float32))
     assert 1.0 <= k < 200.0 and store.isects_per_prim["s6"] == k
+
+
+def test_full_size_mix_matches_alone_and_fp64(device):
+    """Serving at full size: two resident 2M-primitive scenes (6-D and 7-D),
+    requests at 640x480 / 1280x720 / 1920x1080: every image is bit-identical
+    alone and in the mixed batches, and a 1080p view matches the FP64 oracle
+    to 1e-4 max-abs on 64 random tiles (pixels away from the T cut-off),
+    blending the served view's own tile lists."""
+    from oracle import ubs_oracle as O
+
+    st = SceneStore(device)
+    scs = {}
+    for name, D, seed in (("big6", 6, 40), ("big7", 7, 41)):
+        scs[name] = scenes.make_scene(2_000_000, D, seed=seed)
+        st.add(name, BetaParams.from_scene(scs[name], device, requires_grad=False, use_quats=True))
+    rng = np.random.Generator(np.random.PCG64(9))
+    sizes = [(640, 480), (1280, 720), (1920, 1080)]
+    reqs = []
+    for i in range(6):
+        W, H = sizes[i % 3]
+        d = rng.normal(0, 1, 3)
+        vm = scenes.look_at_viewmat(rng.uniform(3, 5) * d / np.linalg.norm(d))
+        reqs.append(ViewRequest(["big6", "big7"][i % 2], torch.tensor(vm, dtype=torch.float32),
+                                torch.tensor(scenes.intrinsics(W, H), dtype=torch.float32), W, H,
+                                t=float(rng.uniform(0, 1)), rid=i))
+    print("full-size serving: 2 x 2M primitives, 6 requests", flush=True)
+    mixed, srv = _serve(st, reqs, max_views=16, streams=2)
+    alone, _ = _serve(st, reqs, max_views=1, streams=1)
+    assert srv.batches == 2  # one batch per scene
+    for r in reqs:
+        assert torch.equal(mixed[r.rid].image, alone[r.rid].image), r.rid
+    # FP64 oracle on 64 tiles of the 1080p 7-D view (rid 5), in the served view's tile lists
+    r = reqs[5]
+    p = st.scenes[r.scene]
+    ts = torch.tensor([r.t], dtype=torch.float32, device=device)
+    with torch.no_grad():
+        img, _, meta = render_views(p, r.viewmat.to(device)[None].contiguous(), r.K.to(device)[None].contiguous(),
+                                    r.width, r.height, timestamps=ts)
+    assert torch.equal(img[0], mixed[r.rid].image)
+    sc = scs[r.scene]
+    tw, th = -(-r.width // 16), -(-r.height // 16)
+    tm = np.zeros(th * tw, bool)
+    tm[np.random.Generator(np.random.PCG64(1)).choice(th * tw, 64, replace=False)] = True
+    tm = tm.reshape(th, tw)
+    struct = tuple(meta[k].cpu().numpy() for k in ("tiles_per_gauss", "isect_ids", "flatten_ids", "isect_offsets"))
+    leaves = {k: getattr(sc, k) for k in ("xyz", "mean_rest", "scale_raw", "l_triangle", "opacity_raw", "beta_raw",
+                                         "rgb", "quats")}
+    ref, _, rm = O.beta_step(leaves, r.viewmat.numpy(), r.K.numpy(), r.width, r.height,
+                             np.zeros((r.height, r.width, 3)), timestamp=r.t, dtype=np.float64, structure=struct,
+                             want_grads=False, tile_mask=tm)
+    _, _, _, mT, mS = O.raster_fwd(rm["means2d"], rm["conics"], rm["colors"], rm["opacities"], rm["betas"], r.width,
+                                   r.height, 16, rm["isect_offsets"], rm["flatten_ids"], masks=tm[None], margins=True)
+    pm = np.kron(tm, np.ones((16, 16), bool))[:r.height, :r.width]
+    safe = pm & (mT[0] > 1e-3) & (mS[0] > 1e-6)
+    assert safe.sum() > 0.9 * pm.sum()
+    err = np.abs(mixed[r.rid].image.cpu().numpy() - ref)[safe].max()
+    assert err <= 1e-4, err
