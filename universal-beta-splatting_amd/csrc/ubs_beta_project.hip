@@ -137,8 +137,19 @@ __global__ __launch_bounds__(kThreads) void beta_project_fwd_kernel(
     }
 }
 
+// Minimum waves per SIMD of the 7-D backward.  Unbounded it takes 317
+// registers (one wave per SIMD); held to two waves it spills 55 values to
+// scratch and still runs faster (0.262 vs 0.291 ms at 1M primitives,
+// 1352x1014).  The 6-D kernel fits two waves by itself (236 registers).
+#ifndef UBS_PROJ_BWD_WAVES_7
+#define UBS_PROJ_BWD_WAVES_7 2
+#endif
+#ifndef UBS_PROJ_BWD_WAVES_6
+#define UBS_PROJ_BWD_WAVES_6 1
+#endif
+
 template <int D, bool QUAT>
-__global__ __launch_bounds__(kThreads) void beta_project_bwd_kernel(
+__global__ __launch_bounds__(kThreads, D == 7 ? UBS_PROJ_BWD_WAVES_7 : UBS_PROJ_BWD_WAVES_6) void beta_project_bwd_kernel(
     int64_t N, uint32_t C, const float *__restrict__ xyz, const float *__restrict__ mean_rest,
     const float *__restrict__ scale_raw, const float *__restrict__ ltri, const float *__restrict__ opacity_raw,
     const float *__restrict__ beta_raw, const float *__restrict__ quats, const float *__restrict__ cam_pos,
