@@ -257,6 +257,22 @@ def test_radix_sort_skips_constant_passes(device, n, const_bytes):
     assert torch.equal(got_k, full_k) and torch.equal(got_v, full_v)
 
 
+@pytest.mark.parametrize("end_bit,const_byte", [(46, None), (33, 3), (64, 2)])
+def test_radix_sort_large_tiles(device, end_bit, const_byte):
+    """Sorts of 4M keys and more run with the larger tiles; passes over a
+    digit every key shares take the copy-through path on the device."""
+    n = (1 << 22) + 54321
+    g = np.random.Generator(np.random.PCG64(end_bit))
+    keys = g.integers(-(1 << 62), 1 << 62, size=n // 5, dtype=np.int64)[g.integers(0, n // 5, n)]
+    if const_byte is not None:
+        keys = (keys & ~np.int64(0xFF << (8 * const_byte))) | np.int64(0x3C << (8 * const_byte))
+    vals = np.arange(n, dtype=np.int32)
+    want_k, want_v = O.sort_pairs(keys, vals, 0, end_bit)
+    got_k, got_v = W.radix_sort_pairs(cu(keys), cu(vals), 0, end_bit, check_status=True)
+    assert_same(host(got_k), want_k, "keys")
+    assert_same(host(got_v), want_v, "values")
+
+
 def test_isect_depth_range(device):
     g = np.random.Generator(np.random.PCG64(3))
     C, N = 2, 5000

@@ -32,8 +32,21 @@ constexpr int kSortWaves = kSortThreads / 64;
 #ifndef UBS_SORT_ITEMS
 #define UBS_SORT_ITEMS 24
 #endif
+// A sort of fewer keys than kSmallSortN (the C*N depth keys of a view) takes
+// fewer keys per thread: at 24 a 1M-key sort has 163 tiles, under one per
+// CU, and each pass waits on one workgroup's serial ranking (12 measured
+// best of 4 / 8 / 12 at 1M keys, by a few microseconds per pass).
+#ifndef UBS_SORT_ITEMS_SMALL
+#define UBS_SORT_ITEMS_SMALL 12
+#endif
+#ifndef UBS_SORT_SMALL_N
+#define UBS_SORT_SMALL_N (1 << 22)
+#endif
 constexpr int kItems = UBS_SORT_ITEMS;
-constexpr int kTile = kSortThreads * kItems;  // 6144 keys per workgroup
+constexpr int kItemsSmall = UBS_SORT_ITEMS_SMALL;
+constexpr int64_t kSmallSortN = UBS_SORT_SMALL_N;
+
+inline int sort_tile_keys(int64_t n) { return kSortThreads * (n < kSmallSortN ? kItemsSmall : kItems); }
 constexpr uint32_t kFlagAgg = 1u << 30;
 constexpr uint32_t kFlagPrefix = 2u << 30;
 constexpr uint32_t kValueMask = (1u << 30) - 1u;
@@ -49,7 +62,8 @@ struct SortLayout {
 static SortLayout sort_layout(int64_t n, int begin_bit, int end_bit) {
     SortLayout L{};
     L.passes = end_bit > begin_bit ? (end_bit - begin_bit + kRadixBits - 1) / kRadixBits : 0;
-    L.tiles = (n + kTile - 1) / kTile;
+    const int tile_keys = sort_tile_keys(n);
+    L.tiles = (n + tile_keys - 1) / tile_keys;
     size_t off = 0;
     L.status_off = off;
     off += 256;
@@ -113,6 +127,7 @@ __global__ void sort_hist_scan_kernel(uint32_t *__restrict__ hist) {
     h[d] = acc;
 }
 
+template <int ITEMS>
 __global__ __launch_bounds__(kSortThreads) void sort_pass_kernel(
     int64_t n, const int64_t *__restrict__ keys_in, int64_t *__restrict__ keys_out,
     const int32_t *__restrict__ vals_in, int32_t *__restrict__ vals_out, int shift, int bits,
@@ -122,8 +137,9 @@ __global__ __launch_bounds__(kSortThreads) void sort_pass_kernel(
     __shared__ uint32_t s_wave_cnt[kSortWaves][kRadix];
     __shared__ uint32_t s_digit_start[kRadix];
     __shared__ uint32_t s_out_base[kRadix];
-    __shared__ int64_t s_keys[kTile];
-    __shared__ int32_t s_vals[kTile];
+    constexpr int TILE = kSortThreads * ITEMS;
+    __shared__ int64_t s_keys[TILE];
+    __shared__ int32_t s_vals[TILE];
 
     const int tid = threadIdx.x, wave = tid >> 6, lane = (int)lane_id();
     const uint32_t mask = (1u << bits) - 1u;
@@ -131,15 +147,30 @@ __global__ __launch_bounds__(kSortThreads) void sort_pass_kernel(
     for (int k = tid; k < kSortWaves * kRadix; k += kSortThreads) (&s_wave_cnt[0][0])[k] = 0;
     __syncthreads();
     const int64_t tile = s_tile;
-    const int64_t base = tile * kTile;
+    const int64_t base = tile * TILE;
+
+    // a pass whose digit every key shares (a constant exponent byte of the
+    // depths) is the identity of a stable sort: the tile is copied through
+    // (the exclusive digit offsets show it: one digit spans all n keys)
+    {
+        const uint32_t next = tid + 1 < kRadix ? digit_base[tid + 1] : (uint32_t)n;
+        if (__syncthreads_or(next - digit_base[tid] == (uint32_t)n)) {
+            const int64_t end = min(base + (int64_t)TILE, n);
+            for (int64_t j = base + tid; j < end; j += kSortThreads) {
+                keys_out[j] = keys_in[j];
+                vals_out[j] = vals_in[j];
+            }
+            return;
+        }
+    }
 
     // ---- load (striped inside the wave: item i of lane l is wave_base + i*64 + l)
-    int64_t key[kItems];
-    int32_t val[kItems];
-    uint32_t dig[kItems], rank[kItems];
-    const int64_t wave_base = base + (int64_t)wave * 64 * kItems;
+    int64_t key[ITEMS];
+    int32_t val[ITEMS];
+    uint32_t dig[ITEMS], rank[ITEMS];
+    const int64_t wave_base = base + (int64_t)wave * 64 * ITEMS;
 #pragma unroll
-    for (int i = 0; i < kItems; ++i) {
+    for (int i = 0; i < ITEMS; ++i) {
         const int64_t idx = wave_base + i * 64 + lane;
         const bool ok = idx < n;
         key[i] = ok ? keys_in[idx] : 0;
@@ -151,7 +182,7 @@ __global__ __launch_bounds__(kSortThreads) void sort_pass_kernel(
     volatile uint32_t *cnt = s_wave_cnt[wave];
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 #pragma unroll
-    for (int i = 0; i < kItems; ++i) {
+    for (int i = 0; i < ITEMS; ++i) {
         const int64_t idx = wave_base + i * 64 + lane;
         const bool ok = idx < n;
         uint64_t peers = __ballot(ok);
@@ -211,7 +242,7 @@ __global__ __launch_bounds__(kSortThreads) void sort_pass_kernel(
 
     // ---- stage the tile in LDS in digit order
 #pragma unroll
-    for (int i = 0; i < kItems; ++i) {
+    for (int i = 0; i < ITEMS; ++i) {
         const int64_t idx = wave_base + i * 64 + lane;
         if (idx < n) {
             const uint32_t pos = s_digit_start[dig[i]] + s_wave_cnt[wave][dig[i]] + rank[i];
@@ -270,7 +301,7 @@ __global__ __launch_bounds__(kSortThreads) void sort_pass_kernel(
 
     // ---- write out: consecutive LDS slots of one digit go to consecutive
     // global slots
-    const int64_t count = min((int64_t)kTile, n - base);
+    const int64_t count = min((int64_t)TILE, n - base);
     for (int j = tid; j < count; j += kSortThreads) {
         const int64_t k = s_keys[j];
         const uint32_t pos = s_out_base[digit_of(k, shift, mask)] + (uint32_t)j;
@@ -336,7 +367,8 @@ int ubs_radix_sort_pairs_skip(int64_t n, int64_t *keys, int64_t *keys_alt, int32
         if ((skip_passes >> p) & 1u) continue;
         const int shift = begin_bit + p * kRadixBits;
         const int bits = std::min(kRadixBits, end_bit - shift);
-        sort_pass_kernel<<<(unsigned)L.tiles, kSortThreads, 0, st>>>(
+        auto *pass = n < kSmallSortN ? sort_pass_kernel<kItemsSmall> : sort_pass_kernel<kItems>;
+        pass<<<(unsigned)L.tiles, kSortThreads, 0, st>>>(
             n, kb[run & 1], kb[(run + 1) & 1], vb[run & 1], vb[(run + 1) & 1], shift, bits,
             hist + (size_t)p * kRadix, counters + p, lookback + (size_t)p * L.tiles * kRadix, status);
         UBS_LAUNCH_CHECK("sort_pass_kernel");
