@@ -182,6 +182,129 @@ __device__ __forceinline__ void wave_totals(const float (&v)[NV], float (&out)[W
     }
 }
 
+// ---- row stage by reduce-scatter -------------------------------------------
+// After the two permlane levels every 16-lane row (quarter) holds, per
+// group register, partial sums of one value that still need a 16-lane sum.
+// Instead of four DPP adds per register, the rows are reduce-scattered too:
+// v_add_f32_dpp with a bank mask (banks = 4-lane groups of a row) writes
+// only the lanes of one half or one bank pair, the others keep the tied old
+// operand, so two instructions combine two registers into one at each of
+// the 8- and 4-lane levels, and two quad steps finish every bank.  After
+// row_totals, bank b of register k holds the total of group
+// row_group<G>(k, b) (-1: a copy of another bank's total).
+// Each DPP instruction is preceded by s_nop 1: a DPP read of a VGPR needs
+// two wait states after the VALU write, which the compiler does not insert
+// for inline assembly.
+// row_ror:N makes lane l of a row read lane (l - N) mod 16.
+#define UBS_ADD_DPP(R, OLD, SRC, OTHER, CTRL, BANKS)                                                          \
+    asm("s_nop 1\n\tv_add_f32_dpp %0, %1, %2 " CTRL " row_mask:0xf bank_mask:" BANKS                          \
+        : "=v"(R) : "v"(SRC), "v"(OTHER), "0"(OLD))
+// lanes 0-7 of each row: src[l + 8] + other; lanes 8-15: old
+__device__ __forceinline__ float add_dpp_lo8(float old, float src, float other) {
+    float r;
+    UBS_ADD_DPP(r, old, src, other, "row_ror:8", "0x3");
+    return r;
+}
+// lanes 8-15: src[l - 8] + other; lanes 0-7: old
+__device__ __forceinline__ float add_dpp_hi8(float old, float src, float other) {
+    float r;
+    UBS_ADD_DPP(r, old, src, other, "row_ror:8", "0xc");
+    return r;
+}
+// banks 0 and 2 (lanes 0-3, 8-11): src[l + 4] + other; banks 1 and 3: old
+__device__ __forceinline__ float add_dpp_even4(float old, float src, float other) {
+    float r;
+    UBS_ADD_DPP(r, old, src, other, "row_ror:12", "0x5");
+    return r;
+}
+// banks 1 and 3 (lanes 4-7, 12-15): src[l - 4] + other; banks 0 and 2: old
+__device__ __forceinline__ float add_dpp_odd4(float old, float src, float other) {
+    float r;
+    UBS_ADD_DPP(r, old, src, other, "row_ror:4", "0xa");
+    return r;
+}
+// every lane: x[l + 8 mod 16] + x[l]
+__device__ __forceinline__ float add_dpp_all8(float x) {
+    float r;
+    UBS_ADD_DPP(r, x, x, x, "row_ror:8", "0xf");
+    return r;
+}
+// every lane: x[7 - l] + x[l] inside each 8-lane half of a row (a quad sum
+// after it gives the half's total; a row rotate would mix the halves)
+__device__ __forceinline__ float add_dpp_half_mirror(float x) {
+    float r;
+    UBS_ADD_DPP(r, x, x, x, "row_half_mirror", "0xf");
+    return r;
+}
+// every lane: the sum over its quad
+__device__ __forceinline__ float add_dpp_quad(float x) {
+    float a, b;
+    UBS_ADD_DPP(a, x, x, x, "quad_perm:[1,0,3,2]", "0xf");
+    UBS_ADD_DPP(b, a, a, a, "quad_perm:[2,3,0,1]", "0xf");
+    return b;
+}
+#undef UBS_ADD_DPP
+
+template <int G>
+struct RowTotals {
+    static constexpr int kH = (G + 1) / 2;   // registers after the 8-lane level
+    static constexpr int kRegs = (kH + 1) / 2;  // after the 4-lane level
+};
+
+// group whose total bank b (lanes 4b..4b+3 of a row) of register k holds
+// after row_totals: banks 0/2 come from the even 8-lane register, 1/3 from
+// the odd one; banks 0/1 from its lower half, 2/3 from its upper half
+template <int G>
+__host__ __device__ constexpr int row_group(int k, int b) {
+    constexpr int H = RowTotals<G>::kH;
+    const int h = (b & 1) ? 2 * k + 1 : 2 * k;
+    if (h >= H) return -1;
+    const int g = (b & 2) ? 2 * h + 1 : 2 * h;
+    return g < G ? g : -1;
+}
+
+template <int NV>
+__device__ __forceinline__ void wave_totals_rows(const float (&v)[NV],
+                                                 float (&out)[RowTotals<WaveTotals<NV>::kGroups>::kRegs]) {
+    constexpr int F = WaveTotals<NV>::kFull, T = WaveTotals<NV>::kTail;
+    constexpr int G = WaveTotals<NV>::kGroups;
+    constexpr int H = RowTotals<G>::kH, K = RowTotals<G>::kRegs;
+    float g[G];
+#pragma unroll
+    for (int r = 0; r < F; ++r)
+        g[r] = swap16_sum(swap32_sum(v[4 * r], v[4 * r + 1]), swap32_sum(v[4 * r + 2], v[4 * r + 3]));
+    if constexpr (T == 3) {
+        g[F] = swap16_sum(swap32_sum(v[4 * F], v[4 * F + 1]), swap32_sum(v[4 * F + 2], 0.f));
+    } else if constexpr (T == 2) {
+        const float u = swap32_sum(v[4 * F], v[4 * F + 1]);
+        g[F] = swap16_sum(u, u);
+    } else if constexpr (T == 1) {
+        const float u = swap32_sum(v[4 * F], v[4 * F]);
+        g[F] = swap16_sum(u, u);
+    }
+    float h[H];
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+        if (2 * i + 1 < G) {
+            const float lo = add_dpp_lo8(g[2 * i + 1], g[2 * i], g[2 * i]);
+            h[i] = add_dpp_hi8(lo, g[2 * i + 1], lo);
+        } else {
+            h[i] = add_dpp_all8(g[2 * i]);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        float x;
+        if (2 * k + 1 < H) {
+            const float ev = add_dpp_even4(h[2 * k + 1], h[2 * k], h[2 * k]);
+            x = add_dpp_odd4(ev, h[2 * k + 1], ev);
+        } else {
+            x = add_dpp_half_mirror(h[2 * k]);
+        }
+        out[k] = add_dpp_quad(x);
+    }
+}
+
 __device__ __forceinline__ int wave_max_i32(int v) {
     // butterfly through ds_swizzle-free shuffles (used once per tile)
     for (int o = 32; o > 0; o >>= 1) {

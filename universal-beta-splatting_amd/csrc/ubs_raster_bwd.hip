@@ -10,10 +10,11 @@
 //     function, so the divisors are bit-identical;
 //   * forms the gradients of rasterize_to_pixels_bwd.cu:200-247, branch-free
 //     (lanes without a contribution compute exact zeros);
-//   * sums them over the 64 lanes with a permlane reduce-scatter that leaves
-//     each 16-lane quarter with different totals (wave_totals), and one lane
-//     per quarter adds them into the primitive's LDS accumulator
-//     (ds_add_f32), so the four waves of a tile share one set of partial sums
+//   * sums them over the 64 lanes with a reduce-scatter (permlane swaps
+//     across the 16-lane rows, bank-masked DPP adds inside them) that leaves
+//     each 4-lane bank with different totals (wave_totals_rows), and one lane
+//     per total adds it into the primitive's LDS accumulator (ds_add_f32),
+//     so the four waves of a tile share one set of partial sums
 //     (finishing with half-row sums and two adding lanes per total measured
 //     slower: 3.23 vs 3.09 ms, the same-address LDS adds cost more than the
 //     DPP steps they save);
@@ -107,17 +108,21 @@ __global__ __launch_bounds__(TS *TS, UBS_BWD_WAVES_PER_EU) void raster_bwd_kerne
     for (int k = tid; k < NV * (BS + 1); k += BS) (&s_acc[0][0])[k] = 0.f;
     __syncthreads();
     const int32_t wave_bin_max = s_wmax[wave];
-    // the value each quarter leader (lanes 0, 16, 32, 48) adds per group
-    // after wave_totals every lane of quarter Q holds the totals of all groups;
-    // lane 16 Q + r adds group 16 p + r's in pass p, so ceil(NG2 / 16)
-    // ds_add_f32 (one for up to 14 colour channels) store the 2 NV totals of a
+    // the value each lane adds: after wave_totals_rows, bank b of row Q holds
+    // in register k the total of value wave_total_index(row_group(k, b), Q);
+    // lane 16 Q + 4 b + j adds register 4 p + j in pass p, so ceil(K / 4)
+    // ds_add_f32 (one for up to 5 colour channels) store the 2 NV totals of a
     // pair of primitives
-    constexpr int NPASS = (NG2 + 15) / 16;
-    const int my_r = lane & 15;
+    constexpr int NREG = RowTotals<NG2>::kRegs;
+    constexpr int NPASS = (NREG + 3) / 4;
+    const int my_j = lane & 3, my_bank = (lane >> 2) & 3;
     int my_idx[NPASS];
 #pragma unroll
-    for (int p = 0; p < NPASS; ++p)
-        my_idx[p] = 16 * p + my_r < NG2 ? wave_total_index<2 * NV>(16 * p + my_r, lane >> 4) : -1;
+    for (int p = 0; p < NPASS; ++p) {
+        const int k = 4 * p + my_j;
+        const int grp = k < NREG ? row_group<NG2>(k, my_bank) : -1;
+        my_idx[p] = grp >= 0 ? wave_total_index<2 * NV>(grp, lane >> 4) : -1;
+    }
 
     // batches are the forward's, [range_start + k BS, ...), taken last first
     const int32_t nb = range_end > range_start ? (range_end - range_start + BS - 1) / BS : 0;
@@ -282,15 +287,15 @@ __global__ __launch_bounds__(TS *TS, UBS_BWD_WAVES_PER_EU) void raster_bwd_kerne
                 const bool c1v = pixel_grads(r1, t1, v, NV, two);
                 const bool any = __ballot(c0v || c1v) != 0ull;
                 if (!any) continue;
-                // wave totals, scattered over the four 16-lane quarters; one
-                // ds_add_f32 per pass adds them into the primitives' accumulators
-                float tot[NG2];
-                wave_totals<2 * NV>(v, tot);
+                // wave totals, scattered over the rows and banks; one ds_add_f32
+                // per pass adds them into the primitives' accumulators
+                float tot[NREG];
+                wave_totals_rows<2 * NV>(v, tot);
 #pragma unroll
                 for (int p = 0; p < NPASS; ++p) {
-                    float mine = tot[16 * p];
+                    float mine = tot[4 * p];
 #pragma unroll
-                    for (int r = 16 * p + 1; r < NG2 && r < 16 * p + 16; ++r) mine = my_r == r - 16 * p ? tot[r] : mine;
+                    for (int k = 4 * p + 1; k < NREG && k < 4 * p + 4; ++k) mine = my_j == k - 4 * p ? tot[k] : mine;
                     pend_val[p] = mine;
                 }
                 pend_t0 = t0;
