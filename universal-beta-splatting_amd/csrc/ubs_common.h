@@ -107,14 +107,6 @@ __device__ __forceinline__ float wave_sum(float v) {
     return (r0 + r1) + (r2 + r3);
 }
 
-__device__ __forceinline__ float dpp_row_sum(float v) {
-    v += __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0xB1, 0xF, 0xF, true));
-    v += __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x4E, 0xF, 0xF, true));
-    v += __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x124, 0xF, 0xF, true));
-    v += __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x128, 0xF, 0xF, true));
-    return v;
-}
-
 // Pairwise reduce-scatter of two per-lane values across the wave halves
 // (v_permlane32_swap): lanes 0-31 get a[l] + a[l+32], lanes 32-63 get
 // b[l-32] + b[l].
@@ -135,23 +127,19 @@ __device__ __forceinline__ float swap16_sum(float a, float b) {
     return __uint_as_float(lo) + __uint_as_float(hi);
 }
 
-// Wave totals of NV per-lane values at ~2.5 VALU ops per value instead of
-// ~11 for a butterfly per value: two reduce-scatter levels (permlane swaps)
-// leave each 16-lane quarter with partial sums of a different value, and DPP
-// row sums finish them.  Values go in groups of four; after the call every
-// lane of quarter Q (lanes 16Q..16Q+15) holds in out[r] the wave total of
-// value wave_total_index<NV>(r, Q), or of nothing when that returns -1.
-// A trailing group of one or two values costs one group-level instead of a
-// padded four.  Must be called with all 64 lanes active.
+// Wave totals of NV per-lane values (wave_totals_rows below) at ~2.4 VALU
+// ops per value instead of ~11 for a butterfly per value: two reduce-scatter
+// levels (permlane swaps) leave each 16-lane quarter Q with partial sums of
+// value wave_total_index<NV>(r, Q) in group register r (or of nothing when
+// that returns -1); values go in groups of four, and a trailing group of one
+// or two values costs one group-level instead of a padded four.  Must be
+// called with all 64 lanes active.
 template <int NV>
 struct WaveTotals {
     static constexpr int kFull = NV / 4;
     static constexpr int kTail = NV % 4;
     static constexpr int kGroups = kFull + (kTail ? 1 : 0);
 };
-
-// Quarter Q of a full group holds value 4r + {0, 2, 1, 3}[Q] (bit-reversed Q).
-__device__ __forceinline__ int quarter_value(int q) { return ((q & 1) << 1) | (q >> 1); }
 
 template <int NV>
 __host__ __device__ constexpr int wave_total_index(int r, int q) {
@@ -162,24 +150,6 @@ __host__ __device__ constexpr int wave_total_index(int r, int q) {
     }
     if (T == 1) return q == 0 ? 4 * r : -1;
     return q == 0 ? 4 * r : (q == 2 ? 4 * r + 1 : -1);  // T == 2
-}
-
-template <int NV>
-__device__ __forceinline__ void wave_totals(const float (&v)[NV], float (&out)[WaveTotals<NV>::kGroups]) {
-    constexpr int F = WaveTotals<NV>::kFull, T = WaveTotals<NV>::kTail;
-#pragma unroll
-    for (int r = 0; r < F; ++r)
-        out[r] = dpp_row_sum(swap16_sum(swap32_sum(v[4 * r], v[4 * r + 1]), swap32_sum(v[4 * r + 2], v[4 * r + 3])));
-    if constexpr (T == 3) {
-        out[F] = dpp_row_sum(swap16_sum(swap32_sum(v[4 * F], v[4 * F + 1]), swap32_sum(v[4 * F + 2], 0.f)));
-    } else if constexpr (T == 2) {
-        // halves hold the two values; the self-swap sums each half's rows
-        const float u = swap32_sum(v[4 * F], v[4 * F + 1]);
-        out[F] = dpp_row_sum(swap16_sum(u, u));
-    } else if constexpr (T == 1) {
-        const float u = swap32_sum(v[4 * F], v[4 * F]);
-        out[F] = dpp_row_sum(swap16_sum(u, u));
-    }
 }
 
 // ---- row stage by reduce-scatter -------------------------------------------
