@@ -156,9 +156,10 @@ __host__ __device__ constexpr int wave_total_index(int r, int q) {
 // After the two permlane levels every 16-lane row (quarter) holds, per
 // group register, partial sums of one value that still need a 16-lane sum.
 // Instead of four DPP adds per register, the rows are reduce-scattered too:
-// v_add_f32_dpp with a bank mask (banks = 4-lane groups of a row) writes
-// only the lanes of one half or one bank pair, the others keep the tied old
-// operand, so two instructions combine two registers into one at each of
+// an all-lane DPP add of one register, then a DPP add of the other with a
+// bank mask (banks = 4-lane groups of a row) that rewrites only its half or
+// bank pair while the other lanes keep the first result (the tied old
+// operand), so two instructions combine two registers into one at each of
 // the 8- and 4-lane levels, and two quad steps finish every bank.  After
 // row_totals, bank b of register k holds the total of group
 // row_group<G>(k, b) (-1: a copy of another bank's total).
@@ -169,22 +170,10 @@ __host__ __device__ constexpr int wave_total_index(int r, int q) {
 #define UBS_ADD_DPP(R, OLD, SRC, OTHER, CTRL, BANKS)                                                          \
     asm("s_nop 1\n\tv_add_f32_dpp %0, %1, %2 " CTRL " row_mask:0xf bank_mask:" BANKS                          \
         : "=v"(R) : "v"(SRC), "v"(OTHER), "0"(OLD))
-// lanes 0-7 of each row: src[l + 8] + other; lanes 8-15: old
-__device__ __forceinline__ float add_dpp_lo8(float old, float src, float other) {
-    float r;
-    UBS_ADD_DPP(r, old, src, other, "row_ror:8", "0x3");
-    return r;
-}
 // lanes 8-15: src[l - 8] + other; lanes 0-7: old
 __device__ __forceinline__ float add_dpp_hi8(float old, float src, float other) {
     float r;
     UBS_ADD_DPP(r, old, src, other, "row_ror:8", "0xc");
-    return r;
-}
-// banks 0 and 2 (lanes 0-3, 8-11): src[l + 4] + other; banks 1 and 3: old
-__device__ __forceinline__ float add_dpp_even4(float old, float src, float other) {
-    float r;
-    UBS_ADD_DPP(r, old, src, other, "row_ror:12", "0x5");
     return r;
 }
 // banks 1 and 3 (lanes 4-7, 12-15): src[l - 4] + other; banks 0 and 2: old
@@ -197,6 +186,12 @@ __device__ __forceinline__ float add_dpp_odd4(float old, float src, float other)
 __device__ __forceinline__ float add_dpp_all8(float x) {
     float r;
     UBS_ADD_DPP(r, x, x, x, "row_ror:8", "0xf");
+    return r;
+}
+// every lane: x[l + 4 mod 16] + x[l] (lanes 0-3 and 8-11 stay in their half)
+__device__ __forceinline__ float add_dpp_all12(float x) {
+    float r;
+    UBS_ADD_DPP(r, x, x, x, "row_ror:12", "0xf");
     return r;
 }
 // every lane: x[7 - l] + x[l] inside each 8-lane half of a row (a quad sum
@@ -256,8 +251,10 @@ __device__ __forceinline__ void wave_totals_rows(const float (&v)[NV],
 #pragma unroll
     for (int i = 0; i < H; ++i) {
         if (2 * i + 1 < G) {
-            const float lo = add_dpp_lo8(g[2 * i + 1], g[2 * i], g[2 * i]);
-            h[i] = add_dpp_hi8(lo, g[2 * i + 1], lo);
+            // lanes 0-7 from an all-lane add of the even register, then 8-15
+            // overwritten from the odd one (the first result is the tied
+            // old operand, so no copy is needed)
+            h[i] = add_dpp_hi8(add_dpp_all8(g[2 * i]), g[2 * i + 1], g[2 * i + 1]);
         } else {
             h[i] = add_dpp_all8(g[2 * i]);
         }
@@ -266,8 +263,7 @@ __device__ __forceinline__ void wave_totals_rows(const float (&v)[NV],
     for (int k = 0; k < K; ++k) {
         float x;
         if (2 * k + 1 < H) {
-            const float ev = add_dpp_even4(h[2 * k + 1], h[2 * k], h[2 * k]);
-            x = add_dpp_odd4(ev, h[2 * k + 1], ev);
+            x = add_dpp_odd4(add_dpp_all12(h[2 * k]), h[2 * k + 1], h[2 * k + 1]);
         } else {
             x = add_dpp_half_mirror(h[2 * k]);
         }
