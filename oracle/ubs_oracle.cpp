@@ -634,14 +634,34 @@ void proj_fwd(int64_t C, int64_t N, const T *means, const T *covars, const T *qu
         if (ortho) { m2x = G.fx * G.pc[0] + G.cx; m2y = G.fy * G.pc[1] + G.cy; }
         else { m2x = G.fx * G.pc[0] * G.rz + G.cx; m2y = G.fy * G.pc[1] * G.rz + G.cy; }
         T c00 = G.cov2d[0][0], c01 = G.cov2d[0][1], c10 = G.cov2d[1][0], c11 = G.cov2d[1][1];
+        const T o00 = c00, o11 = c11;
         const T det_orig = c00 * c11 - c01 * c10;
         c00 += eps2d;
         c11 += eps2d;
-        const T det = c00 * c11 - c01 * c10;
-        const T cmp = std::sqrt(std::max(T(0), det_orig / det));
-        if (det <= T(0)) { radii[idx] = 0; continue; }
-        const T invd = T(1) / det;
-        const T i00 = c11 * invd, i01 = -c01 * invd, i11 = c00 * invd;
+        T det = c00 * c11 - c01 * c10;
+        T cmp, i00, i01, i11;
+        // The build's FP64 branch for a near-singular blurred 2-D covariance
+        // (det < 2^-8 c00 c11; the reference inverts in fp32 throughout,
+        // fully_fused_projection_fwd.cu:137-150, utils.cuh:438-448): the
+        // determinant of the same fp32 entries in FP64, then cull, inverse and
+        // compensation from it, each rounded once to T.
+        if (det < T(0.00390625f) * (c00 * c11)) {
+            const double d64 = (double)c00 * (double)c11 - (double)c01 * (double)c10;
+            if (!(d64 > 0.0)) { radii[idx] = 0; continue; }
+            const double o64 = (double)o00 * (double)o11 - (double)c01 * (double)c10;
+            cmp = std::sqrt(std::max(T(0), T(o64 / d64)));
+            i00 = T((double)c11 / d64);
+            i01 = T(-(double)c01 / d64);
+            i11 = T((double)c00 / d64);
+            det = T(d64);
+        } else {
+            cmp = std::sqrt(std::max(T(0), det_orig / det));
+            if (det <= T(0)) { radii[idx] = 0; continue; }
+            const T invd = T(1) / det;
+            i00 = c11 * invd;
+            i01 = -c01 * invd;
+            i11 = c00 * invd;
+        }
         const T b = T(0.5f) * (c00 + c11);
         const T v1 = b + std::sqrt(std::max(T(0.01f), b * b - det));
         const T radius = std::ceil(std::sqrt(v1));

@@ -369,3 +369,35 @@ def test_image_loss_gradient_finite_differences(lam):
     # L1 only: the mean of |x - y|, gradient sign / n
     l1, g1 = O.image_loss(x, y, 0.0)
     assert np.isclose(l1, np.abs(x - y).mean()) and np.allclose(g1, np.sign(x - y) / x.size)
+
+
+def test_projection_near_singular_inverse_is_fp64():
+    """Blurred 2-D covariances from near-singular to indefinite, with the exact
+    inverse known (util.near_singular_2d_cases): where det < 2^-8 c00 c11 the
+    projection culls by the sign of the exact determinant and its conic is
+    the exact inverse to 1 ulp; elsewhere it is the fp32 formula bit for bit.
+    The fp32 formula alone is far off on the near-singular cases (a
+    cancellation of up to 24 bits), which is what the FP64 branch is for."""
+    from util import expected_conic_2d, near_singular_2d_cases
+
+    means, cov6, vm, K, W, H, cs, c01s = near_singular_2d_cases()
+    r, m2, d, cn, _ = O.proj_fwd(means, cov6, None, None, vm, K, W, H)
+    n_branch = n_fp32_bad = 0
+    for i in range(cs.size):
+        det_exact, exact, det32, fp32, branch = expected_conic_2d(cs[i], c01s[i])
+        if branch:
+            n_branch += 1
+            if det_exact <= 0:
+                assert r[0, i] == 0, (i, cs[i], c01s[i])
+                n_fp32_bad += int(det32 > 0)
+                continue
+            assert r[0, i] > 0, i
+            ulp = np.spacing(np.abs(exact))
+            assert (np.abs(cn[0, i].astype(np.float64) - exact) <= ulp).all(), (i, cn[0, i], exact)
+            if fp32 is None or np.abs(fp32.astype(np.float64) / exact - 1.0).max() > 1e-3:
+                n_fp32_bad += 1
+        else:
+            assert det_exact > 0 and r[0, i] > 0
+            assert (cn[0, i] == fp32).all(), (i, cn[0, i], fp32)
+    assert n_branch >= 20 and n_branch < cs.size
+    assert n_fp32_bad >= 10  # the fp32 formula mis-culls or errs by > 1e-3 there

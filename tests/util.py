@@ -100,3 +100,106 @@ def rel_l2(a, b):
     b = np.asarray(b, np.float64)
     den = np.linalg.norm(b)
     return float(np.linalg.norm(a - b) / (den if den > 0 else 1.0))
+
+
+def near_singular_2d_cases():
+    """Projection inputs whose blurred 2-D covariance is [[c, c01], [c01, c]]
+    exactly, from near-singular to singular and indefinite.
+
+    The camera is the identity with fx = fy = z = 4 and the means sit on the
+    optical axis, so the perspective Jacobian is the identity and the 2-D
+    covariance equals the 3-D covariance's upper-left block bit for bit; the
+    blur adds 0.3 in fp32.  With c01 = c - k ulp(c) the exact determinant is
+    k ulp(c) (2c - k ulp(c)): positive and tiny for k >= 1 (far below what
+    fp32 resolves in c^2), zero for k = 0, negative for k < 0.  Returns
+    (means [M,3], cov6 [M,6], viewmats [1,4,4], Ks [1,3,3], W, H, c [M],
+    c01 [M]) in float32."""
+    W = H = 64
+    K = np.array([[[4.0, 0.0, 32.0], [0.0, 4.0, 32.0], [0.0, 0.0, 1.0]]], np.float32)
+    vm = np.eye(4, dtype=np.float32)[None]
+    blur = np.float32(0.3)
+    rows, cs, c01s = [], [], []
+    for lam in (16.0, 64.0, 256.0, 1024.0):
+        s = np.float32(lam / 2)
+        c = np.float32(s + blur)
+        ulp = np.float32(np.spacing(c))
+        for sign in (1.0, -1.0):
+            for k in (-3, -1, 0, 1, 2, 5, 40, 1000, 30000, 200000):
+                c01 = np.float32(sign * np.float64(c - np.float32(k) * ulp))
+                if abs(np.float64(c01)) >= 2 ** 24 * np.float64(ulp):
+                    continue
+                # the 3-D entry whose blurred 2-D counterpart is c01 (no blur off-diagonal)
+                rows.append([s, c01, 0.0, s, 0.0, s])
+                cs.append(c)
+                c01s.append(c01)
+    M = len(rows)
+    means = np.tile(np.array([[0.0, 0.0, 4.0]], np.float32), (M, 1))
+    return (means, np.asarray(rows, np.float32), vm, K, W, H, np.asarray(cs, np.float32),
+            np.asarray(c01s, np.float32))
+
+
+def expected_conic_2d(c, c01):
+    """Per case of near_singular_2d_cases: (exact determinant as a Fraction,
+    the exact inverse's three entries rounded to float32, the fp32 formula's
+    determinant and inverse, whether the FP64 branch applies)."""
+    from fractions import Fraction
+
+    c, c01 = np.float32(c), np.float32(c01)
+    det_exact = Fraction(float(c)) * Fraction(float(c)) - Fraction(float(c01)) * Fraction(float(c01))
+    det32 = np.float32(np.float32(c * c) - np.float32(c01 * c01))
+    branch = bool(det32 < np.float32(np.float32(0.00390625) * np.float32(c * c)))
+    exact = None
+    if det_exact > 0:
+        exact = np.array([float(Fraction(float(c)) / det_exact), float(-Fraction(float(c01)) / det_exact),
+                          float(Fraction(float(c)) / det_exact)]).astype(np.float32)
+    fp32 = None
+    if det32 > 0:
+        invd = np.float32(np.float32(1.0) / det32)
+        fp32 = np.array([c * invd, -c01 * invd, c * invd], np.float32)
+    return det_exact, exact, det32, fp32, branch
+
+
+def cross_correlated_6d(N, seed=0):
+    """6-D primitives with strong spatial/view cross-covariance and unequal
+    Beta factors below 4 (beta_adj = beta / 4 < 1 per query dim), for which
+    the conditioned covariance's mirrored upper triangle is often indefinite
+    (about one in five): means [N,6], covars [N,6,6], opacities [N,1],
+    betas [N,3], query [N,3], float32."""
+    g = np.random.Generator(np.random.PCG64(seed))
+    L = np.tril(g.normal(0.0, 1.0, (N, 6, 6)))
+    s = np.exp(g.normal(-2.5, 0.7, (N, 6)))
+    L = L * s[:, :, None]
+    idx = np.arange(6)
+    L[:, idx, idx] = np.abs(L[:, idx, idx]) + 1e-3 * s
+    cov = L @ np.transpose(L, (0, 2, 1))
+    means = np.concatenate([g.uniform(-0.3, 0.3, (N, 3)), g.uniform(-1.0, 1.0, (N, 3))], 1)
+    opac = g.uniform(0.3, 0.9, (N, 1))
+    betas = g.uniform(0.2, 3.9, (N, 3))
+    q = g.normal(0.0, 1.0, (N, 3))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    f = lambda a: np.ascontiguousarray(a, np.float32)
+    return f(means), f(cov), f(opac), f(betas), f(q)
+
+
+def blurred_det_ratio(means3, V33, viewmat, K, W, H, eps2d=0.3):
+    """det / (c00 c11) of the blurred 2-D covariance of each primitive (mean
+    means3 [N,3], covariance the mirrored upper triangle of V33 [N,3,3]) in
+    one camera, evaluated in FP64 with the projection's Jacobian and clamp;
+    the projection's FP64 branch applies below 2^-8."""
+    V = np.asarray(V33, np.float64)
+    S = np.triu(V) + np.transpose(np.triu(V, 1), (0, 2, 1))
+    vm, K = np.asarray(viewmat, np.float64), np.asarray(K, np.float64)
+    R, t = vm[:3, :3], vm[:3, 3]
+    pc = np.asarray(means3, np.float64) @ R.T + t
+    Sc = R @ S @ R.T
+    fx, fy, cx, cy = K[0, 0], K[1, 1], K[0, 2], K[1, 2]
+    x, y, z = pc[:, 0], pc[:, 1], pc[:, 2]
+    tfx, tfy = 0.5 * W / fx, 0.5 * H / fy
+    tx = z * np.minimum((W - cx) / fx + 0.3 * tfx, np.maximum(-(cx / fx + 0.3 * tfx), x / z))
+    ty = z * np.minimum((H - cy) / fy + 0.3 * tfy, np.maximum(-(cy / fy + 0.3 * tfy), y / z))
+    J = np.zeros((pc.shape[0], 2, 3))
+    J[:, 0, 0], J[:, 0, 2] = fx / z, -fx * tx / z ** 2
+    J[:, 1, 1], J[:, 1, 2] = fy / z, -fy * ty / z ** 2
+    c2 = J @ Sc @ np.transpose(J, (0, 2, 1))
+    c00, c11, c01 = c2[:, 0, 0] + eps2d, c2[:, 1, 1] + eps2d, c2[:, 0, 1]
+    return (c00 * c11 - c01 * c01) / (c00 * c11)

@@ -15,6 +15,16 @@
 
 namespace ubs {
 
+// Below this fraction of c00 c11 the fp32 determinant c00 c11 - c01^2 of the
+// blurred 2-D covariance has lost more than 15 bits to cancellation (its
+// relative error bound is 2^-23 c00 c11 / det); there the projection takes the
+// determinant, the culling test, the inverse and the compensation in FP64.
+// Elongated primitives reach it (det / (c00 c11) ~ 1.2 / lambda_max for a
+// needle at 45 degrees), and so do conditioned covariances whose mirrored
+// upper triangle is indefinite (the conditioning is not symmetric for unequal
+// Beta factors), where det approaches 0 from above.
+constexpr float kDetFp64Ratio = 0.00390625f;  // 2^-8
+
 struct Geom {
     float R[3][3], t[3], pc[3], Sw[3][3], Sc[3][3];
     float fx, fy, cx, cy, W, H;
@@ -153,14 +163,32 @@ __device__ __forceinline__ bool proj_fwd_body(const float *__restrict__ means, c
         m2y = G.fy * G.pc[1] * G.rz + G.cy;
     }
     float c00 = G.cov2d[0][0], c01 = G.cov2d[0][1], c10 = G.cov2d[1][0], c11 = G.cov2d[1][1];
+    const float o00 = c00, o11 = c11;
     const float det_orig = c00 * c11 - c01 * c10;
     c00 += eps2d;
     c11 += eps2d;
-    const float det = c00 * c11 - c01 * c10;
-    const float cmp = sqrtf(fmaxf(0.f, det_orig / det));
-    if (det <= 0.f) return false;
-    const float invd = 1.f / det;
-    const float i00 = c11 * invd, i01 = -c01 * invd, i11 = c00 * invd;
+    float det = c00 * c11 - c01 * c10;
+    float cmp, i00, i01, i11;
+    if (det < kDetFp64Ratio * (c00 * c11)) {
+        // near-singular (or indefinite) 2-D covariance: the culling test, the
+        // inverse and the compensation from an FP64 determinant of the same
+        // fp32 entries (the products are exact in FP64, one rounding)
+        const double d64 = (double)c00 * (double)c11 - (double)c01 * (double)c10;
+        if (!(d64 > 0.0)) return false;
+        const double o64 = (double)o00 * (double)o11 - (double)c01 * (double)c10;
+        cmp = sqrtf(fmaxf(0.f, (float)(o64 / d64)));
+        i00 = (float)((double)c11 / d64);
+        i01 = (float)(-(double)c01 / d64);
+        i11 = (float)((double)c00 / d64);
+        det = (float)d64;
+    } else {
+        cmp = sqrtf(fmaxf(0.f, det_orig / det));
+        if (det <= 0.f) return false;
+        const float invd = 1.f / det;
+        i00 = c11 * invd;
+        i01 = -c01 * invd;
+        i11 = c00 * invd;
+    }
     const float b = 0.5f * (c00 + c11);
     const float v1 = b + sqrtf(fmaxf(0.01f, b * b - det));
     const float radius = ceilf(sqrtf(v1));
