@@ -140,20 +140,45 @@ def algorithmic_bytes(name: str, c: dict, n_isect: int, C: int = 1) -> float:
     return float("nan")
 
 
-def measured_counters(entry: str, workload: str, n_isect: int):
-    """(HBM bytes per call, VALU issue share and LDS busy share of the entry's
-    main kernel) from the committed rocprofv3 --pmc summary of this same
-    workload, or Nones when there is none for it."""
+FP32_PEAK_TFLOPS = 157.3
+
+# FP32 operations per lane evaluation of the reference's per-pair math
+# (rasterize_to_pixels_fwd.cu:140-172, rasterize_to_pixels_bwd.cu:196-247):
+# an fma counts 2, a transcendental (exp2, log2, reciprocal) 1, a compare 0.
+BLEND_FLOP_MODEL = {
+    "ubs_rasterize_to_pixels_fwd": "11 per live lane evaluation (offset and sigma) + 8 per in-support one "
+                                   "((1-sigma)^beta, alpha, next T) + 7 per contributing one (vis, 3 colours)",
+    "ubs_rasterize_to_pixels_bwd": "11 per live lane evaluation (offset and sigma) + 70 per contributing one "
+                                   "(alpha again, T recovery, 3 colour and alpha gradients, sigma, conic, xy, "
+                                   "opacity and beta gradients, the running colour sum)",
+}
+
+
+def blend_flops(entry: str, pairs: dict) -> float:
+    if entry == "ubs_rasterize_to_pixels_fwd":
+        f = pairs["fwd"]
+        return 11.0 * f["lane_evals_live"] + 8.0 * f["lane_evals_in_support"] + 7.0 * f["lane_evals_contributing"]
+    b = pairs["bwd"]
+    return 11.0 * b["lane_evals_live"] + 70.0 * b["lane_evals_contributing"]
+
+
+def measured_counters(entry: str, workload: str, n_isect: int) -> dict:
+    """HBM bytes per call and the VALU / LDS use of the entry's main kernel
+    from the committed rocprofv3 --pmc summary of this same workload (empty
+    when there is none for it)."""
     try:
         with open(os.path.join(ROOT, "profiles", "pmc_traffic.json")) as f:
             d = json.load(f)
     except (OSError, ValueError):
-        return None, None, None
+        return {}
     if d.get("workload") != workload or d.get("n_isects") != n_isect or entry not in d.get("entries", {}):
-        return None, None, None
+        return {}
     e = d["entries"][entry]
     util = d.get("valu", {}).get(e["kernels"][0], {}) if e.get("kernels") else {}
-    return e["traffic_bytes"], util.get("valu_busy"), util.get("lds_util")
+    return {"traffic": e.get("traffic_bytes"), "valu_busy": util.get("valu_busy"),
+            "valu_busy_formula": d.get("valu_busy_formula"), "lds_busy": util.get("lds_util"),
+            "valu_insts_per_wave_step": util.get("valu_insts_per_wave_step"),
+            "trans_insts_per_wave_step": util.get("trans_insts_per_wave_step")}
 
 
 def cpu_baseline(cfg: dict, seed: int = 0):
@@ -176,6 +201,101 @@ def cpu_baseline(cfg: dict, seed: int = 0):
     return {"value": 1.0 / dt, "unit": "frames/s", "cores": O.num_threads(), "kind": "port",
             "sample": f"one full fwd+bwd frame of the same workload through the oracle (float32, "
                       f"{O.num_threads()} OpenMP threads): {dt:.2f} s"}
+
+
+def _probe_lib():
+    """libubs_probe.so: the measurement build (VALU calibration kernels, the
+    blend kernels with pair counters), built beside libubs_hip.so."""
+    import ctypes
+
+    path = os.path.join(ROOT, "universal-beta-splatting_amd", "gsplat", "cuda", "libubs_probe.so")
+    lib = ctypes.CDLL(path)
+    vp, u32, i64, i32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int64, ctypes.c_int
+    lib.ubs_probe_valu.argtypes = [i32, i32, i32, vp, vp]
+    lib.ubs_probe_raster_fwd_stats.argtypes = [u32, u32, i64] + [vp] * 5 + [u32] * 4 + [vp] * 8
+    lib.ubs_probe_raster_bwd_stats.argtypes = [u32, u32, i64] + [vp] * 5 + [u32] * 4 + [vp] * 10
+    for f in (lib.ubs_probe_valu, lib.ubs_probe_raster_fwd_stats, lib.ubs_probe_raster_bwd_stats):
+        f.restype = i32
+    return lib
+
+
+def valu_calibration(blocks: int = 4096, iters: int = 200, reps: int = 5) -> None:
+    """The VALU calibration kernels (8 independent chains of v_fma_f32, or of
+    v_exp_f32, per thread; 128 x iters instructions each): HIP-event time per
+    dispatch, for the utilisation formula's check under rocprofv3 --pmc."""
+    import torch
+
+    from gsplat.cuda import _wrapper as W
+
+    lib = _probe_lib()
+    out = torch.empty(blocks * 256, dtype=torch.float32, device="cuda:0")
+    st = W.stream_of(out)
+    res = {}
+    for kind, name in ((0, "fma_f32"), (1, "exp_f32")):
+        assert lib.ubs_probe_valu(kind, blocks, iters, W.ptr(out), st) == 0
+        torch.cuda.synchronize()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+        for a, b in ev:
+            a.record()
+            assert lib.ubs_probe_valu(kind, blocks, iters, W.ptr(out), st) == 0
+            b.record()
+        torch.cuda.synchronize()
+        ms = sorted(a.elapsed_time(b) for a, b in ev)[reps // 2]
+        waves = blocks * 4
+        insts = waves * iters * 128
+        res[name] = {"ms": round(ms, 4), "wave_instructions": insts,
+                     "wave_insts_per_ns": round(insts / (ms * 1e6), 2)}
+    print(json.dumps({"valu_calibration": res, "blocks": blocks, "threads": 256, "iters": iters}))
+
+
+def blend_pair_stats(params, meta, img, tgt, W_: int, H_: int) -> dict:
+    """Counts of the blend's work on this step's tile lists, from the
+    counting builds of the two kernels (libubs_probe.so): (wave, primitive)
+    steps, lane evaluations and how many of them fall inside the support or
+    contribute.  The L1 loss's image gradient drives the backward."""
+    import torch
+
+    from gsplat.cuda import _wrapper as W
+
+    lib = _probe_lib()
+    dev = img.device
+    with torch.no_grad():
+        rgb = params.rgb.detach()[None].contiguous()
+        m2, cn, op, be = (meta[k].detach().contiguous() for k in ("means2d", "conics", "opacities", "betas"))
+        offs, flat = meta["isect_offsets"], meta["flatten_ids"]
+        C, N = op.shape
+        n = int(flat.numel())
+        th, tw = offs.shape[1:3]
+        rc = torch.empty((C, H_, W_, 3), device=dev)
+        ra = torch.empty((C, H_, W_, 1), device=dev)
+        last = torch.empty((C, H_, W_), dtype=torch.int32, device=dev)
+        Tf = torch.empty((C, H_, W_), device=dev)
+        ws = torch.empty((int(W.lib().ubs_rasterize_workspace_bytes(C, N, 1)),), dtype=torch.uint8, device=dev)
+        sf = torch.zeros(5, dtype=torch.int64, device=dev)
+        sb = torch.zeros(7, dtype=torch.int64, device=dev)
+        st = W.stream_of(rc)
+        assert lib.ubs_probe_raster_fwd_stats(C, N, n, W.ptr(m2), W.ptr(cn), W.ptr(rgb), W.ptr(op), W.ptr(be), W_, H_,
+                                              tw, th, W.ptr(offs), W.ptr(flat), W.ptr(rc), W.ptr(ra), W.ptr(last),
+                                              W.ptr(Tf), W.ptr(ws), W.ptr(sf), st) == 0
+        vrc = (torch.sign(img.detach() - tgt) / float(H_ * W_ * 3)).contiguous()
+        vra = torch.zeros_like(ra)
+        vcol = torch.empty((C, N, 3), device=dev)
+        assert lib.ubs_probe_raster_bwd_stats(C, N, n, W.ptr(m2), W.ptr(cn), W.ptr(rgb), W.ptr(op), W.ptr(be), W_, H_,
+                                              tw, th, W.ptr(offs), W.ptr(flat), W.ptr(last), W.ptr(Tf), W.ptr(vrc),
+                                              W.ptr(vra), W.ptr(vcol), W.ptr(ws), W.ptr(sb), st) == 0
+        torch.cuda.synchronize()
+        f, b = [int(x) for x in sf.cpu()], [int(x) for x in sb.cpu()]
+    return {
+        "fwd": {"wave_primitive_steps": f[0], "lane_evals_live": f[1], "lane_evals_in_support": f[2],
+                "lane_evals_contributing": f[3], "entries_staged": f[4],
+                "in_support_frac": round(f[2] / max(f[1], 1), 4)},
+        "bwd": {"wave_primitive_steps": b[0], "lane_evals_live": b[1], "lane_evals_contributing": b[2],
+                "reduce_scatter_steps": b[3], "entries_staged": b[4], "reduce_scatter_steps_le4_lanes": b[5],
+                "reduce_scatter_steps_le16_lanes": b[6],
+                "contributing_frac": round(b[2] / max(b[1], 1), 4)},
+        "source": "counting builds of raster_fwd_kernel<16,3> / raster_bwd_kernel<16,3> (libubs_probe.so) on "
+                  "this step's tile lists",
+    }
 
 
 def copy_calibration(mib: int, reps: int = 5) -> None:
@@ -206,6 +326,10 @@ def main():
                          "c4: batched 8-view step at mixed resolutions (reports views/s); "
                          "c5: render serving of 4 resident scenes (reports MPix/s, p50/p99)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-pair-stats", action="store_true",
+                    help="leave out the blend's pair counts (counting builds of the blend kernels)")
+    ap.add_argument("--no-chain", action="store_true",
+                    help="leave out the drop-in chain's fwd+bwd rate (the reference's unchanged call sequence)")
     ap.add_argument("--no-serve", action="store_true",
                     help="leave out the batched-serve measurement (for counter runs that average per kernel)")
     ap.add_argument("--lambda-dssim", type=float, default=0.0,
@@ -229,6 +353,8 @@ def main():
                          "(default, as the reference) or stored by the forward after every batch of 256 entries")
     ap.add_argument("--fp64-transmittance", action="store_true",
                     help="recover transmittance in fp64 in the blend backward")
+    ap.add_argument("--valu-calibration", action="store_true",
+                    help="only run the VALU calibration kernels (fma and exp chains) and exit")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (the known-size reference for "
                          "scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings) and exit")
@@ -239,6 +365,9 @@ def main():
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.valu_calibration:
+        valu_calibration()
         return
 
     import numpy as np
@@ -339,10 +468,41 @@ def main():
         torch.cuda.synchronize()
         fwd_ms = (time.perf_counter() - f0) * 1e3 / args.steps
 
+    # ---- the drop-in chain (the reference's unchanged BetaModel.render: the
+    # _wrapper ops for rotation, covariance and conditioning, then
+    # rasterization() with covars) on the same scene, same loss, fwd+bwd
+    chain = None
+    if args.path == "fused" and not args.no_chain:
+        def chain_step():
+            for p in leaves:
+                p.grad = None
+            img, _, _ = beta.render(params, vm, K, cfg["W"], cfg["H"], timestamp=t_view, rest=rest)
+            (img[0] - tgt).abs().mean().backward()
+
+        for _ in range(3):
+            chain_step()
+        with no_gc():
+            torch.cuda.synchronize()
+            c0 = time.perf_counter()
+            for _ in range(args.steps):
+                chain_step()
+            torch.cuda.synchronize()
+            chain_ms = (time.perf_counter() - c0) * 1e3 / args.steps
+        chain = {"frames_per_s": round(1e3 / chain_ms, 3), "ms_per_step": round(chain_ms, 3),
+                 "path": "reference call chain: l_triangle_to_rotmat, rot_scale_l_triangle_to_covar, "
+                         "cond_mean_convariance_opacity, rasterization(covars=...), torch L1"}
+
     # ---- batched serving of the same scene: 8 views at the same size in one
     # pass (fused projection, one isect/sort/blend over all views) against
     # the same 8 views rendered one at a time; forward only
     serve = None if args.no_serve else serve_views(params, cfg, args.steps)
+
+    # ---- counts of the blend's work on this step's lists (counting builds)
+    pairs = None
+    if not args.no_pair_stats and args.path == "fused":
+        img_s, _, meta_s = fwd_only()
+        pairs = blend_pair_stats(params, meta_s, img_s, tgt, cfg["W"], cfg["H"])
+        del img_s, meta_s
 
     # ---- roofline of the dominant kernel launch
     per_call = {k: (sum(v) / len(v), len(v) / args.steps) for k, v in launches.items()}
@@ -350,15 +510,29 @@ def main():
     avg_ms = per_call[dominant][0]
     alg = algorithmic_bytes(dominant, cfg, n_isect)
     achieved = alg / (avg_ms * 1e-3) / 1e9
-    traffic, valu_busy, lds_busy = measured_counters(dominant, cfg["workload"], n_isect)
-    roofline = {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": alg}
-    if traffic is not None:
+    pmc = measured_counters(dominant, cfg["workload"], n_isect)
+    hbm = {"achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc.get("traffic"),
+           "algorithmic_bytes_per_launch": alg}
+    blend = dominant in ("ubs_rasterize_to_pixels_fwd", "ubs_rasterize_to_pixels_bwd")
+    if blend and pairs is not None:
+        # the blend does no dense contraction: its bound is the FP32 VALU (with
+        # the transcendental unit beside it), priced at the published peak
+        flop = blend_flops(dominant, pairs)
+        tf = flop / (avg_ms * 1e-3) / 1e12
+        roofline = {"bound": "valu", "kernel": dominant, "achieved": round(tf, 2), "peak": FP32_PEAK_TFLOPS,
+                    "unit": "TFLOP/s", "frac": round(tf / FP32_PEAK_TFLOPS, 4), "traffic": pmc.get("traffic"),
+                    "avg_launch_ms": round(avg_ms, 4), "algorithmic_flop_per_launch": flop,
+                    "flop_model": BLEND_FLOP_MODEL[dominant], "hbm": hbm}
+    else:
+        roofline = {"bound": "hbm", "kernel": dominant, **hbm, "avg_launch_ms": round(avg_ms, 4)}
+    if pmc.get("traffic") is not None:
         roofline["traffic_source"] = ("profiles/pmc_traffic.json: rocprofv3 FETCH_SIZE and WRITE_SIZE of this "
                                       "workload, scaled against a 1 GiB device copy")
-        roofline["valu_busy"] = valu_busy  # the blend is VALU-bound, not HBM-bound
-        roofline["lds_busy"] = lds_busy
+    for k in ("valu_busy", "valu_busy_formula", "valu_insts_per_wave_step", "trans_insts_per_wave_step",
+              "lds_busy"):
+        if pmc.get(k) is not None:
+            roofline[k] = pmc[k]
     def gbps(k, ms):
         r = algorithmic_bytes(k, cfg, n_isect) / (ms * 1e-3) / 1e9
         return round(r, 1) if math.isfinite(r) else None  # JSON has no NaN
@@ -368,6 +542,12 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(cfg)
+
+    # ---- the render-serving workload, compact (its own line: --config c5)
+    serving = None
+    if not args.no_serve and args.config == "c2":
+        torch.cuda.empty_cache()
+        serving = serve_summary(args)
 
     line = {
         "metric": METRIC,
@@ -389,10 +569,14 @@ def main():
                    "sort_passes_skipped": cfg["sort_skipped"], "optimizer_step": "adam" if args.adam else None,
                    "transmittance": args.transmittance, "sort": args.sort},
         "fwd_frames_per_s": round(1e3 / fwd_ms, 3),
+        "chain_frames_per_s": None if chain is None else chain["frames_per_s"],
+        "drop_in_chain": chain,
         "fwd_bwd_mpix_per_s": round(cfg["W"] * cfg["H"] * 1e3 / ms_per_step / 1e6, 2),
         "latency_ms": {"p50": round(step_ms[len(step_ms) // 2], 3),
                        "p99": round(step_ms[min(len(step_ms) - 1, int(math.ceil(0.99 * len(step_ms))) - 1)], 3)},
         "batched_serve": serve,
+        "serving_c5": serving,
+        "blend_pairs": pairs,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "stages": stages,
@@ -445,11 +629,17 @@ def serve_views(params, cfg, steps: int, views: int = 8) -> dict:
             "speedup": round(out["per_view"]["ms_per_batch"] / out["batched"]["ms_per_batch"], 3)}
 
 
-def serve_bench(args, cfg):
-    """Render serving (gsplat.serve): resident scenes, Poisson arrivals of
-    view requests of mixed resolution, continuous batching against one view
-    at a time.  value = MPix/s of the batched server with every request
-    pending at once (saturation); p50/p99 latency per arrival rate."""
+def serve_measure(args, cfg, rates, n_sat: int, duration_s: float, modes=None, log=sys.stderr):
+    """Render serving (gsplat.serve) of cfg's resident scenes: Poisson
+    arrivals of view requests of mixed resolution and a saturation run with
+    every request pending at once, per server mode:
+      graphs   -- batches of one image size over any resident scenes as
+                  replayed hipGraphs with device-sized lists (the default);
+      batched  -- one scene's views per batch through render_batch (host
+                  read of the intersection count per resolution group);
+      per_view -- one view at a time in arrival order (the reference's
+                  structure, BetaModel.view per request).
+    Returns (store, saturation, poisson) with latency_summary dicts."""
     import numpy as np
     import torch
 
@@ -465,7 +655,7 @@ def serve_bench(args, cfg):
         vm = torch.tensor(scenes.look_at_viewmat(np.array([0.0, 0.4, 4.0])), dtype=torch.float32)
         store.calibrate(name, vm, torch.tensor(scenes.intrinsics(1920, 1080), dtype=torch.float32), t=0.5)
         print(f"scene {name}: {cfg['N']} primitives, {D}-D, {store.isects_per_prim[name]:.1f} isects/primitive",
-              file=sys.stderr, flush=True)
+              file=log, flush=True)
     rng = np.random.Generator(np.random.PCG64(11))
     names = [n for n, _, _ in cfg["scenes"]]
 
@@ -477,15 +667,18 @@ def serve_bench(args, cfg):
                            torch.tensor(scenes.intrinsics(W, H), dtype=torch.float32), W, H,
                            t=float(rng.uniform(0, 1)), rid=i)
 
-    modes = {"batched": dict(max_views=args.serve_max_views, streams=args.serve_streams),
-             "per_view": dict(max_views=1, streams=1)}
+    all_modes = {"graphs": dict(max_views=args.serve_max_views, streams=args.serve_streams, graphs=True),
+                 "batched": dict(max_views=args.serve_max_views, streams=args.serve_streams),
+                 "per_view": dict(max_views=1, streams=1)}
+    modes = {k: all_modes[k] for k in (modes or all_modes)}
+    servers = {m: RenderServer(store, keep_images=False, **kw) for m, kw in modes.items()}
 
     def run(reqs, arrivals, mode):
         with no_gc():
             return _run(reqs, arrivals, mode)
 
     def _run(reqs, arrivals, mode):
-        srv = RenderServer(store, keep_images=False, **modes[mode])
+        srv = servers[mode]  # kept across runs: its captured graphs are reused
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         i, done = 0, []
@@ -501,44 +694,83 @@ def serve_bench(args, cfg):
             done += srv.results
             srv.results = []
         wall = max(r.done for r in done) - t0
-        return latency_summary(done, wall)
+        out = latency_summary(done, wall)
+        out["overflows"] = srv.overflows
+        return out
 
-    warm = [make_request(-1 - k) for k in range(12)]
+    # warm-up: every (size, slot count) batch shape the runs can form is
+    # captured before the timed runs
     for mode in modes:
+        warm = [make_request(-1 - k) for k in range(3 * args.serve_max_views)]
         run(warm, [0.0] * len(warm), mode)
-
-    n_sat = cfg["saturation_views"]
-    sat_reqs = [make_request(k) for k in range(n_sat)]
+        for v in (1, 2, 3, 5):
+            warm = [make_request(-100 - k) for k in range(v * len(cfg["sizes"]))]
+            run(warm, [0.0] * len(warm), mode)
     saturation = {}
+    sat_reqs = [make_request(k) for k in range(n_sat)]
     for mode in modes:
         saturation[mode] = run(sat_reqs, [0.0] * n_sat, mode)
-        print(f"saturation {mode}: {saturation[mode]}", file=sys.stderr, flush=True)
+        print(f"saturation {mode}: {saturation[mode]}", file=log, flush=True)
     poisson = {}
-    for lam in cfg["rates"]:
-        gaps = rng.exponential(1.0 / lam, size=int(lam * cfg["duration_s"] * 2) + 8)
+    for lam in rates:
+        gaps = rng.exponential(1.0 / lam, size=int(lam * duration_s * 2) + 8)
         arr = np.cumsum(gaps)
-        arr = arr[arr < cfg["duration_s"]]
+        arr = arr[arr < duration_s]
         reqs = [make_request(k) for k in range(arr.size)]
         poisson[str(lam)] = {}
         for mode in modes:
             poisson[str(lam)][mode] = run(reqs, list(arr), mode)
-            print(f"poisson {lam}/s {mode}: {poisson[str(lam)][mode]}", file=sys.stderr, flush=True)
-    b, pv = saturation["batched"], saturation["per_view"]
+            print(f"poisson {lam}/s {mode}: {poisson[str(lam)][mode]}", file=log, flush=True)
+    return store, saturation, poisson
+
+
+def serve_bench(args, cfg):
+    """The render-serving workload as its own bench line: value = MPix/s of
+    the graph-batched server with every request pending at once
+    (saturation); p50/p99 latency per arrival rate, per server mode."""
+    import torch
+
+    store, saturation, poisson = serve_measure(args, cfg, cfg["rates"], cfg["saturation_views"], cfg["duration_s"])
+    b, pv = saturation["graphs"], saturation["per_view"]
     line = {
         "metric": "batched-serve MPix/s (4 resident 2M-primitive scenes, mixed resolution); p50/p99 latency",
-        "value": b["mpix_per_s"], "unit": "MPix/s", "n_gpus": 1, "steps": n_sat, "warmup": len(warm),
+        "value": b["mpix_per_s"], "unit": "MPix/s", "n_gpus": 1, "steps": cfg["saturation_views"], "warmup": 0,
         "ms_per_step": round(1e3 / b["frames_per_s"], 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "fp32",
         "data": "synthetic (seeded scenes, cameras and arrivals, gsplat/scenes.py)",
-        "config": {"workload": cfg["workload"], "primitives_per_scene": cfg["N"], "scenes": len(names),
+        "config": {"workload": cfg["workload"], "primitives_per_scene": cfg["N"], "scenes": len(cfg["scenes"]),
                    "resident_gb": round(store.resident_bytes() / 1e9, 3), "max_views_per_batch": args.serve_max_views,
-                   "sort": args.sort, "streams": args.serve_streams, "peak_hbm_gb": round(torch.cuda.max_memory_allocated() / 1e9, 2)},
+                   "sort": args.sort, "streams": args.serve_streams,
+                   "peak_hbm_gb": round(torch.cuda.max_memory_allocated() / 1e9, 2)},
         "saturation": saturation,
         "speedup_vs_per_view": round(b["mpix_per_s"] / pv["mpix_per_s"], 3),
+        "speedup_batched_eager_vs_per_view": round(saturation["batched"]["mpix_per_s"] / pv["mpix_per_s"], 3),
         "poisson": poisson,
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(line))
+
+
+def serve_summary(args) -> dict:
+    """A compact run of the render-serving workload for the headline line:
+    4 resident 2M-primitive scenes, 48 pending requests at once
+    (saturation) and 1 s of Poisson arrivals at 400 requests/s, graph-batched
+    server against one view at a time."""
+    import io
+
+    cfg = CONFIGS["c5"]
+    log = io.StringIO()
+    store, sat, poi = serve_measure(args, cfg, (400,), 48, 1.0, modes=("graphs", "per_view"), log=log)
+    g, pv = sat["graphs"], sat["per_view"]
+    p400 = poi["400"]
+    out = {"workload": cfg["workload"],
+           "saturated_mpix_per_s": {"graphs": g["mpix_per_s"], "per_view": pv["mpix_per_s"]},
+           "speedup_vs_per_view": round(g["mpix_per_s"] / pv["mpix_per_s"], 3),
+           "poisson_400_per_s": {m: {"p50_ms": v["p50_ms"], "p99_ms": v["p99_ms"], "views": v["views"],
+                                     "mean_batch_views": v["mean_batch_views"]} for m, v in p400.items()},
+           "overflows": g.get("overflows", 0)}
+    del store
+    return out
 
 
 def batch_bench(args, cfg):

@@ -205,6 +205,43 @@ int ubs_beta_project_fwd(int64_t N, int64_t D, uint32_t C, const float *xyz, con
  * 242-348, cond_mean_convariance_opacity_bwd.cu:565-630,
  * rot_scale_l_triangle_to_covar_bwd.cu:238-293, l_triagnle_to_rotmat_bwd.cu:
  * 27-58 and torch's activation backward); follows ubs_set_cond_backward_exact. */
+/* The same projection writing view c's rows at c * row_stride (row_stride
+ * >= N): several scenes' views share one [C, row_stride] batch of projected
+ * rows, each scene's call writing its views' first N entries; the caller
+ * zero-fills the radii of the rows past a scene's N, which then count as
+ * culled.  No counterpart in the reference (the render server's batches
+ * across resident scenes). */
+int ubs_beta_project_fwd_rows(int64_t N, int64_t D, uint32_t C, const float *xyz, const float *mean_rest,
+                              const float *scale_raw, const float *l_triangle, const float *opacity_raw,
+                              const float *beta_raw, const float *quats, const float *cam_pos,
+                              const float *timestamps, const float *viewmats, const float *Ks, int32_t width,
+                              int32_t height, const int32_t *image_wh, float eps2d, float near_plane,
+                              float far_plane, float radius_clip, int ortho, int32_t *radii, float *means2d,
+                              float *depths, float *conics, float *opacities, float *betas, int64_t row_stride,
+                              void *stream);
+
+/* One view's scene in a batch over several resident scenes: the leaf
+ * tensors as ubs_beta_project_fwd takes them (quats may be NULL), rgb [N,3],
+ * N (0 marks an empty slot) and D (6 or 7). */
+typedef struct ubs_view_scene {
+    const float *xyz, *mean_rest, *scale_raw, *l_triangle, *opacity_raw, *beta_raw, *quats, *rgb;
+    int64_t N, D;
+} ubs_view_scene;
+
+/* Forward projection of C views of several scenes (the render server's
+ * batches): table (device memory, C entries) names each view's scene; view
+ * c's rows are entries c * row_stride + n (row_stride >= the largest N),
+ * image_wh [C,2] per view, and colors [C, row_stride, 3] receives each view's
+ * scene colours.  Entries past a view's N get radius 0.  Per view the
+ * results equal ubs_beta_project_fwd's for that scene bit for bit.  No
+ * counterpart in the reference (BetaModel.view renders one request at a
+ * time, scene/beta_model.py:725-831). */
+int ubs_beta_project_fwd_table(uint32_t C, int64_t row_stride, const ubs_view_scene *table, const float *cam_pos,
+                               const float *timestamps, const float *viewmats, const float *Ks,
+                               const int32_t *image_wh, float eps2d, float near_plane, float far_plane,
+                               float radius_clip, int32_t *radii, float *means2d, float *depths, float *conics,
+                               float *opacities, float *betas, float *colors, void *stream);
+
 int ubs_beta_project_bwd(int64_t N, int64_t D, uint32_t C, const float *xyz, const float *mean_rest,
                          const float *scale_raw, const float *l_triangle, const float *opacity_raw,
                          const float *beta_raw, const float *quats, const float *cam_pos, const float *timestamps,
@@ -297,6 +334,31 @@ int ubs_isect_tiles_emit_ordered(uint32_t C, uint32_t N, const float *means2d, c
                                  uint32_t tile_width, uint32_t tile_height, int64_t *isect_ids, int32_t *flatten_ids,
                                  void *stream);
 
+/* Device-sized intersection lists (no host read of n_isects): the caller
+ * allocates the lists for a capacity `cap` and passes the count that
+ * ubs_isect_tiles_count[_range] left in device memory (n_dev).
+ *   ubs_isect_tiles_emit_ordered_cap: ubs_isect_tiles_emit_ordered storing
+ *     only the entries at index < cap (a count above cap is an overflow the
+ *     caller detects from *n_dev after the work and repeats with more room);
+ *   ubs_radix_sort_pairs_dev: ubs_radix_sort_pairs of min(*n_dev, n_cap)
+ *     keys, workspace ubs_radix_sort_workspace_bytes(n_cap, ...); every pass
+ *     runs, so *result_in_alt is known on return;
+ *   ubs_isect_offset_encode_dev: ubs_isect_offset_encode of min(*n_dev,
+ *     n_cap) sorted ids.
+ * The launches of a batch then have fixed shapes and no host
+ * synchronisation, so a caller can capture them in a hipGraph and replay
+ * it.  No counterpart in the reference (it reads n_isects on the host,
+ * isect_tiles.cu:180-181). */
+int ubs_isect_tiles_emit_ordered_cap(uint32_t C, uint32_t N, const float *means2d, const int32_t *radii,
+                                     const float *depths, const int32_t *order, const int64_t *cum, uint32_t ts,
+                                     uint32_t tw, uint32_t th, int64_t *isect_ids, int32_t *flatten_ids,
+                                     int64_t cap, void *stream);
+int ubs_radix_sort_pairs_dev(int64_t n_cap, const int64_t *n_dev, int64_t *keys, int64_t *keys_alt, int32_t *vals,
+                             int32_t *vals_alt, int begin_bit, int end_bit, void *workspace, size_t ws_bytes,
+                             int *result_in_alt, void *stream);
+int ubs_isect_offset_encode_dev(int64_t n_cap, const int64_t *n_dev, const int64_t *isect_ids, uint32_t C,
+                                uint32_t tile_width, uint32_t tile_height, int32_t *offsets, void *stream);
+
 /* Depth sort inside each (camera, tile) segment of isect ids that are
  * already sorted stably on their camera/tile bits (ubs_radix_sort_pairs_skip
  * with begin_bit 32): keys_in/vals_in [n] -> keys_out/vals_out [n], each
@@ -379,6 +441,18 @@ int ubs_rasterize_to_pixels_bwd(uint32_t C, uint32_t N, int64_t n_isects, uint32
  * needs no initialisation; the pair must see the same tile lists. */
 size_t ubs_transmittance_checkpoint_floats(uint32_t C, int64_t n_isects, uint32_t tile_width,
                                            uint32_t tile_height, uint32_t tile_size);
+
+/* ubs_rasterize_to_pixels_fwd over lists sized on the device: n_cap is the
+ * lists' capacity, *n_dev (device int64) the intersection count (clamped to
+ * n_cap).  No counterpart in the reference (see ubs_radix_sort_pairs_dev). */
+int ubs_rasterize_to_pixels_fwd_dev(uint32_t C, uint32_t N, int64_t n_cap, const int64_t *n_dev,
+                                    uint32_t channels, const float *means2d, const float *conics,
+                                    const float *colors, const float *opacities, const float *betas,
+                                    const float *backgrounds, const uint8_t *masks, uint32_t W, uint32_t H,
+                                    uint32_t tile_size, uint32_t tw, uint32_t th, const int32_t *tile_offsets,
+                                    const int32_t *flatten_ids, float *render_colors, float *render_alphas,
+                                    int32_t *last_ids, float *T_final, void *workspace, size_t workspace_bytes,
+                                    void *stream);
 
 int ubs_rasterize_to_pixels_fwd_ckpt(uint32_t C, uint32_t N, int64_t n_isects, uint32_t channels,
                                      const float *means2d, const float *conics, const float *colors,

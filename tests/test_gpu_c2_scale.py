@@ -21,7 +21,7 @@ from gsplat import beta, rasterization, scenes
 from gsplat.cuda import _wrapper as W
 from oracle import ubs_oracle as O
 
-from util import assert_same, rel_l2
+from util import BRANCH_FLIP_FRAC, assert_same, check_branch_pixels, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -92,12 +92,20 @@ def test_full_size_structure_bit_exact(device, case):
         assert_same(host(meta[k])[vis], rm[k][vis], k)
     assert rm["isect_ids"].size > 5_000_000
     # the float32 blend of the same lists on the 64 tiles, away from the T cut-off
-    _, _, _, mT, mS = O.raster_fwd(rm["means2d"], rm["conics"], rm["colors"], rm["opacities"], rm["betas"], WD, HT,
-                                   16, rm["isect_offsets"], rm["flatten_ids"], masks=tm[None], margins=True)
+    oc, _, olast, mT, mS = O.raster_fwd(rm["means2d"], rm["conics"], rm["colors"], rm["opacities"], rm["betas"], WD,
+                                        HT, 16, rm["isect_offsets"], rm["flatten_ids"], masks=tm[None], margins=True)
     pm = np.kron(tm, np.ones((16, 16), bool))[:HT, :WD]
     safe = pm & (mT[0] > 1e-3) & (mS[0] > 1e-6)
     assert safe.sum() > 0.95 * pm.sum()
     assert np.abs(host(img[0]) - ref_img[0])[safe].max() <= 5e-5
+    # the pixels near a branch point, with their last ids: the same kernel on
+    # the same lists gives the image bit for bit, and its last ids
+    rc, _, last, _ = W.raster_fwd_raw(meta["means2d"], meta["conics"], p.rgb[None].contiguous(), meta["opacities"],
+                                      meta["betas"], None, None, WD, HT, 16, meta["isect_offsets"],
+                                      meta["flatten_ids"])
+    assert_same(host(rc), host(img), "image")
+    check_branch_pixels(host(img[0]), host(last[0]), oc[0], olast[0], pm, safe, atol=5e-5,
+                        what=f"{case} near branch points")
 
 
 @pytest.mark.parametrize("case", ["c2", "c3"])
@@ -125,6 +133,20 @@ def test_full_size_tiles_image_and_gradients_vs_fp64(device, case):
     assert safe.sum() > 0.95 * pm.sum()
     err = np.abs(host(img[0]) - ref_img)[safe].max()
     assert err <= 1e-4, err
+    # every pixel of the 64 tiles: the float32 oracle's blend of the GPU's own
+    # projected lists bounds the pixels near a branch point (colour and last
+    # id), and all but a few of them are within 1e-4 of FP64 as well
+    g32 = {k: host(meta[k]) for k in ("means2d", "conics", "opacities", "betas", "isect_offsets", "flatten_ids")}
+    oc, _, olast = O.raster_fwd(g32["means2d"], g32["conics"], host(p.rgb)[None], g32["opacities"], g32["betas"],
+                                WD, HT, 16, g32["isect_offsets"], g32["flatten_ids"], masks=tm[None])
+    _, _, last, _ = W.raster_fwd_raw(meta["means2d"], meta["conics"], p.rgb.detach()[None].contiguous(),
+                                     meta["opacities"], meta["betas"], None, None, WD, HT, 16, meta["isect_offsets"],
+                                     meta["flatten_ids"])
+    check_branch_pixels(host(img[0]), host(last[0]), oc[0], olast[0], pm, safe, atol=5e-5,
+                        what=f"{case} near branch points")
+    far64 = pm & (np.abs(host(img[0]) - ref_img).max(axis=-1) > 1e-4)
+    print(f"{case}: {int(far64.sum())} of {int(pm.sum())} pixels beyond 1e-4 of FP64", flush=True)
+    assert far64.sum() <= max(1, int(BRANCH_FLIP_FRAC * pm.sum())), int(far64.sum())
     touched = 0
     for k, t in p.leaves().items():
         g = host(t.grad)

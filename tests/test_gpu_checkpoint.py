@@ -19,7 +19,7 @@ from gsplat import beta, scenes
 from gsplat.cuda import _wrapper as W
 from oracle import ubs_oracle as O
 
-from util import assert_same, opaque_scene, rel_l2
+from util import assert_same, check_branch_pixels, opaque_scene, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -100,6 +100,10 @@ def test_checkpoint_pair(device, C, channels, ts, mask):
         off_px = ~np.repeat(np.repeat(masks, ts, axis=1), ts, axis=2)[:, :H_, :W_]
         safe |= off_px
     assert safe.mean() > 0.9
+    c32, _, last32 = O.raster_fwd(x["m2"], x["cn"], x["col"], x["op"], x["be"], W_, H_, ts, x["off"], x["flat"],
+                                  masks=masks)
+    check_branch_pixels(host(rc), host(last), c32, last32, np.ones(safe.shape, bool), safe,
+                        what=f"checkpointed blend fwd channels={channels}")
     # stored values: batch 0 and 1 against the FP64 blend of the cut lists;
     # the batch of the pixel's last contributor stores T_final exactly
     ckh, lasth, Tfh = host(ck), host(last), host(Tf)

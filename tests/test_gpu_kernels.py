@@ -13,7 +13,7 @@ from gsplat import scenes
 from gsplat.cuda import _wrapper as W
 from oracle import ubs_oracle as O
 
-from util import assert_same, conditioned_scene, opaque_scene, rel_l2
+from util import assert_same, check_branch_pixels, conditioned_scene, opaque_scene, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -330,6 +330,8 @@ def test_raster_fwd_close(device, bg):
     assert_same(host(last)[safe], want_last[safe], "last_ids")
     np.testing.assert_allclose(host(Tf)[safe], 1.0 - want_a[..., 0][safe], atol=2e-5)
     assert (want_a > 0.9).mean() > 0.05  # the cutoff is exercised
+    check_branch_pixels(host(rc), host(last), want_c, want_last, np.ones(safe.shape, bool), safe,
+                        what=f"blend fwd bg={bg}")
 
 
 def test_raster_bwd_close(device):
@@ -354,7 +356,8 @@ def test_raster_bwd_close(device):
 
 @pytest.mark.parametrize("C,channels,ts,bg,mask", [
     (1, 1, 16, False, False), (2, 3, 8, True, False), (1, 4, 32, False, False), (1, 7, 16, True, True),
-    (2, 32, 16, False, False), (1, 35, 8, False, True), (1, 3, 32, True, False)])
+    (2, 32, 16, False, False), (1, 35, 8, False, True), (1, 3, 32, True, False), (1, 2, 16, True, False),
+    (2, 10, 16, False, True), (1, 18, 8, True, False)])
 def test_raster_bwd_variants(device, C, channels, ts, bg, mask):
     """Blend backward over channel counts (chunked register tiles), tile
     sizes, cameras, backgrounds and tile masks.  Upstream gradients are
@@ -381,6 +384,11 @@ def test_raster_bwd_variants(device, C, channels, ts, bg, mask):
         safe |= off_px
     assert safe.mean() > 0.9
     np.testing.assert_allclose(host(rc)[safe], want_c[safe], atol=3e-5)
+    # the pixels near a branch point against the float32 blend of the same lists
+    c32, _, last32 = O.raster_fwd(x["m2"], x["cn"], x["col"], x["op"], x["be"], W_, H_, ts, x["off"], x["flat"],
+                                  backgrounds=bgs, masks=masks)
+    check_branch_pixels(host(rc), host(last), c32, last32, np.ones(safe.shape, bool), safe, atol=3e-5,
+                        what=f"blend fwd C={C} channels={channels} ts={ts}")
     vrc = (g.normal(0, 1, host(rc).shape) * safe[..., None]).astype(np.float32)
     vra = (g.normal(0, 1, host(ra).shape) * safe[..., None]).astype(np.float32)
     got = W.raster_bwd_raw(cu(x["m2"]), cu(x["cn"]), cu(x["col"]), cu(x["op"]), cu(x["be"]),

@@ -9,7 +9,10 @@ import torch
 
 from gsplat import scenes
 from gsplat.beta import BetaParams, render_views
+from gsplat.cuda._wrapper import raster_fwd_raw
 from gsplat.serve import RenderServer, SceneStore, ViewRequest, latency_summary
+
+from util import BRANCH_FLIP_FRAC, check_branch_pixels
 
 pytestmark = pytest.mark.gpu
 
@@ -138,3 +141,17 @@ def test_full_size_mix_matches_alone_and_fp64(device):
     assert safe.sum() > 0.9 * pm.sum()
     err = np.abs(mixed[r.rid].image.cpu().numpy() - ref)[safe].max()
     assert err <= 1e-4, err
+    # the pixels near a branch point: the float32 oracle's blend of the served
+    # view's own projected lists, colour and last id
+    g32 = {k: meta[k].cpu().numpy() for k in ("means2d", "conics", "opacities", "betas", "isect_offsets",
+                                                "flatten_ids")}
+    rgb = p.rgb[None].contiguous()
+    oc, _, olast = O.raster_fwd(g32["means2d"], g32["conics"], rgb.cpu().numpy(), g32["opacities"], g32["betas"],
+                                r.width, r.height, 16, g32["isect_offsets"], g32["flatten_ids"], masks=tm[None])
+    with torch.no_grad():
+        _, _, last, _ = raster_fwd_raw(meta["means2d"], meta["conics"], rgb, meta["opacities"], meta["betas"], None,
+                                       None, r.width, r.height, 16, meta["isect_offsets"], meta["flatten_ids"])
+    check_branch_pixels(mixed[r.rid].image.cpu().numpy(), last[0].cpu().numpy(), oc[0], olast[0], pm, safe,
+                        atol=5e-5, what="served 1080p view near branch points")
+    far64 = pm & (np.abs(mixed[r.rid].image.cpu().numpy() - ref).max(axis=-1) > 1e-4)
+    assert far64.sum() <= max(1, int(BRANCH_FLIP_FRAC * pm.sum())), int(far64.sum())

@@ -203,3 +203,33 @@ def blurred_det_ratio(means3, V33, viewmat, K, W, H, eps2d=0.3):
     c2 = J @ Sc @ np.transpose(J, (0, 2, 1))
     c00, c11, c01 = c2[:, 0, 0] + eps2d, c2[:, 1, 1] + eps2d, c2[:, 0, 1]
     return (c00 * c11 - c01 * c01) / (c00 * c11)
+
+
+# Largest share of the checked pixels whose blend may take the other side of
+# a branch point (the T <= 1e-4 stop, the sigma < 0 skip) than the float32
+# oracle does: the GPU's (1 - sigma)^beta is exp2(beta log2(1 - sigma)) on the
+# hardware transcendentals, the oracle's is libm pow, and the two differ in
+# the last bits of alpha and T.
+BRANCH_FLIP_FRAC = 5e-4
+
+
+def check_branch_pixels(got_c, got_last, want_c, want_last, checked, safe, atol=2e-5,
+                        max_frac=BRANCH_FLIP_FRAC, what="blend"):
+    """Bound the pixels that the close comparisons leave out.
+
+    got_* / want_* are the GPU blend and the float32 oracle's blend of the
+    same lists (colour [..., H, W, D], last ids [..., H, W]); `checked` is the
+    pixel mask of the comparison, `safe` the pixels away from a branch point.
+    Every checked pixel near a branch point must agree to `atol` in colour and
+    exactly in last id, except at most `max_frac` of the checked pixels (and
+    at least one): a decision taken the other way.  Returns the counts."""
+    checked = np.asarray(checked, bool)
+    near = checked & ~np.asarray(safe, bool)
+    dc = np.abs(np.asarray(got_c, np.float64) - np.asarray(want_c, np.float64)).max(axis=-1)
+    bad = near & ((dc > atol) | (np.asarray(got_last) != np.asarray(want_last)))
+    n_checked, n_near, n_bad = int(checked.sum()), int(near.sum()), int(bad.sum())
+    allowed = max(1, int(max_frac * n_checked))
+    print(f"{what}: {n_near} of {n_checked} checked pixels near a branch point "
+          f"({n_near / max(n_checked, 1):.2%}); {n_bad} take the other branch (allowed {allowed})", flush=True)
+    assert n_bad <= allowed, (what, n_bad, allowed)
+    return n_checked, n_near, n_bad

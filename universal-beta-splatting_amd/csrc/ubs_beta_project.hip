@@ -95,6 +95,43 @@ __device__ __forceinline__ void beta_query(const BetaPrim<D> &bp, const float *_
     if constexpr (D == 7) q[3] = timestamps ? timestamps[c] : 0.f;
 }
 
+// View c of a primitive: its query, the conditioning on it and the EWA
+// projection of the conditioned 3-D primitive, written at row entry idx.
+template <int D>
+__device__ __forceinline__ void beta_view_fwd(const BetaPrim<D> &bp, uint32_t c, int64_t idx,
+                                              const float *__restrict__ cam_pos,
+                                              const float *__restrict__ timestamps,
+                                              const float *__restrict__ viewmats, const float *__restrict__ Ks,
+                                              int32_t W, int32_t H, const int32_t *__restrict__ image_wh,
+                                              float eps2d, float near_plane, float far_plane, float radius_clip,
+                                              int ortho, int32_t *__restrict__ radii, float *__restrict__ means2d,
+                                              float *__restrict__ depths, float *__restrict__ conics,
+                                              float *__restrict__ opacities, float *__restrict__ betas) {
+    constexpr int CQ = D - 3;
+    float q[CQ];
+    beta_query<D>(bp, cam_pos, timestamps, c, q);
+    CondRegs<CQ> st;
+    cond_setup<CQ>(bp.mean, bp.Sig, bp.beta + 1, q, st);
+    float m3[3], V9[9], oc;
+    cond_fwd_body<CQ>(st, bp.mean, bp.opac, m3, V9, &oc);
+    opacities[idx] = oc;
+    betas[idx] = bp.beta[0];
+    const int32_t Wc = image_wh ? image_wh[2 * c] : W, Hc = image_wh ? image_wh[2 * c + 1] : H;
+    ProjOut o;
+    if (!proj_fwd_body(m3, V9, 1, nullptr, nullptr, viewmats + (size_t)c * 16, Ks + (size_t)c * 9, 0u, Wc, Hc,
+                       eps2d, near_plane, far_plane, radius_clip, ortho != 0, o)) {
+        radii[idx] = 0;
+        return;
+    }
+    radii[idx] = o.radius;
+    means2d[idx * 2] = o.m2[0];
+    means2d[idx * 2 + 1] = o.m2[1];
+    depths[idx] = o.depth;
+    conics[idx * 3] = o.conic[0];
+    conics[idx * 3 + 1] = o.conic[1];
+    conics[idx * 3 + 2] = o.conic[2];
+}
+
 template <int D, bool QUAT>
 __global__ __launch_bounds__(kThreads) void beta_project_fwd_kernel(
     int64_t N, uint32_t C, const float *__restrict__ xyz, const float *__restrict__ mean_rest,
@@ -104,36 +141,57 @@ __global__ __launch_bounds__(kThreads) void beta_project_fwd_kernel(
     int32_t W, int32_t H, const int32_t *__restrict__ image_wh, float eps2d, float near_plane, float far_plane,
     float radius_clip, int ortho, int32_t *__restrict__ radii, float *__restrict__ means2d,
     float *__restrict__ depths, float *__restrict__ conics, float *__restrict__ opacities,
-    float *__restrict__ betas) {
-    constexpr int CQ = D - 3;
+    float *__restrict__ betas, int64_t row_stride) {
     const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
     BetaPrim<D> bp;
     beta_prim_setup<D, QUAT>(n, xyz, mean_rest, scale_raw, ltri, opacity_raw, beta_raw, quats, bp);
+    for (uint32_t c = 0; c < C; ++c)
+        beta_view_fwd<D>(bp, c, (int64_t)c * row_stride + n, cam_pos, timestamps, viewmats, Ks, W, H, image_wh,
+                         eps2d, near_plane, far_plane, radius_clip, ortho, radii, means2d, depths, conics,
+                         opacities, betas);
+}
+
+// Views of several resident scenes in one batch: view c projects the scene
+// table[c] names into row c (entries c * row_stride + n).  The launch for D
+// takes the views of D-dim scenes and the empty slots (N = 0) when D = 6;
+// entries past a view's N get radius 0 (culled).  A primitive's
+// view-independent state is kept while consecutive views share the scene.
+// The colours of each view's scene are copied into its row as well, so the
+// blend reads one [C, row_stride, 3] array.
+template <int D>
+__global__ __launch_bounds__(kThreads) void beta_project_fwd_table_kernel(
+    uint32_t C, int64_t row_stride, const ubs_view_scene *__restrict__ table, const float *__restrict__ cam_pos,
+    const float *__restrict__ timestamps, const float *__restrict__ viewmats, const float *__restrict__ Ks,
+    const int32_t *__restrict__ image_wh, float eps2d, float near_plane, float far_plane, float radius_clip,
+    int32_t *__restrict__ radii, float *__restrict__ means2d, float *__restrict__ depths,
+    float *__restrict__ conics, float *__restrict__ opacities, float *__restrict__ betas,
+    float *__restrict__ colors) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= row_stride) return;
+    BetaPrim<D> bp;
+    const float *held = nullptr;  // the scene whose state bp holds
     for (uint32_t c = 0; c < C; ++c) {
-        const int64_t idx = (int64_t)c * N + n;
-        float q[CQ];
-        beta_query<D>(bp, cam_pos, timestamps, c, q);
-        CondRegs<CQ> st;
-        cond_setup<CQ>(bp.mean, bp.Sig, bp.beta + 1, q, st);
-        float m3[3], V9[9], oc;
-        cond_fwd_body<CQ>(st, bp.mean, bp.opac, m3, V9, &oc);
-        opacities[idx] = oc;
-        betas[idx] = bp.beta[0];
-        const int32_t Wc = image_wh ? image_wh[2 * c] : W, Hc = image_wh ? image_wh[2 * c + 1] : H;
-        ProjOut o;
-        if (!proj_fwd_body(m3, V9, 1, nullptr, nullptr, viewmats + (size_t)c * 16, Ks + (size_t)c * 9, 0u, Wc, Hc,
-                           eps2d, near_plane, far_plane, radius_clip, ortho != 0, o)) {
+        const ubs_view_scene vs = table[c];
+        if ((vs.N > 0 ? vs.D : 6) != D) continue;
+        const int64_t idx = (int64_t)c * row_stride + n;
+        if (n >= vs.N) {
             radii[idx] = 0;
             continue;
         }
-        radii[idx] = o.radius;
-        means2d[idx * 2] = o.m2[0];
-        means2d[idx * 2 + 1] = o.m2[1];
-        depths[idx] = o.depth;
-        conics[idx * 3] = o.conic[0];
-        conics[idx * 3 + 1] = o.conic[1];
-        conics[idx * 3 + 2] = o.conic[2];
+        if (vs.xyz != held) {
+            if (vs.quats)
+                beta_prim_setup<D, true>(n, vs.xyz, vs.mean_rest, vs.scale_raw, vs.l_triangle, vs.opacity_raw,
+                                         vs.beta_raw, vs.quats, bp);
+            else
+                beta_prim_setup<D, false>(n, vs.xyz, vs.mean_rest, vs.scale_raw, vs.l_triangle, vs.opacity_raw,
+                                          vs.beta_raw, nullptr, bp);
+            held = vs.xyz;
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) colors[idx * 3 + k] = vs.rgb[n * 3 + k];
+        beta_view_fwd<D>(bp, c, idx, cam_pos, timestamps, viewmats, Ks, 0, 0, image_wh, eps2d, near_plane,
+                         far_plane, radius_clip, 0, radii, means2d, depths, conics, opacities, betas);
     }
 }
 
@@ -168,11 +226,16 @@ __global__ __launch_bounds__(kThreads, D == 7 ? UBS_PROJ_BWD_WAVES_7 : UBS_PROJ_
     beta_prim_setup<D, QUAT>(n, xyz, mean_rest, scale_raw, ltri, opacity_raw, beta_raw, quats, bp);
     // gradients of the activated mean, covariance, opacity and Beta
     // parameters, summed over the views
-    float gmean[D], gSig[D * D], gop = 0.f, gbeta[D - 2];
+    // The covariance VJP reads its gradient G only as G[a][b] + G[b][a]
+    // (Sigma = L L^T), so the views' gradients are summed as that symmetric
+    // part, upper triangle only: gU[a][b] = sum of G[a][b] + G[b][a] for
+    // a < b and of G[a][a] on the diagonal (28 sums instead of 49 for 7-D).
+    constexpr int NU = D * (D + 1) / 2;
+    float gmean[D], gU[NU], gop = 0.f, gbeta[D - 2];
 #pragma unroll
     for (int k = 0; k < D; ++k) gmean[k] = 0.f;
 #pragma unroll
-    for (int k = 0; k < D * D; ++k) gSig[k] = 0.f;
+    for (int k = 0; k < NU; ++k) gU[k] = 0.f;
 #pragma unroll
     for (int k = 0; k < D - 2; ++k) gbeta[k] = 0.f;
     for (uint32_t c = 0; c < C; ++c) {
@@ -199,16 +262,40 @@ __global__ __launch_bounds__(kThreads, D == 7 ? UBS_PROJ_BWD_WAVES_7 : UBS_PROJ_
         cond_bwd_body<CQ>(st, bp.opac, vmean, Gv, v_opacities[idx], exact, gm, gV, &go, gb);
 #pragma unroll
         for (int k = 0; k < D; ++k) gmean[k] += gm[k];
+        {
+            int u = 0;
 #pragma unroll
-        for (int k = 0; k < D * D; ++k) gSig[k] += gV[k];
+            for (int a = 0; a < D; ++a)
+#pragma unroll
+                for (int b = a; b < D; ++b, ++u)
+                    gU[u] += a == b ? gV[a * D + a] : gV[a * D + b] + gV[b * D + a];
+        }
         gop += go;
         gbeta[0] += v_betas[idx];
 #pragma unroll
         for (int k = 0; k < CQ; ++k) gbeta[1 + k] += gb[k];
     }
-    // covariance, rotation and activation VJPs, once per primitive
+    // covariance, rotation and activation VJPs, once per primitive.  The
+    // rotation, scales and L factor are set up again here rather than kept
+    // through the view loop (which reads only the mean, Sigma, opacity and
+    // Beta parameters): the same values, ~70 fewer registers live in the loop
+    BetaPrim<D> bq;
+    beta_prim_setup<D, QUAT>(n, xyz, mean_rest, scale_raw, ltri, opacity_raw, beta_raw, quats, bq);
+    // a G whose G[a][b] + G[b][a] is gU exactly: the triangle above the
+    // diagonal, zeros below, the diagonal as summed (x + 0 and d + d are exact)
+    float gSig[D * D];
+    {
+        int u = 0;
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+#pragma unroll
+            for (int b = a; b < D; ++b, ++u) {
+                gSig[a * D + b] = gU[u];
+                if (b != a) gSig[b * D + a] = 0.f;
+            }
+    }
     float dR[9], dS[D];
-    covar_bwd_body<D>(bp.R, bp.scale, bp.L, bp.P, false, gSig, true, nullptr, FULL_M - 3, dR, dS,
+    covar_bwd_body<D>(bq.R, bq.scale, bq.L, bq.P, false, gSig, true, nullptr, FULL_M - 3, dR, dS,
                       v_ltri + n * FULL_M);
     if constexpr (QUAT) {
         float qv[4], vq[4];
@@ -246,6 +333,15 @@ using namespace ubs;
 
 extern "C" {
 
+static int beta_project_fwd_impl(int64_t N, int64_t D, uint32_t C, const float *xyz, const float *mean_rest,
+                                 const float *scale_raw, const float *l_triangle, const float *opacity_raw,
+                                 const float *beta_raw, const float *quats, const float *cam_pos,
+                                 const float *timestamps, const float *viewmats, const float *Ks, int32_t width,
+                                 int32_t height, const int32_t *image_wh, float eps2d, float near_plane,
+                                 float far_plane, float radius_clip, int ortho, int32_t *radii, float *means2d,
+                                 float *depths, float *conics, float *opacities, float *betas, int64_t row_stride,
+                                 void *stream);
+
 int ubs_beta_project_fwd(int64_t N, int64_t D, uint32_t C, const float *xyz, const float *mean_rest,
                          const float *scale_raw, const float *l_triangle, const float *opacity_raw,
                          const float *beta_raw, const float *quats, const float *cam_pos, const float *timestamps,
@@ -253,8 +349,37 @@ int ubs_beta_project_fwd(int64_t N, int64_t D, uint32_t C, const float *xyz, con
                          const int32_t *image_wh, float eps2d, float near_plane, float far_plane, float radius_clip,
                          int ortho, int32_t *radii, float *means2d, float *depths, float *conics, float *opacities,
                          float *betas, void *stream) {
+    return beta_project_fwd_impl(N, D, C, xyz, mean_rest, scale_raw, l_triangle, opacity_raw, beta_raw, quats,
+                                 cam_pos, timestamps, viewmats, Ks, width, height, image_wh, eps2d, near_plane,
+                                 far_plane, radius_clip, ortho, radii, means2d, depths, conics, opacities, betas, N,
+                                 stream);
+}
+
+int ubs_beta_project_fwd_rows(int64_t N, int64_t D, uint32_t C, const float *xyz, const float *mean_rest,
+                              const float *scale_raw, const float *l_triangle, const float *opacity_raw,
+                              const float *beta_raw, const float *quats, const float *cam_pos,
+                              const float *timestamps, const float *viewmats, const float *Ks, int32_t width,
+                              int32_t height, const int32_t *image_wh, float eps2d, float near_plane,
+                              float far_plane, float radius_clip, int ortho, int32_t *radii, float *means2d,
+                              float *depths, float *conics, float *opacities, float *betas, int64_t row_stride,
+                              void *stream) {
+    UBS_REQUIRE(row_stride >= N, "row_stride %lld < N %lld", (long long)row_stride, (long long)N);
+    return beta_project_fwd_impl(N, D, C, xyz, mean_rest, scale_raw, l_triangle, opacity_raw, beta_raw, quats,
+                                 cam_pos, timestamps, viewmats, Ks, width, height, image_wh, eps2d, near_plane,
+                                 far_plane, radius_clip, ortho, radii, means2d, depths, conics, opacities, betas,
+                                 row_stride, stream);
+}
+
+static int beta_project_fwd_impl(int64_t N, int64_t D, uint32_t C, const float *xyz, const float *mean_rest,
+                                 const float *scale_raw, const float *l_triangle, const float *opacity_raw,
+                                 const float *beta_raw, const float *quats, const float *cam_pos,
+                                 const float *timestamps, const float *viewmats, const float *Ks, int32_t width,
+                                 int32_t height, const int32_t *image_wh, float eps2d, float near_plane,
+                                 float far_plane, float radius_clip, int ortho, int32_t *radii, float *means2d,
+                                 float *depths, float *conics, float *opacities, float *betas, int64_t row_stride,
+                                 void *stream) {
     UBS_REQUIRE(D == 6 || D == 7, "the fused Beta projection is defined for D = 6 or 7 (got %lld)", (long long)D);
-    UBS_REQUIRE(N >= 0 && (uint64_t)N * C < (1ull << 31), "C*N must be < 2^31");
+    UBS_REQUIRE(N >= 0 && (uint64_t)row_stride * C < (1ull << 31), "C*N must be < 2^31");
     if (N == 0 || C == 0) return UBS_OK;
     UBS_REQUIRE(xyz && mean_rest && scale_raw && l_triangle && opacity_raw && beta_raw && cam_pos && viewmats &&
                     Ks && radii && means2d && depths && conics && opacities && betas,
@@ -265,7 +390,7 @@ int ubs_beta_project_fwd(int64_t N, int64_t D, uint32_t C, const float *xyz, con
     beta_project_fwd_kernel<DD, QQ><<<grid, kThreads, 0, st>>>(                                                  \
         N, C, xyz, mean_rest, scale_raw, l_triangle, opacity_raw, beta_raw, quats, cam_pos, timestamps, viewmats, \
         Ks, width, height, image_wh, eps2d, near_plane, far_plane, radius_clip, ortho, radii, means2d, depths,    \
-        conics, opacities, betas)
+        conics, opacities, betas, row_stride)
     if (D == 6) {
         if (quats) UBS_BP_FWD(6, true);
         else UBS_BP_FWD(6, false);
@@ -275,6 +400,29 @@ int ubs_beta_project_fwd(int64_t N, int64_t D, uint32_t C, const float *xyz, con
     }
 #undef UBS_BP_FWD
     UBS_LAUNCH_CHECK("beta_project_fwd_kernel");
+    return UBS_OK;
+}
+
+int ubs_beta_project_fwd_table(uint32_t C, int64_t row_stride, const ubs_view_scene *table, const float *cam_pos,
+                               const float *timestamps, const float *viewmats, const float *Ks,
+                               const int32_t *image_wh, float eps2d, float near_plane, float far_plane,
+                               float radius_clip, int32_t *radii, float *means2d, float *depths, float *conics,
+                               float *opacities, float *betas, float *colors, void *stream) {
+    UBS_REQUIRE(row_stride >= 0 && (uint64_t)row_stride * C < (1ull << 31), "C*row_stride must be < 2^31");
+    if (row_stride == 0 || C == 0) return UBS_OK;
+    UBS_REQUIRE(table && cam_pos && viewmats && Ks && image_wh && radii && means2d && depths && conics && opacities &&
+                    betas && colors,
+                "null pointer");
+    hipStream_t st = as_stream(stream);
+    const unsigned grid = grid_for(row_stride);
+    beta_project_fwd_table_kernel<6><<<grid, kThreads, 0, st>>>(
+        C, row_stride, table, cam_pos, timestamps, viewmats, Ks, image_wh, eps2d, near_plane, far_plane, radius_clip,
+        radii, means2d, depths, conics, opacities, betas, colors);
+    UBS_LAUNCH_CHECK("beta_project_fwd_table_kernel<6>");
+    beta_project_fwd_table_kernel<7><<<grid, kThreads, 0, st>>>(
+        C, row_stride, table, cam_pos, timestamps, viewmats, Ks, image_wh, eps2d, near_plane, far_plane, radius_clip,
+        radii, means2d, depths, conics, opacities, betas, colors);
+    UBS_LAUNCH_CHECK("beta_project_fwd_table_kernel<7>");
     return UBS_OK;
 }
 

@@ -280,6 +280,13 @@ __device__ __forceinline__ int wave_max_i32(int v) {
     return v;
 }
 
+// Sum of a per-lane count over the wave (measurement counters; not hot).
+__device__ __forceinline__ unsigned long long wave_sum_u64(uint32_t v) {
+    unsigned long long s = v;
+    for (int o = 32; o > 0; o >>= 1) s += (unsigned long long)__shfl_xor((long long)s, o, 64);
+    return s;
+}
+
 __device__ __forceinline__ unsigned lane_id() { return __lane_id(); }
 
 }  // namespace ubs

@@ -214,7 +214,8 @@ __global__ __launch_bounds__(256) void isect_emit_kernel(uint32_t C, uint32_t N,
                                                          uint32_t th, uint32_t tile_bits,
                                                          int64_t *__restrict__ isect_ids,
                                                          int32_t *__restrict__ flatten_ids,
-                                                         const int32_t *__restrict__ order) {
+                                                         const int32_t *__restrict__ order,
+                                                         int64_t cap = INT64_MAX) {
     const uint64_t CN = (uint64_t)C * N;
     const uint64_t pos = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;  // place in emission order
     const int lane = (int)(threadIdx.x & 63);
@@ -255,7 +256,7 @@ __global__ __launch_bounds__(256) void isect_emit_kernel(uint32_t C, uint32_t N,
         const uint32_t py0 = __shfl(y0, p, 64);
         const int32_t pfid = __shfl(fid, p, 64);
         const int64_t pkey = __shfl(key_hi, p, 64);
-        if (k < total) {
+        if (k < total && wave_start + (int64_t)k < cap) {  // entries past a capacity are not stored
             const uint32_t local = k - pe;
             const uint32_t q = local / pbw;
             const int64_t tile = (int64_t)(py0 + q) * tw + (px0 + (local - q * pbw));
@@ -267,9 +268,17 @@ __global__ __launch_bounds__(256) void isect_emit_kernel(uint32_t C, uint32_t N,
 
 // offsets[k] = number of sorted isects whose (cam, tile) index is < k.
 // reference: isect_tiles.cu:287-333
-__global__ void isect_offsets_kernel(int64_t n, const int64_t *__restrict__ ids, uint32_t tile_bits,
-                                     int64_t n_tiles, int64_t total_tiles, int32_t *__restrict__ offsets) {
+// n_dev (may be null): the count in device memory, clamped to n_cap (the
+// grid's size); a device count of 0 zero-fills the offsets.
+__global__ void isect_offsets_kernel(int64_t n_cap, const int64_t *__restrict__ n_dev,
+                                     const int64_t *__restrict__ ids, uint32_t tile_bits, int64_t n_tiles,
+                                     int64_t total_tiles, int32_t *__restrict__ offsets) {
+    const int64_t n = n_dev ? min(max(*n_dev, (int64_t)0), n_cap) : n_cap;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n == 0) {
+        for (int64_t k = i; k < total_tiles; k += (int64_t)gridDim.x * blockDim.x) offsets[k] = 0;
+        return;
+    }
     if (i >= n) return;
     const uint64_t mask = (1ull << tile_bits) - 1ull;
     const uint64_t hi = (uint64_t)ids[i] >> 32;
@@ -400,16 +409,35 @@ int ubs_isect_scan_ordered(uint32_t C, uint32_t N, const int32_t *tpg, const int
     return UBS_OK;
 }
 
+static int emit_ordered(uint32_t C, uint32_t N, const float *means2d, const int32_t *radii, const float *depths,
+                        const int32_t *order, const int64_t *cum, uint32_t ts, uint32_t tw, uint32_t th,
+                        int64_t *isect_ids, int32_t *flatten_ids, int64_t cap, void *stream);
+
 int ubs_isect_tiles_emit_ordered(uint32_t C, uint32_t N, const float *means2d, const int32_t *radii,
                                  const float *depths, const int32_t *order, const int64_t *cum, uint32_t ts,
                                  uint32_t tw, uint32_t th, int64_t *isect_ids, int32_t *flatten_ids, void *stream) {
+    return emit_ordered(C, N, means2d, radii, depths, order, cum, ts, tw, th, isect_ids, flatten_ids, INT64_MAX,
+                        stream);
+}
+
+int ubs_isect_tiles_emit_ordered_cap(uint32_t C, uint32_t N, const float *means2d, const int32_t *radii,
+                                     const float *depths, const int32_t *order, const int64_t *cum, uint32_t ts,
+                                     uint32_t tw, uint32_t th, int64_t *isect_ids, int32_t *flatten_ids,
+                                     int64_t cap, void *stream) {
+    UBS_REQUIRE(cap >= 0, "cap must be >= 0");
+    return emit_ordered(C, N, means2d, radii, depths, order, cum, ts, tw, th, isect_ids, flatten_ids, cap, stream);
+}
+
+static int emit_ordered(uint32_t C, uint32_t N, const float *means2d, const int32_t *radii, const float *depths,
+                        const int32_t *order, const int64_t *cum, uint32_t ts, uint32_t tw, uint32_t th,
+                        int64_t *isect_ids, int32_t *flatten_ids, int64_t cap, void *stream) {
     const uint64_t total = (uint64_t)C * N;
     if (total == 0) return UBS_OK;
     UBS_REQUIRE(total < (1ull << 31), "C*N must be < 2^31");
     UBS_REQUIRE(means2d && radii && depths && order && cum && isect_ids && flatten_ids, "null pointer");
     const uint32_t tile_bits = bit_width_u32(tw * th);
     isect_emit_kernel<<<grid_for((int64_t)total), kThreads, 0, as_stream(stream)>>>(
-        C, N, means2d, radii, depths, cum, ts, tw, th, tile_bits, isect_ids, flatten_ids, order);
+        C, N, means2d, radii, depths, cum, ts, tw, th, tile_bits, isect_ids, flatten_ids, order, cap);
     UBS_LAUNCH_CHECK("isect_emit_kernel");
     return UBS_OK;
 }
@@ -423,8 +451,21 @@ int ubs_isect_offset_encode(int64_t n, const int64_t *ids, uint32_t C, uint32_t 
     if (n == 0) return zero_async(offsets, (size_t)total_tiles * sizeof(int32_t), st);
     UBS_REQUIRE(ids, "null pointer");
     UBS_REQUIRE(n < (1ll << 31), "n_isects must be < 2^31");
-    isect_offsets_kernel<<<grid_for(n), kThreads, 0, st>>>(n, ids, bit_width_u32((uint32_t)n_tiles), n_tiles,
-                                                           total_tiles, offsets);
+    isect_offsets_kernel<<<grid_for(n), kThreads, 0, st>>>(n, nullptr, ids, bit_width_u32((uint32_t)n_tiles),
+                                                           n_tiles, total_tiles, offsets);
+    UBS_LAUNCH_CHECK("isect_offsets_kernel");
+    return UBS_OK;
+}
+
+int ubs_isect_offset_encode_dev(int64_t n_cap, const int64_t *n_dev, const int64_t *ids, uint32_t C, uint32_t tw,
+                                uint32_t th, int32_t *offsets, void *stream) {
+    hipStream_t st = as_stream(stream);
+    const int64_t n_tiles = (int64_t)tw * th, total_tiles = (int64_t)C * n_tiles;
+    if (total_tiles == 0) return UBS_OK;
+    UBS_REQUIRE(offsets && n_dev && (ids || n_cap == 0), "null pointer");
+    UBS_REQUIRE(n_cap >= 0 && n_cap < (1ll << 31), "n_cap must be in [0, 2^31)");
+    isect_offsets_kernel<<<grid_for(std::max<int64_t>(n_cap, 1)), kThreads, 0, st>>>(
+        n_cap, n_dev, ids, bit_width_u32((uint32_t)n_tiles), n_tiles, total_tiles, offsets);
     UBS_LAUNCH_CHECK("isect_offsets_kernel");
     return UBS_OK;
 }

@@ -159,36 +159,60 @@ __device__ __forceinline__ uint32_t subblock_mask_exact(float mx, float my, floa
 // Packed per-(camera, primitive) records of the blend: r0 = (x, y, a, b),
 // r1 = (c, opacity, beta, 0) for the forward inputs, and the same layout for
 // the backward's gradients (v_x, v_y, v_a, v_b | v_c, v_opacity, v_beta, 0).
-// One 32-byte record per intersection gather instead of four arrays.
+// One record per intersection gather instead of five arrays.  With at most
+// 4 colour channels blended in one pass the record is "wide": 64 bytes, r2 =
+// the colours (zero-padded to 4) and r3 unused, so one aligned 64-byte read
+// (and, for the gradients, one 64-byte line of atomics) serves a primitive
+// where the narrow 32-byte record and a separate 12-byte colour gather touch
+// two lines.
 constexpr size_t kRecordFloats = 8;
+constexpr size_t kWideRecordFloats = 16;
 
 inline size_t raster_workspace_bytes(uint32_t C, uint32_t N, int backward) {
-    const size_t one = ((size_t)C * N * kRecordFloats * sizeof(float) + 255) & ~(size_t)255;
+    const size_t one = ((size_t)C * N * kWideRecordFloats * sizeof(float) + 255) & ~(size_t)255;
     return backward ? 2 * one : one;
 }
 
 static __global__ void raster_pack_kernel(uint64_t CN, const float *__restrict__ means2d,
                                           const float *__restrict__ conics, const float *__restrict__ opacities,
-                                          const float *__restrict__ betas, float4 *__restrict__ rec) {
+                                          const float *__restrict__ betas, float4 *__restrict__ rec,
+                                          const float *__restrict__ colors = nullptr, uint32_t channels = 0) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= CN) return;
     const float2 xy = reinterpret_cast<const float2 *>(means2d)[i];
-    rec[2 * i] = make_float4(xy.x, xy.y, conics[3 * i], conics[3 * i + 1]);
-    rec[2 * i + 1] = make_float4(conics[3 * i + 2], opacities[i], betas[i], 0.f);
+    const float4 r0 = make_float4(xy.x, xy.y, conics[3 * i], conics[3 * i + 1]);
+    const float4 r1 = make_float4(conics[3 * i + 2], opacities[i], betas[i], 0.f);
+    if (colors) {  // wide
+        const float *cp = colors + i * channels;
+        rec[4 * i] = r0;
+        rec[4 * i + 1] = r1;
+        rec[4 * i + 2] = make_float4(cp[0], channels > 1 ? cp[1] : 0.f, channels > 2 ? cp[2] : 0.f,
+                                     channels > 3 ? cp[3] : 0.f);
+    } else {
+        rec[2 * i] = r0;
+        rec[2 * i + 1] = r1;
+    }
 }
 
 static __global__ void raster_unpack_grads_kernel(uint64_t CN, const float4 *__restrict__ vrec,
                                                   float *__restrict__ v_means2d, float *__restrict__ v_conics,
-                                                  float *__restrict__ v_opacities, float *__restrict__ v_betas) {
+                                                  float *__restrict__ v_opacities, float *__restrict__ v_betas,
+                                                  float *__restrict__ v_colors = nullptr, uint32_t channels = 0) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= CN) return;
-    const float4 r0 = vrec[2 * i], r1 = vrec[2 * i + 1];
+    const uint64_t stride = v_colors ? 4 : 2;  // float4s per record: wide or narrow
+    const float4 r0 = vrec[stride * i], r1 = vrec[stride * i + 1];
     reinterpret_cast<float2 *>(v_means2d)[i] = make_float2(r0.x, r0.y);
     v_conics[3 * i] = r0.z;
     v_conics[3 * i + 1] = r0.w;
     v_conics[3 * i + 2] = r1.x;
     v_opacities[i] = r1.y;
     v_betas[i] = r1.z;
+    if (v_colors) {
+        const float4 r2 = vrec[stride * i + 2];
+        const float c4[4] = {r2.x, r2.y, r2.z, r2.w};
+        for (uint32_t k = 0; k < channels; ++k) v_colors[i * channels + k] = c4[k];
+    }
 }
 
 // Transmittance checkpoints (optional): the forward stores every pixel's T
@@ -221,6 +245,12 @@ inline int pick_chunk(uint32_t remaining, uint32_t tile_size) {
     for (int k : kChunkSizes)
         if ((uint32_t)k <= remaining && (uint32_t)k <= cap) return k;
     return 1;
+}
+
+// Whether the blend of `channels` channels uses wide records (one pass over
+// all channels, at most 4).
+inline bool raster_wide_records(uint32_t channels, uint32_t tile_size) {
+    return channels <= 4 && (int)channels == pick_chunk(channels, tile_size);
 }
 
 }  // namespace ubs

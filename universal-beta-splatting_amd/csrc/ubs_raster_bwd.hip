@@ -31,10 +31,15 @@
 
 namespace ubs {
 
-template <int TS, int CDIM, bool F64T>
 #ifndef UBS_BWD_WAVES_PER_EU
 #define UBS_BWD_WAVES_PER_EU 1
 #endif
+// STATS (measurement builds only; the library's kernels have it off) counts
+// into stats[0..6]: (wave, primitive) evaluations, lane evaluations at or
+// before the pixel's last contributor, of those that contribute, pair steps
+// that run the wave reduce-scatter, staged entries, and the reduce-scatter
+// steps with at most 4 and at most 16 contributing lanes.
+template <int TS, int CDIM, bool F64T, bool STATS = false>
 __global__ __launch_bounds__(TS *TS, UBS_BWD_WAVES_PER_EU) void raster_bwd_kernel(
     uint32_t C, uint32_t N, int64_t n_isects, uint32_t channels, uint32_t coff,
     const float *__restrict__ means2d, const float *__restrict__ conics, const float *__restrict__ colors,
@@ -44,8 +49,8 @@ __global__ __launch_bounds__(TS *TS, UBS_BWD_WAVES_PER_EU) void raster_bwd_kerne
     const float *__restrict__ render_alphas, const int32_t *__restrict__ last_ids, const float *__restrict__ T_final,
     const float *__restrict__ v_render_colors, const float *__restrict__ v_render_alphas,
     float *__restrict__ v_means2d, float *__restrict__ v_conics, float *__restrict__ v_colors,
-    float *__restrict__ v_opacities, float *__restrict__ v_betas, const float4 *__restrict__ rec,
-    float *__restrict__ vrec, const float *__restrict__ T_ckpt) {
+    float *__restrict__ v_opacities, float *__restrict__ v_betas, const float4 *__restrict__ rec, int wide_rec,
+    float *__restrict__ vrec, const float *__restrict__ T_ckpt, unsigned long long *__restrict__ stats = nullptr) {
     constexpr int BS = TS * TS;
     constexpr int SB = TS / 8;
     constexpr int NW = BS / 64;
@@ -78,8 +83,11 @@ __global__ __launch_bounds__(TS *TS, UBS_BWD_WAVES_PER_EU) void raster_bwd_kerne
     const bool first_chunk = (coff == 0);
 
     const int64_t tbase = (int64_t)cam * th * tw;
-    const int32_t range_start = tile_offsets[tbase + tile];
-    const int32_t range_end = (cam == C - 1 && tile == th * tw - 1) ? (int32_t)n_isects : tile_offsets[tbase + tile + 1];
+    // a valid sort gives ordered offsets within [0, n_isects]; the clamps keep
+    // the lists of a sort that gave up on a look-back wait inside the buffers
+    const int32_t off_hi = (cam == C - 1 && tile == th * tw - 1) ? (int32_t)n_isects : tile_offsets[tbase + tile + 1];
+    const int32_t range_start = min(max(tile_offsets[tbase + tile], 0), (int32_t)n_isects);
+    const int32_t range_end = min(max(off_hi, range_start), (int32_t)n_isects);
     const float px = (float)j + 0.5f, py = (float)i + 0.5f;
     const float x0 = (float)(tx * TS), y0 = (float)(ty * TS);
 
@@ -124,6 +132,7 @@ __global__ __launch_bounds__(TS *TS, UBS_BWD_WAVES_PER_EU) void raster_bwd_kerne
         my_idx[p] = grp >= 0 ? wave_total_index<2 * NV>(grp, lane >> 4) : -1;
     }
 
+    uint32_t st_steps = 0, st_live = 0, st_valid = 0, st_rs = 0, st_staged = 0, st_k4 = 0, st_k16 = 0;
     // batches are the forward's, [range_start + k BS, ...), taken last first
     const int32_t nb = range_end > range_start ? (range_end - range_start + BS - 1) / BS : 0;
     for (int32_t b = 0; b < nb; ++b) {
@@ -131,15 +140,26 @@ __global__ __launch_bounds__(TS *TS, UBS_BWD_WAVES_PER_EU) void raster_bwd_kerne
         const int32_t batch_lo = range_start + kb * BS;
         const int32_t batch_end = min(range_end, batch_lo + BS) - 1;  // isect index of entry t is batch_end - t
         const int32_t cnt = batch_end + 1 - batch_lo;
+        if constexpr (STATS) st_staged += (uint32_t)(tid < cnt);
         // with checkpoints, a pixel that has entries here restarts from the
         // forward's T after this batch (the forward ran this batch for it)
         if (T_ckpt && inside && bin_final >= batch_lo)
             T = (TT)T_ckpt[ckpt_slot(range_start, logical, kb, BS) * BS + tid];
         if (tid < cnt) {
-            const int32_t g = flatten_ids[batch_end - tid];
+            // an id outside [0, C N) (only from a sort that gave up) stages
+            // primitive 0 with an empty sub-block mask: never blended
+            const int32_t g_raw = flatten_ids[batch_end - tid];
+            const bool g_ok = (uint64_t)(uint32_t)g_raw < (uint64_t)C * N;
+            const int32_t g = g_ok ? g_raw : 0;
             float2 xy;
             float ca, cb, cc, op, be;
-            if (rec) {  // one 32-byte record per (camera, primitive)
+            float4 cw = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (rec && wide_rec) {  // one 64-byte record per (camera, primitive), colours included
+                const float4 r0 = rec[4 * (int64_t)g], r1 = rec[4 * (int64_t)g + 1];
+                cw = rec[4 * (int64_t)g + 2];
+                xy = make_float2(r0.x, r0.y);
+                ca = r0.z; cb = r0.w; cc = r1.x; op = r1.y; be = r1.z;
+            } else if (rec) {  // one 32-byte record per (camera, primitive)
                 const float4 r0 = rec[2 * (int64_t)g], r1 = rec[2 * (int64_t)g + 1];
                 xy = make_float2(r0.x, r0.y);
                 ca = r0.z; cb = r0.w; cc = r1.x; op = r1.y; be = r1.z;
@@ -152,15 +172,16 @@ __global__ __launch_bounds__(TS *TS, UBS_BWD_WAVES_PER_EU) void raster_bwd_kerne
             s_geo2[tid] = make_float4(cc, op, be, __int_as_float(g));
             const float *cp = colors + (int64_t)g * channels + coff;
             if constexpr (kColVec) {
-                s_colv[tid] = make_float4(cp[0], CDIM > 1 ? cp[1] : 0.f, CDIM > 2 ? cp[2] : 0.f,
-                                          CDIM > 3 ? cp[3] : 0.f);
+                s_colv[tid] = (rec && wide_rec) ? cw
+                                                : make_float4(cp[0], CDIM > 1 ? cp[1] : 0.f, CDIM > 2 ? cp[2] : 0.f,
+                                                              CDIM > 3 ? cp[3] : 0.f);
             } else {
 #pragma unroll
                 for (int k = 0; k < CDIM; ++k) s_col[k][tid] = cp[k];
             }
             float hx, hy;
             support_box(ca, cb, cc, hx, hy);
-            s_mask[tid] = subblock_mask_exact<SB>(xy.x, xy.y, ca, cb, cc, hx, hy, x0, y0);
+            s_mask[tid] = g_ok ? subblock_mask_exact<SB>(xy.x, xy.y, ca, cb, cc, hx, hy, x0, y0) : 0u;
         }
         __syncthreads();
         // ---- back-to-front over this sub-block's primitives of the batch
@@ -286,6 +307,16 @@ __global__ __launch_bounds__(TS *TS, UBS_BWD_WAVES_PER_EU) void raster_bwd_kerne
                 const bool c0v = pixel_grads(r0, t0, v, 0, true);
                 const bool c1v = pixel_grads(r1, t1, v, NV, two);
                 const bool any = __ballot(c0v || c1v) != 0ull;
+                if constexpr (STATS) {
+                    st_steps += two ? 2u : 1u;
+                    st_live += (uint32_t)(t0 >= t_min) + (uint32_t)(two && t1 >= t_min);
+                    st_valid += (uint32_t)c0v + (uint32_t)c1v;
+                    st_rs += (uint32_t)any;
+                    // lanes with a contribution to either primitive of the step
+                    const uint32_t k = (uint32_t)__popcll(__ballot(c0v || c1v));
+                    st_k4 += (uint32_t)(any && k <= 4);
+                    st_k16 += (uint32_t)(any && k <= 16);
+                }
                 if (!any) continue;
                 // wave totals, scattered over the rows and banks; one ds_add_f32
                 // per pass adds them into the primitives' accumulators
@@ -309,8 +340,10 @@ __global__ __launch_bounds__(TS *TS, UBS_BWD_WAVES_PER_EU) void raster_bwd_kerne
         if (vrec) {
             // packed records: lane -> (entry, value) with 8 consecutive lanes on
             // one primitive's 32-byte record, so one atomic instruction covers 8
-            // records (8 floats each) instead of 64 scattered floats
+            // records (8 floats each) instead of 64 scattered floats; with wide
+            // records the colour totals go to the same 64-byte record
             constexpr int kCv = kColVec ? 4 : CDIM;  // lanes per primitive for the colours
+            const int64_t rstride = wide_rec ? (int64_t)kWideRecordFloats : (int64_t)kRecordFloats;
             for (int k = tid; k < cnt * 8; k += BS) {
                 const int t = k >> 3, q = k & 7;
                 const float4 gp = s_geo2[t];  // c, opacity, beta, id
@@ -330,14 +363,15 @@ __global__ __launch_bounds__(TS *TS, UBS_BWD_WAVES_PER_EU) void raster_bwd_kerne
                           : q == 5        ? acc
                                           : 0.f;
                 }
-                if (val != 0.f) atomic_add_f32(vrec + (int64_t)__float_as_int(gp.w) * kRecordFloats + q, val);
+                if (val != 0.f) atomic_add_f32(vrec + (int64_t)__float_as_int(gp.w) * rstride + q, val);
             }
             for (int k = tid; k < cnt * kCv; k += BS) {
                 const int t = k / kCv, q = k % kCv;
                 if (q < CDIM) {
                     const float val = s_acc[7 + q][t];
+                    const int64_t g = __float_as_int(s_geo2[t].w);
                     if (val != 0.f)
-                        atomic_add_f32(v_colors + (int64_t)__float_as_int(s_geo2[t].w) * channels + coff + q, val);
+                        atomic_add_f32(wide_rec ? vrec + g * rstride + 8 + q : v_colors + g * channels + coff + q, val);
                 }
             }
             __syncthreads();
@@ -375,6 +409,16 @@ __global__ __launch_bounds__(TS *TS, UBS_BWD_WAVES_PER_EU) void raster_bwd_kerne
         }
         __syncthreads();
     }
+    if constexpr (STATS) {
+        const uint32_t v[7] = {st_steps, st_live, st_valid, st_rs, st_staged, st_k4, st_k16};
+#pragma unroll
+        for (int q = 0; q < 7; ++q) {
+            // steps, reduce-scatters and their lane-count classes are wave-uniform
+            const unsigned long long tot =
+                (q == 0 || q == 3 || q >= 5) ? (unsigned long long)v[q] : wave_sum_u64(v[q]);
+            if (lane == 0 && tot) atomicAdd(stats + q, tot);
+        }
+    }
 }
 
 template <int TS, int CDIM>
@@ -385,14 +429,15 @@ static int launch_bwd(bool f64t, uint32_t C, uint32_t N, int64_t n_isects, uint3
                       const int32_t *last, const float *Tf, const float *vrc, const float *vra, float *vm2,
                       float *vcon, float *vcol, float *vop, float *vbe, const float4 *rec, float *vrec,
                       const float *ck, hipStream_t st) {
+    const int wide = rec && raster_wide_records(channels, TS) ? 1 : 0;
     if (f64t)
         raster_bwd_kernel<TS, CDIM, true><<<xcd_grid(C, tw, th), TS * TS, 0, st>>>(
             C, N, n_isects, channels, coff, means2d, conics, colors, opac, betas, bg, masks, W, H, tw, th, offs, flat,
-            ra, last, Tf, vrc, vra, vm2, vcon, vcol, vop, vbe, rec, vrec, ck);
+            ra, last, Tf, vrc, vra, vm2, vcon, vcol, vop, vbe, rec, wide, vrec, ck);
     else
         raster_bwd_kernel<TS, CDIM, false><<<xcd_grid(C, tw, th), TS * TS, 0, st>>>(
             C, N, n_isects, channels, coff, means2d, conics, colors, opac, betas, bg, masks, W, H, tw, th, offs, flat,
-            ra, last, Tf, vrc, vra, vm2, vcon, vcol, vop, vbe, rec, vrec, ck);
+            ra, last, Tf, vrc, vra, vm2, vcon, vcol, vop, vbe, rec, wide, vrec, ck);
     UBS_LAUNCH_CHECK("raster_bwd_kernel");
     return UBS_OK;
 }
@@ -473,16 +518,18 @@ static int raster_bwd_impl(uint32_t C, uint32_t N, int64_t n_isects, uint32_t ch
     }
     const float4 *rec = nullptr;
     float *vrec = nullptr;
-    int e = zero_async(v_colors, CN * channels * sizeof(float), st);
+    const bool wide = packed && raster_wide_records(channels, tile_size);
+    int e = wide ? UBS_OK : zero_async(v_colors, CN * channels * sizeof(float), st);
     if (packed) {
         // records in the first half of the workspace, packed gradients in the
         // second; the scalar gradient tensors are written by the unpack below
         float4 *r = static_cast<float4 *>(workspace);
         vrec = reinterpret_cast<float *>(static_cast<char *>(workspace) + raster_workspace_bytes(C, N, 0));
-        if (!e) e = zero_async(vrec, CN * kRecordFloats * sizeof(float), st);
+        if (!e) e = zero_async(vrec, CN * (wide ? kWideRecordFloats : kRecordFloats) * sizeof(float), st);
         if (e) return e;
         UBS_REQUIRE(means2d && conics && opacities && betas, "null pointer");
-        raster_pack_kernel<<<grid_for((int64_t)CN), kThreads, 0, st>>>(CN, means2d, conics, opacities, betas, r);
+        raster_pack_kernel<<<grid_for((int64_t)CN), kThreads, 0, st>>>(CN, means2d, conics, opacities, betas, r,
+                                                                       wide ? colors : nullptr, channels);
         UBS_LAUNCH_CHECK("raster_pack_kernel");
         rec = r;
     } else {
@@ -528,7 +575,8 @@ static int raster_bwd_impl(uint32_t C, uint32_t N, int64_t n_isects, uint32_t ch
     }
     if (packed) {
         raster_unpack_grads_kernel<<<grid_for((int64_t)CN), kThreads, 0, st>>>(
-            CN, reinterpret_cast<const float4 *>(vrec), v_means2d, v_conics, v_opacities, v_betas);
+            CN, reinterpret_cast<const float4 *>(vrec), v_means2d, v_conics, v_opacities, v_betas,
+            wide ? v_colors : nullptr, channels);
         UBS_LAUNCH_CHECK("raster_unpack_grads_kernel");
     }
     return UBS_OK;
@@ -565,5 +613,32 @@ int ubs_rasterize_to_pixels_bwd_ckpt(uint32_t C, uint32_t N, int64_t n_isects, u
                            v_render_colors, v_render_alphas, v_means2d, v_conics, v_colors, v_opacities, v_betas,
                            T_ckpt, ckpt_n, workspace, workspace_bytes, stream);
 }
+
+#ifdef UBS_PROBE_BUILD
+// Measurement build only (libubs_probe.so): the 16x16-tile, 3-channel blend
+// backward with the STATS counters on the library's packed path (its
+// gradients stay packed in the workspace); stats: 7 zeroed uint64.
+__attribute__((visibility("default"))) int ubs_probe_raster_bwd_stats(
+    uint32_t C, uint32_t N, int64_t n_isects, const float *means2d, const float *conics, const float *colors,
+    const float *opacities, const float *betas, uint32_t W, uint32_t H, uint32_t tw, uint32_t th,
+    const int32_t *tile_offsets, const int32_t *flatten_ids, const int32_t *last_ids, const float *T_final,
+    const float *v_render_colors, const float *v_render_alphas, float *v_colors, void *workspace,
+    unsigned long long *stats, void *stream) {
+    hipStream_t st = as_stream(stream);
+    const uint64_t CN = (uint64_t)C * N;
+    float4 *r = static_cast<float4 *>(workspace);
+    float *vrec = reinterpret_cast<float *>(static_cast<char *>(workspace) + raster_workspace_bytes(C, N, 0));
+    int e = zero_async(v_colors, CN * 3 * sizeof(float), st);
+    if (!e) e = zero_async(vrec, CN * kWideRecordFloats * sizeof(float), st);
+    if (e) return e;
+    raster_pack_kernel<<<grid_for((int64_t)CN), kThreads, 0, st>>>(CN, means2d, conics, opacities, betas, r,
+                                                                   colors, 3);
+    raster_bwd_kernel<16, 3, false, true><<<xcd_grid(C, tw, th), 256, 0, st>>>(
+        C, N, n_isects, 3, 0, means2d, conics, colors, opacities, betas, nullptr, nullptr, W, H, tw, th,
+        tile_offsets, flatten_ids, nullptr, last_ids, T_final, v_render_colors, v_render_alphas, nullptr, nullptr,
+        v_colors, nullptr, nullptr, r, 1, vrec, nullptr, stats);
+    return (int)hipGetLastError();
+}
+#endif
 
 }  // extern "C"

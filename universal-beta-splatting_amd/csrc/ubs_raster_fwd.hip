@@ -17,7 +17,11 @@
 
 namespace ubs {
 
-template <int TS, int CDIM>
+// STATS (measurement builds only; the library's kernels have it off) counts
+// into stats[0..4]: (wave, primitive) steps of the blend loop, lane
+// evaluations of a pixel still blending, of those inside the primitive's
+// support (0 <= sigma < 1), of those that contribute, and staged entries.
+template <int TS, int CDIM, bool STATS = false>
 __global__ __launch_bounds__(TS *TS) void raster_fwd_kernel(
     uint32_t C, uint32_t N, int64_t n_isects, uint32_t channels, uint32_t coff,
     const float *__restrict__ means2d, const float *__restrict__ conics, const float *__restrict__ colors,
@@ -25,7 +29,11 @@ __global__ __launch_bounds__(TS *TS) void raster_fwd_kernel(
     const uint8_t *__restrict__ masks, uint32_t W, uint32_t H, uint32_t tw, uint32_t th,
     const int32_t *__restrict__ tile_offsets, const int32_t *__restrict__ flatten_ids,
     float *__restrict__ render_colors, float *__restrict__ render_alphas, int32_t *__restrict__ last_ids,
-    float *__restrict__ T_final, const float4 *__restrict__ rec, float *__restrict__ T_ckpt) {
+    float *__restrict__ T_final, const float4 *__restrict__ rec, int wide_rec, float *__restrict__ T_ckpt,
+    unsigned long long *__restrict__ stats = nullptr, const int64_t *__restrict__ n_dev = nullptr) {
+    // n_dev: the intersection count in device memory (at most n_isects, the
+    // capacity of the lists), for callers that size the lists on the device
+    if (n_dev) n_isects = min(max(*n_dev, (int64_t)0), n_isects);
     constexpr int BS = TS * TS;        // threads = batch size
     constexpr int SB = TS / 8;         // sub-blocks per tile side
     constexpr int NW = BS / 64;        // waves
@@ -68,8 +76,11 @@ __global__ __launch_bounds__(TS *TS) void raster_fwd_kernel(
     }
 
     const int64_t tbase = (int64_t)cam * th * tw;
-    const int32_t range_start = tile_offsets[tbase + tile];
-    const int32_t range_end = (cam == C - 1 && tile == th * tw - 1) ? (int32_t)n_isects : tile_offsets[tbase + tile + 1];
+    // a valid sort gives ordered offsets within [0, n_isects]; the clamps keep
+    // the lists of a sort that gave up on a look-back wait inside the buffers
+    const int32_t off_hi = (cam == C - 1 && tile == th * tw - 1) ? (int32_t)n_isects : tile_offsets[tbase + tile + 1];
+    const int32_t range_start = min(max(tile_offsets[tbase + tile], 0), (int32_t)n_isects);
+    const int32_t range_end = min(max(off_hi, range_start), (int32_t)n_isects);
     const float px = (float)j + 0.5f, py = (float)i + 0.5f;
     const float x0 = (float)(tx * TS), y0 = (float)(ty * TS);
 
@@ -80,17 +91,29 @@ __global__ __launch_bounds__(TS *TS) void raster_fwd_kernel(
 #pragma unroll
     for (int k = 0; k < CDIM; ++k) acc[k] = 0.f;
 
+    uint32_t st_steps = 0, st_live = 0, st_sup = 0, st_take = 0, st_staged = 0;
     const int32_t nb = range_end > range_start ? (range_end - range_start + BS - 1) / BS : 0;
     for (int32_t b = 0; b < nb; ++b) {
         if (__syncthreads_count(done) >= BS) break;
         const int32_t batch_start = range_start + b * BS;
         const int32_t cnt = min(BS, range_end - batch_start);
+        if constexpr (STATS) st_staged += (uint32_t)(tid < cnt);
         // ---- stage one primitive per thread
         if (tid < cnt) {
-            const int32_t g = flatten_ids[batch_start + tid];
+            // an id outside [0, C N) (only from a sort that gave up) stages
+            // primitive 0 with an empty sub-block mask: never blended
+            const int32_t g_raw = flatten_ids[batch_start + tid];
+            const bool g_ok = (uint64_t)(uint32_t)g_raw < (uint64_t)C * N;
+            const int32_t g = g_ok ? g_raw : 0;
             float2 xy;
             float ca, cb, cc, op, be;
-            if (rec) {  // one 32-byte record per (camera, primitive)
+            float4 cw = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (rec && wide_rec) {  // one 64-byte record per (camera, primitive), colours included
+                const float4 r0 = rec[4 * (int64_t)g], r1 = rec[4 * (int64_t)g + 1];
+                cw = rec[4 * (int64_t)g + 2];
+                xy = make_float2(r0.x, r0.y);
+                ca = r0.z; cb = r0.w; cc = r1.x; op = r1.y; be = r1.z;
+            } else if (rec) {  // one 32-byte record per (camera, primitive)
                 const float4 r0 = rec[2 * (int64_t)g], r1 = rec[2 * (int64_t)g + 1];
                 xy = make_float2(r0.x, r0.y);
                 ca = r0.z; cb = r0.w; cc = r1.x; op = r1.y; be = r1.z;
@@ -103,15 +126,16 @@ __global__ __launch_bounds__(TS *TS) void raster_fwd_kernel(
             s_geo2[tid] = make_float4(cc, op, be, 0.f);
             const float *cp = colors + (int64_t)g * channels + coff;
             if constexpr (kColVec) {
-                s_colv[tid] = make_float4(cp[0], CDIM > 1 ? cp[1] : 0.f, CDIM > 2 ? cp[2] : 0.f,
-                                          CDIM > 3 ? cp[3] : 0.f);
+                s_colv[tid] = (rec && wide_rec) ? cw
+                                                : make_float4(cp[0], CDIM > 1 ? cp[1] : 0.f, CDIM > 2 ? cp[2] : 0.f,
+                                                              CDIM > 3 ? cp[3] : 0.f);
             } else {
 #pragma unroll
                 for (int k = 0; k < CDIM; ++k) s_col[k][tid] = cp[k];
             }
             float hx, hy;
             support_box(ca, cb, cc, hx, hy);
-            s_mask[tid] = subblock_mask_exact<SB>(xy.x, xy.y, ca, cb, cc, hx, hy, x0, y0);
+            s_mask[tid] = g_ok ? subblock_mask_exact<SB>(xy.x, xy.y, ca, cb, cc, hx, hy, x0, y0) : 0u;
         }
         __syncthreads();
         // ---- per-wave compaction of the batch to this sub-block's primitives
@@ -152,6 +176,12 @@ __global__ __launch_bounds__(TS *TS) void raster_fwd_kernel(
                 const float nT = T * (1.f - alpha);
                 const bool stop = ok && nT <= 1e-4f;  // this primitive would end the pixel: excluded
                 const bool take = ok && !stop;
+                if constexpr (STATS) {
+                    st_steps += 1;
+                    st_live += (uint32_t)!done;
+                    st_sup += (uint32_t)ok;
+                    st_take += (uint32_t)take;
+                }
                 done = done || stop;
                 const float vis = take ? alpha * T : 0.f;
                 if constexpr (kColVec) {
@@ -172,6 +202,15 @@ __global__ __launch_bounds__(TS *TS) void raster_fwd_kernel(
         if (T_ckpt && write_meta) T_ckpt[ckpt_slot(range_start, logical, b, BS) * BS + tid] = T;
     }
 
+    if constexpr (STATS) {
+        const uint32_t v[5] = {st_steps, st_live, st_sup, st_take, st_staged};
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+            // steps are wave-uniform: one lane's count; the others sum over the wave
+            const unsigned long long tot = q == 0 ? (unsigned long long)v[0] : wave_sum_u64(v[q]);
+            if (lane == 0 && tot) atomicAdd(stats + q, tot);
+        }
+    }
     if (inside) {
 #pragma unroll
         for (int k = 0; k < CDIM; ++k)
@@ -190,10 +229,10 @@ static int launch_fwd(uint32_t C, uint32_t N, int64_t n_isects, uint32_t channel
                       const float *means2d, const float *conics, const float *colors, const float *opac,
                       const float *betas, const float *bg, const uint8_t *masks, uint32_t W, uint32_t H,
                       uint32_t tw, uint32_t th, const int32_t *offs, const int32_t *flat, float *rc, float *ra,
-                      int32_t *last, float *Tf, const float4 *rec, float *ck, hipStream_t st) {
+                      int32_t *last, float *Tf, const float4 *rec, float *ck, const int64_t *n_dev, hipStream_t st) {
     raster_fwd_kernel<TS, CDIM><<<xcd_grid(C, tw, th), TS * TS, 0, st>>>(
         C, N, n_isects, channels, coff, means2d, conics, colors, opac, betas, bg, masks, W, H, tw, th, offs, flat,
-        rc, ra, last, Tf, rec, ck);
+        rc, ra, last, Tf, rec, rec && raster_wide_records(channels, TS) ? 1 : 0, ck, nullptr, n_dev);
     UBS_LAUNCH_CHECK("raster_fwd_kernel");
     return UBS_OK;
 }
@@ -203,11 +242,12 @@ static int launch_fwd_chunk(int cd, uint32_t C, uint32_t N, int64_t n_isects, ui
                             const float *means2d, const float *conics, const float *colors, const float *opac,
                             const float *betas, const float *bg, const uint8_t *masks, uint32_t W, uint32_t H,
                             uint32_t tw, uint32_t th, const int32_t *offs, const int32_t *flat, float *rc,
-                            float *ra, int32_t *last, float *Tf, const float4 *rec, float *ck, hipStream_t st) {
+                            float *ra, int32_t *last, float *Tf, const float4 *rec, float *ck, const int64_t *n_dev,
+                            hipStream_t st) {
 #define UBS_FWD_CASE(K)                                                                                     \
     case K:                                                                                                 \
         return launch_fwd<TS, K>(C, N, n_isects, channels, coff, means2d, conics, colors, opac, betas, bg,  \
-                                 masks, W, H, tw, th, offs, flat, rc, ra, last, Tf, rec, ck, st);
+                                 masks, W, H, tw, th, offs, flat, rc, ra, last, Tf, rec, ck, n_dev, st);
     switch (cd) {
         UBS_FWD_CASE(1)
         UBS_FWD_CASE(2)
@@ -216,17 +256,17 @@ static int launch_fwd_chunk(int cd, uint32_t C, uint32_t N, int64_t n_isects, ui
     case 8:
         if constexpr (TS <= 16)
             return launch_fwd<TS, 8>(C, N, n_isects, channels, coff, means2d, conics, colors, opac, betas, bg,
-                                     masks, W, H, tw, th, offs, flat, rc, ra, last, Tf, rec, ck, st);
+                                     masks, W, H, tw, th, offs, flat, rc, ra, last, Tf, rec, ck, n_dev, st);
         break;
     case 16:
         if constexpr (TS <= 16)
             return launch_fwd<TS, 16>(C, N, n_isects, channels, coff, means2d, conics, colors, opac, betas, bg,
-                                      masks, W, H, tw, th, offs, flat, rc, ra, last, Tf, rec, ck, st);
+                                      masks, W, H, tw, th, offs, flat, rc, ra, last, Tf, rec, ck, n_dev, st);
         break;
     case 32:
         if constexpr (TS <= 16)
             return launch_fwd<TS, 32>(C, N, n_isects, channels, coff, means2d, conics, colors, opac, betas, bg,
-                                      masks, W, H, tw, th, offs, flat, rc, ra, last, Tf, rec, ck, st);
+                                      masks, W, H, tw, th, offs, flat, rc, ra, last, Tf, rec, ck, n_dev, st);
         break;
     default:
         break;
@@ -246,7 +286,7 @@ static int raster_fwd_impl(uint32_t C, uint32_t N, int64_t n_isects, uint32_t ch
                            uint32_t tile_size, uint32_t tw, uint32_t th, const int32_t *tile_offsets,
                            const int32_t *flatten_ids, float *render_colors, float *render_alphas,
                            int32_t *last_ids, float *T_final, float *T_ckpt, size_t ckpt_n, void *workspace,
-                           size_t workspace_bytes, void *stream) {
+                           size_t workspace_bytes, void *stream, const int64_t *n_dev = nullptr) {
     hipStream_t st = as_stream(stream);
     UBS_REQUIRE(tile_size == 8 || tile_size == 16 || tile_size == 32, "tile_size must be 8, 16 or 32 (got %u)",
                 tile_size);
@@ -274,8 +314,10 @@ static int raster_fwd_impl(uint32_t C, uint32_t N, int64_t n_isects, uint32_t ch
         }
         const uint64_t CN = (uint64_t)C * N;
         if (CN && n_isects) {
-            raster_pack_kernel<<<grid_for((int64_t)CN), kThreads, 0, st>>>(CN, means2d, conics, opacities, betas,
-                                                                           static_cast<float4 *>(workspace));
+            const bool wide = raster_wide_records(channels, tile_size);
+            raster_pack_kernel<<<grid_for((int64_t)CN), kThreads, 0, st>>>(
+                CN, means2d, conics, opacities, betas, static_cast<float4 *>(workspace), wide ? colors : nullptr,
+                channels);
             UBS_LAUNCH_CHECK("raster_pack_kernel");
             rec = static_cast<const float4 *>(workspace);
         }
@@ -287,15 +329,15 @@ static int raster_fwd_impl(uint32_t C, uint32_t N, int64_t n_isects, uint32_t ch
         if (tile_size == 16)
             e = launch_fwd_chunk<16>(cd, C, N, n_isects, channels, coff, means2d, conics, colors, opacities, betas,
                                      backgrounds, masks, W, H, tw, th, tile_offsets, flatten_ids, render_colors,
-                                     render_alphas, last_ids, T_final, rec, T_ckpt, st);
+                                     render_alphas, last_ids, T_final, rec, T_ckpt, n_dev, st);
         else if (tile_size == 8)
             e = launch_fwd_chunk<8>(cd, C, N, n_isects, channels, coff, means2d, conics, colors, opacities, betas,
                                     backgrounds, masks, W, H, tw, th, tile_offsets, flatten_ids, render_colors,
-                                    render_alphas, last_ids, T_final, rec, T_ckpt, st);
+                                    render_alphas, last_ids, T_final, rec, T_ckpt, n_dev, st);
         else
             e = launch_fwd_chunk<32>(cd, C, N, n_isects, channels, coff, means2d, conics, colors, opacities, betas,
                                      backgrounds, masks, W, H, tw, th, tile_offsets, flatten_ids, render_colors,
-                                     render_alphas, last_ids, T_final, rec, T_ckpt, st);
+                                     render_alphas, last_ids, T_final, rec, T_ckpt, n_dev, st);
         if (e) return e;
         coff += (uint32_t)cd;
     }
@@ -313,6 +355,21 @@ extern "C" int ubs_rasterize_to_pixels_fwd(uint32_t C, uint32_t N, int64_t n_ise
     return raster_fwd_impl(C, N, n_isects, channels, means2d, conics, colors, opacities, betas, backgrounds, masks, W,
                            H, tile_size, tw, th, tile_offsets, flatten_ids, render_colors, render_alphas, last_ids,
                            T_final, nullptr, 0, workspace, workspace_bytes, stream);
+}
+
+extern "C" int ubs_rasterize_to_pixels_fwd_dev(uint32_t C, uint32_t N, int64_t n_cap, const int64_t *n_dev,
+                                               uint32_t channels, const float *means2d, const float *conics,
+                                               const float *colors, const float *opacities, const float *betas,
+                                               const float *backgrounds, const uint8_t *masks, uint32_t W,
+                                               uint32_t H, uint32_t tile_size, uint32_t tw, uint32_t th,
+                                               const int32_t *tile_offsets, const int32_t *flatten_ids,
+                                               float *render_colors, float *render_alphas, int32_t *last_ids,
+                                               float *T_final, void *workspace, size_t workspace_bytes,
+                                               void *stream) {
+    UBS_REQUIRE(n_dev, "n_dev is null");
+    return raster_fwd_impl(C, N, n_cap, channels, means2d, conics, colors, opacities, betas, backgrounds, masks, W,
+                           H, tile_size, tw, th, tile_offsets, flatten_ids, render_colors, render_alphas, last_ids,
+                           T_final, nullptr, 0, workspace, workspace_bytes, stream, n_dev);
 }
 
 extern "C" int ubs_rasterize_to_pixels_fwd_ckpt(uint32_t C, uint32_t N, int64_t n_isects, uint32_t channels,
@@ -338,3 +395,24 @@ extern "C" size_t ubs_transmittance_checkpoint_floats(uint32_t C, int64_t n_isec
 extern "C" size_t ubs_rasterize_workspace_bytes(uint32_t C, uint32_t N, int backward) {
     return raster_workspace_bytes(C, N, backward);
 }
+
+#ifdef UBS_PROBE_BUILD
+// Measurement build only (libubs_probe.so): the 16x16-tile, 3-channel blend
+// forward with the STATS counters, on the packed-record path of the library's
+// forward; stats: 5 zeroed uint64 on the device.
+extern "C" __attribute__((visibility("default"))) int ubs_probe_raster_fwd_stats(
+    uint32_t C, uint32_t N, int64_t n_isects, const float *means2d, const float *conics, const float *colors,
+    const float *opacities, const float *betas, uint32_t W, uint32_t H, uint32_t tw, uint32_t th,
+    const int32_t *tile_offsets, const int32_t *flatten_ids, float *render_colors, float *render_alphas,
+    int32_t *last_ids, float *T_final, void *workspace, unsigned long long *stats, void *stream) {
+    hipStream_t st = as_stream(stream);
+    const uint64_t CN = (uint64_t)C * N;
+    raster_pack_kernel<<<grid_for((int64_t)CN), kThreads, 0, st>>>(CN, means2d, conics, opacities, betas,
+                                                                   static_cast<float4 *>(workspace), colors, 3);
+    raster_fwd_kernel<16, 3, true><<<xcd_grid(C, tw, th), 256, 0, st>>>(
+        C, N, n_isects, 3, 0, means2d, conics, colors, opacities, betas, nullptr, nullptr, W, H, tw, th,
+        tile_offsets, flatten_ids, render_colors, render_alphas, last_ids, T_final,
+        static_cast<const float4 *>(workspace), 1, nullptr, stats);
+    return (int)hipGetLastError();
+}
+#endif

@@ -84,8 +84,15 @@ __device__ __forceinline__ uint32_t digit_of(int64_t key, int shift, uint32_t ma
 // Digit counts of every pass in one read of the keys.  A digit shared by the
 // whole wave (the constant exponent byte of the depths, a primitive's tile
 // bits) is added once by one lane instead of by 64 serialised LDS atomics.
-__global__ void sort_hist_kernel(int64_t n, const int64_t *__restrict__ keys, int begin_bit, int end_bit,
-                                 int passes, uint32_t *__restrict__ hist) {
+// n_dev (may be null): the key count in device memory, at most n_cap (the
+// sizes the grid and the workspace were laid out for); null: n_cap keys.
+__device__ __forceinline__ int64_t sort_count(int64_t n_cap, const int64_t *n_dev) {
+    return n_dev ? min(max(*n_dev, (int64_t)0), n_cap) : n_cap;
+}
+
+__global__ void sort_hist_kernel(int64_t n_cap, const int64_t *__restrict__ n_dev, const int64_t *__restrict__ keys,
+                                 int begin_bit, int end_bit, int passes, uint32_t *__restrict__ hist) {
+    const int64_t n = sort_count(n_cap, n_dev);
     __shared__ uint32_t h[8][kRadix];
     for (int k = threadIdx.x; k < passes * kRadix; k += blockDim.x) h[k / kRadix][k % kRadix] = 0;
     __syncthreads();
@@ -129,7 +136,7 @@ __global__ void sort_hist_scan_kernel(uint32_t *__restrict__ hist) {
 
 template <int ITEMS>
 __global__ __launch_bounds__(kSortThreads) void sort_pass_kernel(
-    int64_t n, const int64_t *__restrict__ keys_in, int64_t *__restrict__ keys_out,
+    int64_t n_cap, const int64_t *__restrict__ n_dev, const int64_t *__restrict__ keys_in, int64_t *__restrict__ keys_out,
     const int32_t *__restrict__ vals_in, int32_t *__restrict__ vals_out, int shift, int bits,
     const uint32_t *__restrict__ digit_base, uint32_t *__restrict__ tile_counter, uint32_t *lookback,
     uint32_t *status) {
@@ -143,11 +150,16 @@ __global__ __launch_bounds__(kSortThreads) void sort_pass_kernel(
 
     const int tid = threadIdx.x, wave = tid >> 6, lane = (int)lane_id();
     const uint32_t mask = (1u << bits) - 1u;
+    const int64_t n = sort_count(n_cap, n_dev);
     if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
     for (int k = tid; k < kSortWaves * kRadix; k += kSortThreads) (&s_wave_cnt[0][0])[k] = 0;
     __syncthreads();
     const int64_t tile = s_tile;
     const int64_t base = tile * TILE;
+    // with the count in device memory the grid covers n_cap keys: the tiles
+    // past the last key leave (tile ids are handed out in order, so no tile
+    // that a real one waits on is among them)
+    if (base >= n) return;
 
     // a pass whose digit every key shares (a constant exponent byte of the
     // depths) is the identity of a stable sort: the tile is copied through
@@ -329,9 +341,28 @@ int ubs_radix_sort_pairs(int64_t n, int64_t *keys, int64_t *keys_alt, int32_t *v
                                      ws_bytes, result_in_alt, stream);
 }
 
+static int sort_impl(int64_t n, const int64_t *n_dev, int64_t *keys, int64_t *keys_alt, int32_t *vals,
+                     int32_t *vals_alt, int begin_bit, int end_bit, uint32_t skip_passes, void *workspace,
+                     size_t ws_bytes, int *result_in_alt, void *stream);
+
 int ubs_radix_sort_pairs_skip(int64_t n, int64_t *keys, int64_t *keys_alt, int32_t *vals, int32_t *vals_alt,
                               int begin_bit, int end_bit, uint32_t skip_passes, void *workspace, size_t ws_bytes,
                               int *result_in_alt, void *stream) {
+    return sort_impl(n, nullptr, keys, keys_alt, vals, vals_alt, begin_bit, end_bit, skip_passes, workspace,
+                     ws_bytes, result_in_alt, stream);
+}
+
+int ubs_radix_sort_pairs_dev(int64_t n_cap, const int64_t *n_dev, int64_t *keys, int64_t *keys_alt, int32_t *vals,
+                             int32_t *vals_alt, int begin_bit, int end_bit, void *workspace, size_t ws_bytes,
+                             int *result_in_alt, void *stream) {
+    UBS_REQUIRE(n_dev, "n_dev is null");
+    return sort_impl(n_cap, n_dev, keys, keys_alt, vals, vals_alt, begin_bit, end_bit, 0u, workspace, ws_bytes,
+                     result_in_alt, stream);
+}
+
+static int sort_impl(int64_t n, const int64_t *n_dev, int64_t *keys, int64_t *keys_alt, int32_t *vals,
+                     int32_t *vals_alt, int begin_bit, int end_bit, uint32_t skip_passes, void *workspace,
+                     size_t ws_bytes, int *result_in_alt, void *stream) {
     hipStream_t st = as_stream(stream);
     UBS_REQUIRE(result_in_alt, "result_in_alt is null");
     UBS_REQUIRE(begin_bit >= 0 && end_bit <= 64 && begin_bit <= end_bit, "bad bit range [%d, %d)", begin_bit,
@@ -354,7 +385,7 @@ int ubs_radix_sort_pairs_skip(int64_t n, int64_t *keys, int64_t *keys_alt, int32
     uint32_t *counters = reinterpret_cast<uint32_t *>(ws + L.counter_off);
     uint32_t *lookback = reinterpret_cast<uint32_t *>(ws + L.lookback_off);
     const unsigned hist_blocks = (unsigned)std::min<int64_t>((n + 4095) / 4096, 2048);
-    sort_hist_kernel<<<hist_blocks, 256, 0, st>>>(n, keys, begin_bit, end_bit, L.passes, hist);
+    sort_hist_kernel<<<hist_blocks, 256, 0, st>>>(n, n_dev, keys, begin_bit, end_bit, L.passes, hist);
     UBS_LAUNCH_CHECK("sort_hist_kernel");
     sort_hist_scan_kernel<<<L.passes, kRadix, 0, st>>>(hist);
     UBS_LAUNCH_CHECK("sort_hist_scan_kernel");
@@ -369,7 +400,7 @@ int ubs_radix_sort_pairs_skip(int64_t n, int64_t *keys, int64_t *keys_alt, int32
         const int bits = std::min(kRadixBits, end_bit - shift);
         auto *pass = n < kSmallSortN ? sort_pass_kernel<kItemsSmall> : sort_pass_kernel<kItems>;
         pass<<<(unsigned)L.tiles, kSortThreads, 0, st>>>(
-            n, kb[run & 1], kb[(run + 1) & 1], vb[run & 1], vb[(run + 1) & 1], shift, bits,
+            n, n_dev, kb[run & 1], kb[(run + 1) & 1], vb[run & 1], vb[(run + 1) & 1], shift, bits,
             hist + (size_t)p * kRadix, counters + p, lookback + (size_t)p * L.tiles * kRadix, status);
         UBS_LAUNCH_CHECK("sort_pass_kernel");
         ++run;

@@ -27,6 +27,7 @@ from .cuda._wrapper import (
     cond_mean_convariance_opacity,
     isect_begin,
     isect_finish,
+    isect_read_info,
     isect_offset_encode,
     isect_tiles_offsets,
     l_triangle_to_rotmat,
@@ -250,12 +251,12 @@ def render_batch(params: BetaParams, viewmats: Tensor, Ks: Tensor, sizes, timest
         W, H = sizes[order[c0]]
         tw, th = math.ceil(W / float(tile_size)), math.ceil(H / float(tile_size))
         states.append(isect_begin(means2d[c0:c1], radii[c0:c1], depths[c0:c1], tile_size, tw, th))
-    infos = torch.stack([st.info for st in states]).cpu() if states else None
+    infos = isect_read_info(states)
     for g, (c0, c1) in enumerate(groups):
         W, H = sizes[order[c0]]
         Cg = c1 - c0
         tw, th = math.ceil(W / float(tile_size)), math.ceil(H / float(tile_size))
-        _, isect_ids, flatten_ids, offsets = isect_finish(states[g], int(infos[g, 0]), int(infos[g, 1]))
+        _, isect_ids, flatten_ids, offsets = isect_finish(states[g], *infos[g])
         if offsets is None:
             offsets = isect_offset_encode(isect_ids, Cg, tw, th)
         bg = backgrounds[c0:c1] if backgrounds is not None else None

@@ -65,6 +65,16 @@ _SIGNATURES = {
         _c_int, [_c_i64, _vp, _vp, _vp, _vp, _c_int, _c_int, _vp, _c_size, ctypes.POINTER(_c_int), _vp]),
     "ubs_radix_sort_pairs_skip": (
         _c_int, [_c_i64, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_u32, _vp, _c_size, ctypes.POINTER(_c_int), _vp]),
+    "ubs_beta_project_fwd_rows": (
+        _c_int, [_c_i64, _c_i64, _c_u32] + [_vp] * 11 + [_c_i32, _c_i32, _vp, _c_f32, _c_f32, _c_f32, _c_f32,
+                                                       _c_int] + [_vp] * 6 + [_c_i64, _vp]),
+    "ubs_beta_project_fwd_table": (
+        _c_int, [_c_u32, _c_i64] + [_vp] * 6 + [_c_f32, _c_f32, _c_f32, _c_f32] + [_vp] * 7 + [_vp]),
+    "ubs_isect_tiles_emit_ordered_cap": (
+        _c_int, [_c_u32, _c_u32, _vp, _vp, _vp, _vp, _vp, _c_u32, _c_u32, _c_u32, _vp, _vp, _c_i64, _vp]),
+    "ubs_radix_sort_pairs_dev": (
+        _c_int, [_c_i64, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _vp, _c_size, ctypes.POINTER(_c_int), _vp]),
+    "ubs_isect_offset_encode_dev": (_c_int, [_c_i64, _vp, _vp, _c_u32, _c_u32, _c_u32, _vp, _vp]),
     "ubs_radix_sort_status": (_c_int, [_vp, ctypes.POINTER(_c_int), _vp]),
     "ubs_isect_depth_keys": (_c_int, [_c_u32, _c_u32, _vp, _vp, _vp, _vp, _vp]),
     "ubs_isect_ordered_workspace_bytes": (_c_size, [_c_u32, _c_u32]),
@@ -78,6 +88,9 @@ _SIGNATURES = {
     "ubs_rasterize_workspace_bytes": (_c_size, [_c_u32, _c_u32, _c_int]),
     "ubs_rasterize_to_pixels_fwd": (
         _c_int, [_c_u32, _c_u32, _c_i64, _c_u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_u32, _c_u32, _c_u32,
+                 _c_u32, _c_u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_size, _vp]),
+    "ubs_rasterize_to_pixels_fwd_dev": (
+        _c_int, [_c_u32, _c_u32, _c_i64, _vp, _c_u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_u32, _c_u32, _c_u32,
                  _c_u32, _c_u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_size, _vp]),
     "ubs_rasterize_to_pixels_bwd": (
         _c_int, [_c_u32, _c_u32, _c_i64, _c_u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_u32, _c_u32, _c_u32,

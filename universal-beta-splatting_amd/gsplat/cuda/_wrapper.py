@@ -297,8 +297,38 @@ def isect_tiles_offsets(means2d: Tensor, radii: Tensor, depths: Tensor, tile_siz
     computed them on the way (else None): (tiles_per_gauss, isect_ids,
     flatten_ids, isect_offsets or None)."""
     state = isect_begin(means2d, radii, depths, tile_size, tile_width, tile_height, sort)
-    n_isects, packed = (int(x) for x in state.info.cpu())  # host sync, as isect_tiles.cu:181
+    (n_isects, packed), = isect_read_info([state])  # host sync, as isect_tiles.cu:181
     return isect_finish(state, n_isects, packed)
+
+
+# Per device: the sum of the status words of the isect-id sorts that ran after
+# the last host read (nonzero = a look-back wait gave up and that sort's order
+# is invalid).  The next isect_begin folds it into its info, so the host read
+# that already sizes the next step reports it.
+_late_sort_status = {}
+
+
+def _late_status(dev) -> Tensor:
+    t = _late_sort_status.get(dev)
+    if t is None:
+        t = _late_sort_status[dev] = torch.zeros((1,), dtype=torch.int64, device=dev)
+    return t
+
+
+def isect_read_info(states) -> list:
+    """One host read of the info of several isect_begin states: a list of
+    (n_isects, packed depth range).  Raises RuntimeError when a radix sort
+    since the previous read gave up on a look-back wait (its output order,
+    and what was rendered from it, is invalid)."""
+    if not states:
+        return []
+    infos = (torch.stack([st.info for st in states]) if len(states) > 1 else states[0].info[None]).cpu()
+    if bool((infos[:, 2] != 0).any()):
+        for st in states:
+            _late_status(st.info.device).zero_()
+        raise RuntimeError("radix sort look-back exceeded its spin limit: the isect order of this or the "
+                           "previous step is invalid")
+    return [(int(r[0]), int(r[1])) for r in infos]
 
 
 class IsectState:
@@ -330,8 +360,10 @@ def isect_begin(means2d: Tensor, radii: Tensor, depths: Tensor, tile_size: int, 
     S.tile_size, S.tile_width, S.tile_height, S.sort = tile_size, tile_width, tile_height, sort
     S.tpg = torch.empty((C, N), dtype=torch.int32, device=dev)
     S.cum = torch.empty((C * N,), dtype=torch.int64, device=dev)
-    # [n_isects, (~min depth bits, max depth bits) as two uint32], read in one sync
-    S.info = torch.zeros((2,), dtype=torch.int64, device=dev)
+    # [n_isects, (~min depth bits, max depth bits) as two uint32, sort status],
+    # read in one sync
+    S.info = torch.zeros((3,), dtype=torch.int64, device=dev)
+    S.info[2:3].add_(_late_status(dev))
     ws_bytes = int(L.ubs_isect_count_workspace_bytes(C, N))
     ws = torch.empty((ws_bytes,), dtype=torch.uint8, device=dev)
     _call("ubs_isect_tiles_count_range", C, N, ptr(means2d), ptr(radii), ptr(depths), tile_size, tile_width,
@@ -343,7 +375,7 @@ def isect_begin(means2d: Tensor, radii: Tensor, depths: Tensor, tile_size: int, 
         dkeys = torch.empty((C * N,), dtype=torch.int64, device=dev)
         dvals = torch.empty((C * N,), dtype=torch.int32, device=dev)
         _call("ubs_isect_depth_keys", C, N, ptr(radii), ptr(depths), ptr(dkeys), ptr(dvals), st)
-        _, S.order = radix_sort_pairs(dkeys, dvals, 0, 32)
+        _, S.order = radix_sort_pairs(dkeys, dvals, 0, 32, status_out=S.info[2:3])
         S.cum = torch.empty((C * N,), dtype=torch.int64, device=dev)
         ows = int(L.ubs_isect_ordered_workspace_bytes(C, N))
         ow = torch.empty((ows,), dtype=torch.uint8, device=dev)
@@ -368,18 +400,19 @@ def isect_finish(S: IsectState, n_isects: int, packed: int):
     offsets = None
     if n_isects and S.sort:
         end_bit = 32 + int(L.ubs_isect_prefix_bits(C, tw, th))
+        late = _late_status(dev)
         if S.order is not None:  # entries already in depth order: stable on the camera/tile bits finishes it
-            isect_ids, flatten_ids = radix_sort_pairs(isect_ids, flatten_ids, 32, end_bit)
+            isect_ids, flatten_ids = radix_sort_pairs(isect_ids, flatten_ids, 32, end_bit, status_out=late)
         elif _sort_mode == "segmented":
             # stable on the camera/tile bits, then per tile on the depth bits
-            ids1, flat1 = radix_sort_pairs(isect_ids, flatten_ids, 32, end_bit)
+            ids1, flat1 = radix_sort_pairs(isect_ids, flatten_ids, 32, end_bit, status_out=late)
             offsets = isect_offset_encode(ids1, C, tw, th)
             isect_ids, flatten_ids = torch.empty_like(ids1), torch.empty_like(flat1)
             _call("ubs_sort_tile_segments", n_isects, ptr(ids1), ptr(flat1), ptr(isect_ids), ptr(flatten_ids),
                   ptr(offsets), C, tw, th, constant_depth_passes(packed), st)
         else:
             isect_ids, flatten_ids = radix_sort_pairs(isect_ids, flatten_ids, 0, end_bit,
-                                                      skip_passes=constant_depth_passes(packed))
+                                                      skip_passes=constant_depth_passes(packed), status_out=late)
     return S.tpg, isect_ids, flatten_ids, offsets
 
 
@@ -406,12 +439,15 @@ def _vp_at(t: Tensor, index: int):
 
 
 def radix_sort_pairs(keys: Tensor, values: Tensor, begin_bit: int, end_bit: int,
-                     check_status: bool = False, skip_passes: int = 0) -> Tuple[Tensor, Tensor]:
+                     check_status: bool = False, skip_passes: int = 0,
+                     status_out: Optional[Tensor] = None) -> Tuple[Tensor, Tensor]:
     """Stable onesweep radix sort of (int64 key, int32 value) on key bits
     [begin_bit, end_bit); returns new sorted tensors (double-buffered; the
     inputs may be consumed as scratch).  skip_passes: bit p set leaves out
     the 8-bit pass over bits [begin_bit + 8p, +8), which the caller knows
-    every key shares (the pass would be the identity)."""
+    every key shares (the pass would be the identity).  check_status reads
+    the sort's status word now (a host sync); status_out (int64 [1] on the
+    device) instead gets it added, without a sync, for a later read."""
     require_cuda(keys, values)
     assert keys.dtype == torch.int64 and values.dtype == torch.int32
     n = keys.numel()
@@ -430,6 +466,9 @@ def radix_sort_pairs(keys: Tensor, values: Tensor, begin_bit: int, end_bit: int,
     else:
         _call("ubs_radix_sort_pairs", n, ptr(keys), ptr(keys_alt), ptr(values), ptr(vals_alt), begin_bit,
               end_bit, ptr(ws), ws_bytes, ctypes.byref(sel), st)
+    if status_out is not None and n > 0:
+        # the status word (uint32 at offset 0, followed by zeroed bytes) as int64
+        status_out.add_(ws[:8].view(torch.int64))
     if check_status:
         status = ctypes.c_int(0)
         L.check(L.ubs_radix_sort_status(ptr(ws), ctypes.byref(status), st), "radix_sort_status")

@@ -91,6 +91,8 @@ class SceneStore:
         return self.isects_per_prim[name]
 
     def remove(self, name: str) -> None:
+        """Drop a scene.  A batch of it still in flight on a RenderServer keeps
+        its tensors alive until the server collects that batch."""
         self.scenes.pop(name, None)
         self.isects_per_prim.pop(name, None)
 
@@ -121,6 +123,8 @@ class _InFlight:
     images: List[Tensor]
     alphas: List[Tensor]
     bytes: int
+    params: List[BetaParams]  # held until the batch is collected: a scene removed meanwhile stays allocated
+    graph: Optional[object] = None  # the ViewBatchGraph that rendered it (its count is checked at collect)
 
 
 class RenderServer:
@@ -136,8 +140,17 @@ class RenderServer:
     def __init__(self, store: SceneStore, max_views: int = 16, streams: int = 2,
                  hbm_reserve_bytes: int = 1 << 30, tile_size: int = 16,
                  render_fn: Optional[Callable] = None, free_bytes_fn: Optional[Callable[[], int]] = None,
-                 keep_images: bool = True):
+                 keep_images: bool = True, graphs: bool = False, cap_margin: float = 1.5):
         self.store = store
+        # graphs: batches of one image size over any resident scenes run as a
+        # replayed hipGraph per (size, slots) with device-sized lists
+        # (gsplat.batch_graph); otherwise one scene's views per batch through
+        # render_batch (host-sized lists, any mix of sizes)
+        self.graphs = bool(graphs) and render_fn is None
+        self.cap_margin = float(cap_margin)
+        self._pipes: List[Dict[Tuple[int, int, int, int], object]] = []
+        self._pools: List[object] = []
+        self.overflows = 0
         self.keep_images = keep_images
         self.max_views = max(1, int(max_views))
         self.reserve = int(hbm_reserve_bytes)
@@ -149,6 +162,8 @@ class RenderServer:
             if dev.type == "cuda" else [None] * max(1, int(streams))
         self.pending: Deque[ViewRequest] = collections.deque()
         self.inflight: List[Optional[_InFlight]] = [None] * len(self.streams)
+        self._pipes = [dict() for _ in self.streams]
+        self._pools = [None] * len(self.streams)
         self.results: List[ViewResult] = []
         self._next_rid = 0
         self.batches = 0
@@ -170,16 +185,19 @@ class RenderServer:
 
     # -- scheduling -------------------------------------------------------------------------
     def _form_batch(self, budget: int) -> Tuple[List[ViewRequest], int]:
-        """The oldest pending request's scene, then every pending request of
-        that scene in arrival order, while the batch fits max_views and the
-        memory budget (the oldest request is always taken, so the queue
-        cannot starve)."""
+        """The oldest pending request, then every pending request that can
+        share its batch in arrival order -- with graphs, those of the same
+        image size over any scene; without, those of the same scene -- while
+        the batch fits max_views and the memory budget (the oldest request
+        is always taken, so the queue cannot starve)."""
         head = self.pending[0]
+        same = (lambda r: (r.width, r.height) == (head.width, head.height)) if self.graphs else \
+            (lambda r: r.scene == head.scene)
         batch, used, keep = [], 0, collections.deque()
         while self.pending:
             r = self.pending.popleft()
             need = self.store.view_bytes(r.scene, r.width, r.height)
-            if r.scene == head.scene and len(batch) < self.max_views and (not batch or used + need <= budget):
+            if same(r) and len(batch) < self.max_views and (not batch or used + need <= budget):
                 batch.append(r)
                 used += need
             else:
@@ -187,7 +205,30 @@ class RenderServer:
         self.pending = keep
         return batch, used
 
+    def _pipe(self, slot: int, W: int, H: int, n_views: int):
+        """The stream slot's graph for (W, H) and the smallest power-of-two
+        slot count holding n_views, captured on first use; its lists hold
+        cap_margin times the largest resident scene's estimated intersections
+        per slot."""
+        from .batch_graph import ViewBatchGraph
+
+        S = 1 << max(0, (n_views - 1).bit_length())
+        st = self.store
+        R = max(p.xyz.shape[0] for p in st.scenes.values())
+        key = (W, H, S, R)
+        g = self._pipes[slot].get(key)
+        if g is None:
+            area = (W * H) / (1920.0 * 1080.0)
+            per_view = max(st.isects_per_prim[k] * p.xyz.shape[0] for k, p in st.scenes.items()) * max(area, 0.05)
+            cap = int(min((1 << 31) - 1, math.ceil(self.cap_margin * per_view * S) + 1024))
+            g = ViewBatchGraph(st.device, S, W, H, R, cap, tile_size=self.tile_size)
+            g.needs_capture = True
+            self._pipes[slot][key] = g
+        return g
+
     def _launch(self, slot: int, batch: List[ViewRequest], nbytes: int) -> None:
+        if self.graphs:
+            return self._launch_graph(slot, batch, nbytes)
         params = self.store.scenes[batch[0].scene]
         s = self.streams[slot]
         dev = self.store.device
@@ -206,13 +247,72 @@ class RenderServer:
             ev = torch.cuda.Event() if s is not None else None
             if ev is not None:
                 ev.record(s)
-        self.inflight[slot] = _InFlight(s, ev, batch, images, alphas, nbytes)
+        self.inflight[slot] = _InFlight(s, ev, batch, images, alphas, nbytes, [params])
         self.batches += 1
+
+    def _launch_graph(self, slot: int, batch: List[ViewRequest], nbytes: int) -> None:
+        # slots ordered by scene: consecutive views of one scene share the
+        # per-primitive state in the projection
+        batch = sorted(batch, key=lambda r: r.scene)
+        W, H = batch[0].width, batch[0].height
+        dev = self.store.device
+        s = self.streams[slot]
+        caller = torch.cuda.current_stream(dev)
+        scenes_ = [self.store.scenes[r.scene] for r in batch]
+        with torch.no_grad(), torch.cuda.stream(s):
+            s.wait_stream(caller)
+            g = self._pipe(slot, W, H, len(batch))
+            g.set_views(scenes_, torch.stack([r.viewmat for r in batch]), torch.stack([r.K for r in batch]),
+                        [r.t for r in batch])
+            if g.needs_capture:
+                if self._pools[slot] is None:
+                    self._pools[slot] = torch.cuda.graph_pool_handle()
+                g.capture(pool=self._pools[slot])
+                g.needs_capture = False
+            g.replay()
+            # the graph's outputs are overwritten by its next replay: copies
+            images = [g.images[k].clone() for k in range(len(batch))]
+            alphas = [g.alphas[k].clone() for k in range(len(batch))]
+            ev = torch.cuda.Event()
+            ev.record(s)
+        self.inflight[slot] = _InFlight(s, ev, batch, images, alphas, nbytes, scenes_, g)
+        self.batches += 1
+
+    def _rerender(self, f: _InFlight) -> None:
+        """A graph batch whose intersections exceeded its lists' capacity:
+        render its views again through the host-sized path (one scene at a
+        time) and retire the graph, so the next batch of its shape is
+        captured with twice the room."""
+        self.overflows += 1
+        g = f.graph
+        for key, v in list(self._pipes[self.streams.index(f.stream)].items()):
+            if v is g:
+                del self._pipes[self.streams.index(f.stream)][key]
+        self.cap_margin *= 2.0
+        dev = self.store.device
+        with torch.no_grad():
+            for k, r in enumerate(f.reqs):
+                p = self.store.scenes.get(r.scene) or f.params[k]
+                ts = torch.tensor([r.t], dtype=torch.float32, device=dev) if p.D == 7 else None
+                imgs, alphas = self.render_fn(p, r.viewmat.to(dev, torch.float32)[None].contiguous(),
+                                              r.K.to(dev, torch.float32)[None].contiguous(), [(r.width, r.height)],
+                                              timestamps=ts, tile_size=self.tile_size)
+                f.images[k], f.alphas[k] = imgs[0], alphas[0]
 
     def _collect(self, slot: int, now: float) -> None:
         f = self.inflight[slot]
+        if f.graph is not None and f.graph.overflowed():
+            self._rerender(f)
+            now = time.perf_counter()
+        consumer = torch.cuda.current_stream(self.store.device) if f.stream is not None else None
         for r, img, a in zip(f.reqs, f.images, f.alphas):
             keep = self.keep_images
+            if keep and consumer is not None and consumer != f.stream:
+                # the results were allocated on the batch's stream: when the caller
+                # drops them, their memory returns to that stream's pool only after
+                # what the caller's stream has queued by then has run
+                img.record_stream(consumer)
+                a.record_stream(consumer)
             self.results.append(ViewResult(r.rid, r.scene, img if keep else None, a if keep else None, r.arrival,
                                            now, len(f.reqs), r.width * r.height))
         self.inflight[slot] = None
