@@ -373,6 +373,23 @@ int ubs_sort_tile_segments(int64_t n, int64_t *keys_in, int32_t *vals_in, int64_
                            const int32_t *tile_offsets, uint32_t C, uint32_t tile_width, uint32_t tile_height,
                            uint32_t skip_passes, void *stream);
 
+/* The same stable sort (all passes of [begin_bit, end_bit)) with no
+ * workspace argument and no allocation: its histograms, tile counters and
+ * look-back slots live in a 24 MiB device array of the library, one per GPU,
+ * held until the process ends.  Sorts whose scratch would not fit (more than
+ * 16777216 keys at 8 passes; ubs_radix_sort_workspace_bytes tells) are
+ * rejected with UBS_ERR_INVALID before anything is enqueued.  status_dev
+ * (device uint32, may be NULL) receives the status word that
+ * ubs_radix_sort_status reads for the workspace form.  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU (two streams, two
+ * host threads) race and must be serialised by the caller.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under hipGraph capture, serving loops, harnesses that forbid
+ * hipMalloc). */
+int ubs_radix_sort_pairs_noalloc(int64_t n, int64_t *keys, int64_t *keys_alt, int32_t *vals, int32_t *vals_alt,
+                                 int begin_bit, int end_bit, int *result_in_alt, uint32_t *status_dev,
+                                 void *stream);
+
 /* Reads the sort's device-side status word (0 = fine, nonzero = a look-back
  * wait hit its spin limit and the result is invalid).  Synchronises stream. */
 int ubs_radix_sort_status(const void *workspace, int *status, void *stream);

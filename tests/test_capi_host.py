@@ -74,6 +74,18 @@ def test_argument_validation_without_gpu(L):
     assert rc == 1
 
 
+def test_noalloc_sort_rejects_before_any_hip_call(L):
+    """The allocation-free sort checks its bit range and whether the sort's
+    scratch fits its fixed device array before it touches the device."""
+    sel = ctypes.c_int(0)
+    rc = L.ubs_radix_sort_pairs_noalloc(10, None, None, None, None, 10, 5, ctypes.byref(sel), None, None)
+    assert rc == 1 and "bit range" in L.last_error()
+    rc = L.ubs_radix_sort_pairs_noalloc(1 << 25, None, None, None, None, 0, 64, ctypes.byref(sel), None, None)
+    assert rc == 1 and "allocation-free" in L.last_error(), L.last_error()
+    assert L.ubs_radix_sort_workspace_bytes(1 << 24, 0, 64) <= 24 << 20 < L.ubs_radix_sort_workspace_bytes(
+        1 << 25, 0, 64)
+
+
 def test_prefix_bits_match_reference_formula(L):
     import math
     for C, tw, th in ((1, 120, 68), (1, 16, 16), (8, 80, 45), (3, 1, 1), (1, 64, 64)):

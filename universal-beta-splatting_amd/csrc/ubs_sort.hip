@@ -16,6 +16,8 @@
 //   * every look-back wait is bounded: past the spin limit it raises the
 //     status word and stops waiting (the result is then invalid and the
 //     caller is told through ubs_radix_sort_status).
+#include <mutex>
+
 #include "ubs_common.h"
 
 namespace ubs {
@@ -324,6 +326,42 @@ __global__ __launch_bounds__(kSortThreads) void sort_pass_kernel(
     }
 }
 
+// Scratch of ubs_radix_sort_pairs_noalloc: the status word, digit counts,
+// tile counters and look-back slots of one sort, in a file-scope device
+// array instead of a caller workspace (24 MiB: 16M keys at all 8 passes,
+// more keys when fewer passes run).  Each device gets its own copy when the
+// library's code object loads there and keeps it until the process ends.
+// Host code must not pass the array's name to a kernel (it is a host-side
+// stand-in): it uses the device address that hipGetSymbolAddress returns
+// once per device, kept in g_scratch_ptr.
+constexpr size_t kNoAllocScratchBytes = size_t(24) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxDevices];
+
+// Device address of the current device's scratch (looked up on its first use).
+static int scratch_for_current_device(void **out) {
+    int dev = 0;
+    UBS_HIP_CHECK(hipGetDevice(&dev));
+    UBS_REQUIRE(dev >= 0 && dev < kMaxDevices, "device %d beyond the %d the scratch table holds", dev,
+                kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        UBS_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        UBS_REQUIRE(p != nullptr, "hipGetSymbolAddress returned a null scratch address on device %d", dev);
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return UBS_OK;
+}
+
+__global__ void copy_status_kernel(const uint32_t *__restrict__ src, uint32_t *__restrict__ dst) {
+    if (threadIdx.x == 0) dst[0] = src[0];
+}
+
 }  // namespace ubs
 
 using namespace ubs;
@@ -406,6 +444,30 @@ static int sort_impl(int64_t n, const int64_t *n_dev, int64_t *keys, int64_t *ke
         ++run;
     }
     *result_in_alt = run & 1;
+    return UBS_OK;
+}
+
+int ubs_radix_sort_pairs_noalloc(int64_t n, int64_t *keys, int64_t *keys_alt, int32_t *vals, int32_t *vals_alt,
+                                 int begin_bit, int end_bit, int *result_in_alt, uint32_t *status_dev,
+                                 void *stream) {
+    UBS_REQUIRE(result_in_alt, "result_in_alt is null");
+    UBS_REQUIRE(begin_bit >= 0 && end_bit <= 64 && begin_bit <= end_bit, "bad bit range [%d, %d)", begin_bit,
+                end_bit);
+    UBS_REQUIRE(n >= 0 && n <= (int64_t)kValueMask, "n=%lld out of range (max 2^30-1)", (long long)n);
+    const SortLayout L = sort_layout(n, begin_bit, end_bit);
+    UBS_REQUIRE(L.total <= kNoAllocScratchBytes,
+                "n=%lld keys over bits [%d, %d) need %zu bytes of sort scratch; the allocation-free sort holds "
+                "%zu (at most 16777216 keys at 8 passes); use ubs_radix_sort_pairs with a workspace",
+                (long long)n, begin_bit, end_bit, L.total, kNoAllocScratchBytes);
+    void *ws = nullptr;
+    int e = scratch_for_current_device(&ws);
+    if (e) return e;
+    e = ubs_radix_sort_pairs_skip(n, keys, keys_alt, vals, vals_alt, begin_bit, end_bit, 0u, ws,
+                                  kNoAllocScratchBytes, result_in_alt, stream);
+    if (e || !status_dev) return e;
+    copy_status_kernel<<<1, 64, 0, as_stream(stream)>>>(
+        reinterpret_cast<const uint32_t *>(reinterpret_cast<const char *>(ws) + L.status_off), status_dev);
+    UBS_LAUNCH_CHECK("copy_status_kernel");
     return UBS_OK;
 }
 

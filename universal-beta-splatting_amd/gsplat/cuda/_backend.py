@@ -75,6 +75,8 @@ _SIGNATURES = {
     "ubs_radix_sort_pairs_dev": (
         _c_int, [_c_i64, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _vp, _c_size, ctypes.POINTER(_c_int), _vp]),
     "ubs_isect_offset_encode_dev": (_c_int, [_c_i64, _vp, _vp, _c_u32, _c_u32, _c_u32, _vp, _vp]),
+    "ubs_radix_sort_pairs_noalloc": (
+        _c_int, [_c_i64, _vp, _vp, _vp, _vp, _c_int, _c_int, ctypes.POINTER(_c_int), _vp, _vp]),
     "ubs_radix_sort_status": (_c_int, [_vp, ctypes.POINTER(_c_int), _vp]),
     "ubs_isect_depth_keys": (_c_int, [_c_u32, _c_u32, _vp, _vp, _vp, _vp, _vp]),
     "ubs_isect_ordered_workspace_bytes": (_c_size, [_c_u32, _c_u32]),
