@@ -405,11 +405,13 @@ int ubs_isect_offset_encode(int64_t n_isects, const int64_t *isect_ids, uint32_t
  * Tile blending.  tile_size in {8, 16, 32}; any channel count >= 1.
  * ------------------------------------------------------------------------ */
 
-/* Workspace of the blend entry points: room for one 32-byte record per
- * (camera, primitive) -- x, y, conic, opacity, beta -- that the forward and
- * the backward gather per intersection instead of four separate arrays, and
- * in the backward as much again for the packed gradients.  A NULL workspace
- * selects the unpacked path (same results, ~4x the gather traffic). */
+/* Workspace of the blend entry points: room for one 64-byte record per
+ * (camera, primitive) -- x, y, conic, opacity, beta and, for blends of at
+ * most 4 channels in one pass, the colours -- that the forward and the
+ * backward gather per intersection instead of five separate arrays, and in
+ * the backward as much again for the packed gradients.  A NULL workspace
+ * selects the unpacked path (same results within rounding, more gather
+ * traffic). */
 size_t ubs_rasterize_workspace_bytes(uint32_t C, uint32_t N, int backward);
 
 /* means2d [C,N,2], conics [C,N,3], colors [C,N,channels], opacities [C,N],

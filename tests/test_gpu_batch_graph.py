@@ -118,6 +118,9 @@ def test_server_graph_batches_cross_scene(device, two_scenes):
     st = SceneStore(device)
     st.add("s6", a)
     st.add("s7", b)
+    vm = torch.tensor(scenes.look_at_viewmat(np.array([0.0, 0.5, 4.0])), dtype=torch.float32)
+    for name in st.scenes:
+        st.calibrate(name, vm, torch.tensor(scenes.intrinsics(1920, 1080), dtype=torch.float32))
     reqs = _requests(14, ["s6", "s7"], [(128, 96), (200, 150)], 5)
     out, srv = _serve(st, reqs, max_views=8, streams=2)
     assert srv.overflows == 0

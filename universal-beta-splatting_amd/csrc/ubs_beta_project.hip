@@ -280,7 +280,10 @@ __global__ __launch_bounds__(kThreads, D == 7 ? UBS_PROJ_BWD_WAVES_7 : UBS_PROJ_
     // through the view loop (which reads only the mean, Sigma, opacity and
     // Beta parameters): the same values, ~70 fewer registers live in the loop
     BetaPrim<D> bq;
-    beta_prim_setup<D, QUAT>(n, xyz, mean_rest, scale_raw, ltri, opacity_raw, beta_raw, quats, bq);
+    if constexpr (D == 7)
+        beta_prim_setup<D, QUAT>(n, xyz, mean_rest, scale_raw, ltri, opacity_raw, beta_raw, quats, bq);
+    else
+        bq = bp;  // 6-D fits its registers without the set-up again
     // a G whose G[a][b] + G[b][a] is gU exactly: the triangle above the
     // diagonal, zeros below, the diagonal as summed (x + 0 and d + d are exact)
     float gSig[D * D];

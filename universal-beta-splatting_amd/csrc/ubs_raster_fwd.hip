@@ -15,6 +15,12 @@
 
 #include "ubs_raster.h"
 
+// Read two compacted-list entries and both their records before blending
+// either (1), or one entry at a time (0).
+#ifndef UBS_FWD_PAIRED_READS
+#define UBS_FWD_PAIRED_READS 1
+#endif
+
 namespace ubs {
 
 // STATS (measurement builds only; the library's kernels have it off) counts
@@ -45,7 +51,7 @@ __global__ __launch_bounds__(TS *TS) void raster_fwd_kernel(
     __shared__ float4 s_colv[kColVec ? BS : 1];
     __shared__ float s_col[kColVec ? 1 : CDIM][kColVec ? 1 : BS];
     __shared__ uint32_t s_mask[BS];
-    __shared__ uint16_t s_list[NW][BS];  // byte offsets (16 x entry) of the wave's primitives
+    __shared__ __align__(16) uint16_t s_list[NW][BS];  // byte offsets (16 x entry) of the wave's primitives
 
     uint32_t logical;
     if (!xcd_tile(blockIdx.x, C, tw, th, logical)) return;
@@ -163,10 +169,8 @@ __global__ __launch_bounds__(TS *TS) void raster_fwd_kernel(
         for (int k0 = 0; k0 < n_mine; k0 += 8) {
             if (__ballot(!done) == 0ull) break;  // whole wave done (checked every 8)
             const int kend = min(k0 + 8, n_mine);
-            for (int k = k0; k < kend; ++k) {
-                const int off = s_list[wave][k];
-                const float4 g1 = *reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(s_geo) + off);
-                const float4 g2 = *reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(s_geo2) + off);
+            // one primitive of the compacted list: the pixel's front-to-back step
+            auto blend_one = [&](const int off, const float4 g1, const float4 g2, const float4 cv) {
                 const float dx = g1.x - px, dy = g1.y - py;
                 const float sigma = __builtin_fmaf(g1.w * dx, dy, __builtin_fmaf(g1.z * dx, dx, (g2.x * dy) * dy));
                 const bool ok = !done && sigma >= 0.f && sigma < 1.f;
@@ -185,7 +189,6 @@ __global__ __launch_bounds__(TS *TS) void raster_fwd_kernel(
                 done = done || stop;
                 const float vis = take ? alpha * T : 0.f;
                 if constexpr (kColVec) {
-                    const float4 cv = *reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(s_colv) + off);
                     const float c4[4] = {cv.x, cv.y, cv.z, cv.w};
 #pragma unroll
                     for (int q = 0; q < CDIM; ++q) acc[q] = __builtin_fmaf(c4[q], vis, acc[q]);
@@ -195,7 +198,40 @@ __global__ __launch_bounds__(TS *TS) void raster_fwd_kernel(
                 }
                 cur_off = take ? off : cur_off;
                 T = take ? nT : T;
+            };
+            auto rec_at = [&](const int off, float4 &g1, float4 &g2, float4 &cv) {
+                g1 = *reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(s_geo) + off);
+                g2 = *reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(s_geo2) + off);
+                cv = kColVec ? *reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(s_colv) + off)
+                             : make_float4(0.f, 0.f, 0.f, 0.f);
+            };
+#if UBS_FWD_PAIRED_READS
+            // two list entries per LDS read and both records read before either
+            // is blended: half the list reads, one LDS round trip per two steps
+            int k = k0;
+            for (; k + 1 < kend; k += 2) {
+                const uint32_t both = *reinterpret_cast<const uint32_t *>(&s_list[wave][k]);  // k is even
+                const int off0 = (int)(both & 0xffffu), off1 = (int)(both >> 16);
+                float4 a1, a2, ac, b1, b2, bc;
+                rec_at(off0, a1, a2, ac);
+                rec_at(off1, b1, b2, bc);
+                blend_one(off0, a1, a2, ac);
+                blend_one(off1, b1, b2, bc);
             }
+            if (k < kend) {
+                const int off = s_list[wave][k];
+                float4 g1, g2, cv;
+                rec_at(off, g1, g2, cv);
+                blend_one(off, g1, g2, cv);
+            }
+#else
+            for (int k = k0; k < kend; ++k) {
+                const int off = s_list[wave][k];
+                float4 g1, g2, cv;
+                rec_at(off, g1, g2, cv);
+                blend_one(off, g1, g2, cv);
+            }
+#endif
         }
         if (cur_off >= 0) cur = batch_start + (cur_off >> 4);
         // T after this batch, for a backward that restarts each batch from it

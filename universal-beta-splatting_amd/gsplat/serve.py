@@ -33,6 +33,10 @@ from torch import Tensor
 from .beta import BetaParams, render_batch
 
 
+# Capacity bound of a graph batch's intersection lists (the sort's key limit).
+GRAPH_MAX_ISECTS = (1 << 30) - 1
+
+
 @dataclass
 class ViewRequest:
     """One view of a resident scene: world->camera [4,4] row-major,
@@ -99,15 +103,22 @@ class SceneStore:
     def resident_bytes(self) -> int:
         return sum(t.numel() * t.element_size() for p in self.scenes.values() for t in p.leaves().values())
 
+    def view_isects(self, name: str, width: int, height: int) -> float:
+        """Estimated intersections of one view of `name` at width x height.
+        A primitive's footprint in pixels scales with the image width (the
+        focal length); with k intersections per primitive measured at 1080p
+        (calibrate), its tile box is about sqrt(k) tiles wide there, so
+        (s (sqrt(k) - 1) + 1)^2 tiles at scale s = width / 1920."""
+        k = max(1.0, self.isects_per_prim[name])
+        sc = width / 1920.0
+        return self.scenes[name].xyz.shape[0] * (sc * (math.sqrt(k) - 1.0) + 1.0) ** 2
+
     def view_bytes(self, name: str, width: int, height: int) -> int:
         """Working memory one view of `name` needs: projected rows (36 B per
         primitive), the intersection keys/values in both sort buffers (24 B
-        each, at the scene's observed intersections per primitive relative
-        to a 1080p view), count/prefix arrays, and the image."""
+        each, view_isects of them), count/prefix arrays, and the image."""
         N = self.scenes[name].xyz.shape[0]
-        area = (width * height) / (1920.0 * 1080.0)
-        n_isect = self.isects_per_prim[name] * N * max(area, 0.05)
-        return int(36 * N + 12 * N + 24 * n_isect + 24 * width * height)
+        return int(36 * N + 12 * N + 24 * self.view_isects(name, width, height) + 24 * width * height)
 
 
 def free_hbm_bytes(device) -> int:
@@ -193,11 +204,12 @@ class RenderServer:
         head = self.pending[0]
         same = (lambda r: (r.width, r.height) == (head.width, head.height)) if self.graphs else \
             (lambda r: r.scene == head.scene)
+        most = self._graph_max_views(head.width, head.height) if self.graphs else self.max_views
         batch, used, keep = [], 0, collections.deque()
         while self.pending:
             r = self.pending.popleft()
             need = self.store.view_bytes(r.scene, r.width, r.height)
-            if same(r) and len(batch) < self.max_views and (not batch or used + need <= budget):
+            if same(r) and len(batch) < most and (not batch or used + need <= budget):
                 batch.append(r)
                 used += need
             else:
@@ -205,22 +217,61 @@ class RenderServer:
         self.pending = keep
         return batch, used
 
+    def _graph_view_isects(self, W: int, H: int) -> float:
+        """A graph slot's share of the lists: the largest estimate over the
+        resident scenes (any scene may take any slot)."""
+        return max(self.store.view_isects(k, W, H) for k in self.store.scenes)
+
+    def _graph_max_views(self, W: int, H: int) -> int:
+        """Most views of one graph batch at W x H: max_views, lowered so that
+        the lists' capacity stays within what the sort holds."""
+        per = max(1.0, self.cap_margin * self._graph_view_isects(W, H))
+        return max(1, min(self.max_views, int(GRAPH_MAX_ISECTS // per)))
+
+    def prepare_graphs(self, sizes) -> int:
+        """Capture, on every stream, the graph of each batch shape the server
+        can form at these image sizes (every view count up to the size's
+        limit), so that no capture falls inside serving.  Returns the number
+        of graphs captured."""
+        import numpy as np
+
+        from . import scenes as _scenes
+
+        if not self.graphs:
+            return 0
+        p = next(iter(self.store.scenes.values()))
+        vm = torch.tensor(_scenes.look_at_viewmat(np.array([0.0, 0.4, 4.0])), dtype=torch.float32)
+        made = 0
+        for W, H in sizes:
+            K = torch.tensor(_scenes.intrinsics(W, H), dtype=torch.float32)
+            for n in range(1, self._graph_max_views(W, H) + 1):
+                for slot, s in enumerate(self.streams):
+                    g = self._pipe(slot, W, H, n)
+                    if not g.needs_capture:
+                        continue
+                    with torch.no_grad(), torch.cuda.stream(s):
+                        g.set_views([p] * n, vm[None].expand(n, 4, 4), K[None].expand(n, 3, 3), [0.5] * n)
+                        if self._pools[slot] is None:
+                            self._pools[slot] = torch.cuda.graph_pool_handle()
+                        g.capture(pool=self._pools[slot])
+                        g.needs_capture = False
+                    made += 1
+        torch.cuda.synchronize(self.store.device)
+        return made
+
     def _pipe(self, slot: int, W: int, H: int, n_views: int):
-        """The stream slot's graph for (W, H) and the smallest power-of-two
-        slot count holding n_views, captured on first use; its lists hold
-        cap_margin times the largest resident scene's estimated intersections
-        per slot."""
+        """The stream slot's graph for (W, H) and n_views slots, captured on
+        first use (or by prepare_graphs); its lists hold cap_margin times the
+        largest resident scene's estimated intersections per slot."""
         from .batch_graph import ViewBatchGraph
 
-        S = 1 << max(0, (n_views - 1).bit_length())
+        S = int(n_views)
         st = self.store
         R = max(p.xyz.shape[0] for p in st.scenes.values())
         key = (W, H, S, R)
         g = self._pipes[slot].get(key)
         if g is None:
-            area = (W * H) / (1920.0 * 1080.0)
-            per_view = max(st.isects_per_prim[k] * p.xyz.shape[0] for k, p in st.scenes.items()) * max(area, 0.05)
-            cap = int(min((1 << 31) - 1, math.ceil(self.cap_margin * per_view * S) + 1024))
+            cap = min(GRAPH_MAX_ISECTS, int(math.ceil(self.cap_margin * self._graph_view_isects(W, H) * S)) + 1024)
             g = ViewBatchGraph(st.device, S, W, H, R, cap, tile_size=self.tile_size)
             g.needs_capture = True
             self._pipes[slot][key] = g
