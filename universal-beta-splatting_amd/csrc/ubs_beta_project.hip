@@ -195,10 +195,12 @@ __global__ __launch_bounds__(kThreads) void beta_project_fwd_table_kernel(
     }
 }
 
-// Minimum waves per SIMD of the 7-D backward.  Unbounded it takes 317
-// registers (one wave per SIMD); held to two waves it spills 55 values to
-// scratch and still runs faster (0.262 vs 0.291 ms at 1M primitives,
-// 1352x1014).  The 6-D kernel fits two waves by itself (236 registers).
+// Minimum waves per SIMD of the 7-D backward.  Unbounded it once took 317
+// registers (one wave per SIMD), and held to two waves it spilled 55 values
+// to scratch yet ran faster (0.262 vs 0.291 ms at 1M primitives,
+// 1352x1014); with the covariance sums in LDS and the leaves read again
+// after the view loop it fits two waves in 227 registers, no scratch
+// (0.227 ms).  The 6-D kernel fits two waves by itself (225 registers).
 #ifndef UBS_PROJ_BWD_WAVES_7
 #define UBS_PROJ_BWD_WAVES_7 2
 #endif
@@ -231,11 +233,27 @@ __global__ __launch_bounds__(kThreads, D == 7 ? UBS_PROJ_BWD_WAVES_7 : UBS_PROJ_
     // part, upper triangle only: gU[a][b] = sum of G[a][b] + G[b][a] for
     // a < b and of G[a][a] on the diagonal (28 sums instead of 49 for 7-D).
     constexpr int NU = D * (D + 1) / 2;
-    float gmean[D], gU[NU], gop = 0.f, gbeta[D - 2];
+    // 7-D: the NU sums live in LDS, one column per thread (stride kThreads:
+    // the 64 lanes of a wave hit 64 consecutive words), not in registers.
+    // The view loop is the kernel's register peak, and 28 fewer live values
+    // there leave it 227 registers at two waves per SIMD, no scratch (0.227
+    // against 0.229 ms with them in registers at 256, at 1M primitives,
+    // 1352x1014).  6-D keeps them in registers (same time for one view, 1%
+    // faster for eight).
+    constexpr bool kLdsSums = D == 7;
+    __shared__ float s_gU[kLdsSums ? NU : 1][kLdsSums ? kThreads : 1];
+    float gUr[kLdsSums ? 1 : NU];
+    auto gU = [&](int u) -> float & {
+        if constexpr (kLdsSums)
+            return s_gU[u][threadIdx.x];
+        else
+            return gUr[u];
+    };
+    float gmean[D], gop = 0.f, gbeta[D - 2];
 #pragma unroll
     for (int k = 0; k < D; ++k) gmean[k] = 0.f;
 #pragma unroll
-    for (int k = 0; k < NU; ++k) gU[k] = 0.f;
+    for (int k = 0; k < NU; ++k) gU(k) = 0.f;
 #pragma unroll
     for (int k = 0; k < D - 2; ++k) gbeta[k] = 0.f;
     for (uint32_t c = 0; c < C; ++c) {
@@ -268,22 +286,43 @@ __global__ __launch_bounds__(kThreads, D == 7 ? UBS_PROJ_BWD_WAVES_7 : UBS_PROJ_
             for (int a = 0; a < D; ++a)
 #pragma unroll
                 for (int b = a; b < D; ++b, ++u)
-                    gU[u] += a == b ? gV[a * D + a] : gV[a * D + b] + gV[b * D + a];
+                    gU(u) += a == b ? gV[a * D + a] : gV[a * D + b] + gV[b * D + a];
         }
         gop += go;
         gbeta[0] += v_betas[idx];
 #pragma unroll
         for (int k = 0; k < CQ; ++k) gbeta[1 + k] += gb[k];
     }
+    // the mean, opacity and Beta gradients are final after the view loop:
+    // written before the covariance VJP, whose peak then holds 13 values fewer
+#pragma unroll
+    for (int k = 0; k < 3; ++k) v_xyz[n * 3 + k] = gmean[k];
+#pragma unroll
+    for (int k = 3; k < D; ++k) v_mean_rest[n * (D - 3) + (k - 3)] = gmean[k];
+    {
+        const float o = sigmoid_f(opacity_raw[n]);
+        v_opacity_raw[n] = gop * (o * (1.f - o));
+    }
+#pragma unroll
+    for (int k = 0; k < D - 2; ++k) {
+        const int64_t i = n * (D - 2) + k;
+        v_beta_raw[i] = gbeta[k] * (4.f * expf(beta_raw[i]));
+    }
     // covariance, rotation and activation VJPs, once per primitive.  The
     // rotation, scales and L factor are set up again here rather than kept
     // through the view loop (which reads only the mean, Sigma, opacity and
     // Beta parameters): the same values, ~70 fewer registers live in the loop
+    // (the leaves are read again through an index the compiler cannot
+    // match with the first reads, so no loaded value is kept live through
+    // the loop in their place)
     BetaPrim<D> bq;
-    if constexpr (D == 7)
-        beta_prim_setup<D, QUAT>(n, xyz, mean_rest, scale_raw, ltri, opacity_raw, beta_raw, quats, bq);
-    else
+    int64_t n2 = n;
+    if constexpr (D == 7) {
+        asm volatile("" : "+v"(n2));
+        beta_prim_setup<D, QUAT>(n2, xyz, mean_rest, scale_raw, ltri, opacity_raw, beta_raw, quats, bq);
+    } else {
         bq = bp;  // 6-D fits its registers without the set-up again
+    }
     // a G whose G[a][b] + G[b][a] is gU exactly: the triangle above the
     // diagonal, zeros below, the diagonal as summed (x + 0 and d + d are exact)
     float gSig[D * D];
@@ -293,7 +332,7 @@ __global__ __launch_bounds__(kThreads, D == 7 ? UBS_PROJ_BWD_WAVES_7 : UBS_PROJ_
         for (int a = 0; a < D; ++a)
 #pragma unroll
             for (int b = a; b < D; ++b, ++u) {
-                gSig[a * D + b] = gU[u];
+                gSig[a * D + b] = gU(u);
                 if (b != a) gSig[b * D + a] = 0.f;
             }
     }
@@ -303,7 +342,7 @@ __global__ __launch_bounds__(kThreads, D == 7 ? UBS_PROJ_BWD_WAVES_7 : UBS_PROJ_
     if constexpr (QUAT) {
         float qv[4], vq[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) qv[k] = quats[n * 4 + k];
+        for (int k = 0; k < 4; ++k) qv[k] = quats[n2 * 4 + k];
         quat_rot_vjp(qv, dR, vq);
 #pragma unroll
         for (int k = 0; k < 4; ++k) v_quats[n * 4 + k] = vq[k];
@@ -314,20 +353,7 @@ __global__ __launch_bounds__(kThreads, D == 7 ? UBS_PROJ_BWD_WAVES_7 : UBS_PROJ_
         v_ltri[n * FULL_M + 2] = dR[5] - dR[7];
     }
 #pragma unroll
-    for (int k = 0; k < 3; ++k) v_xyz[n * 3 + k] = gmean[k];
-#pragma unroll
-    for (int k = 3; k < D; ++k) v_mean_rest[n * (D - 3) + (k - 3)] = gmean[k];
-#pragma unroll
-    for (int k = 0; k < D; ++k) v_scale_raw[n * D + k] = dS[k] * softplus_grad(scale_raw[n * D + k]);
-    {
-        const float o = sigmoid_f(opacity_raw[n]);
-        v_opacity_raw[n] = gop * (o * (1.f - o));
-    }
-#pragma unroll
-    for (int k = 0; k < D - 2; ++k) {
-        const int64_t i = n * (D - 2) + k;
-        v_beta_raw[i] = gbeta[k] * (4.f * expf(beta_raw[i]));
-    }
+    for (int k = 0; k < D; ++k) v_scale_raw[n * D + k] = dS[k] * softplus_grad(scale_raw[n2 * D + k]);
 }
 
 }  // namespace ubs

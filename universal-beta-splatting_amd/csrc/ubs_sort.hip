@@ -53,7 +53,11 @@ constexpr uint32_t kFlagAgg = 1u << 30;
 constexpr uint32_t kFlagPrefix = 2u << 30;
 constexpr uint32_t kValueMask = (1u << 30) - 1u;
 constexpr uint32_t kSpinLimit = 1u << 22;
-constexpr int kLookback = 8;  // predecessor slots read per look-back round trip
+// predecessor slots read per look-back round trip
+#ifndef UBS_SORT_LOOKBACK
+#define UBS_SORT_LOOKBACK 8
+#endif
+constexpr int kLookback = UBS_SORT_LOOKBACK;
 
 struct SortLayout {
     size_t status_off, hist_off, counter_off, lookback_off, total;
@@ -92,28 +96,46 @@ __device__ __forceinline__ int64_t sort_count(int64_t n_cap, const int64_t *n_de
     return n_dev ? min(max(*n_dev, (int64_t)0), n_cap) : n_cap;
 }
 
+// Each wave reads kHistUnroll x 64 consecutive keys per round (all loads
+// issued before any is counted: one 8-byte load per lane at a time left the
+// kernel short of loads in flight to cover HBM latency).
+#ifndef UBS_SORT_HIST_UNROLL
+#define UBS_SORT_HIST_UNROLL 8
+#endif
+constexpr int kHistUnroll = UBS_SORT_HIST_UNROLL;
+
 __global__ void sort_hist_kernel(int64_t n_cap, const int64_t *__restrict__ n_dev, const int64_t *__restrict__ keys,
                                  int begin_bit, int end_bit, int passes, uint32_t *__restrict__ hist) {
     const int64_t n = sort_count(n_cap, n_dev);
     __shared__ uint32_t h[8][kRadix];
     for (int k = threadIdx.x; k < passes * kRadix; k += blockDim.x) h[k / kRadix][k % kRadix] = 0;
     __syncthreads();
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    const int64_t first = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63);  // wave-uniform
-    for (int64_t w0 = first; w0 < n; w0 += stride) {
-        const int64_t i = w0 + (threadIdx.x & 63);
-        const bool ok = i < n;
-        const int64_t key = ok ? keys[i] : 0;
-        const uint64_t active = __ballot(ok);
-        for (int p = 0; p < passes; ++p) {
-            const int shift = begin_bit + p * kRadixBits;
-            const int bits = min(kRadixBits, end_bit - shift);
-            const uint32_t d = digit_of(key, shift, (1u << bits) - 1u);
-            const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);  // lane 0 is always active here
-            if (__ballot(ok && d == d0) == active) {
-                if ((threadIdx.x & 63) == 0) atomicAdd(&h[p][d0], (uint32_t)__popcll(active));
-            } else if (ok) {
-                atomicAdd(&h[p][d], 1u);
+    const int lane = threadIdx.x & 63;
+    constexpr int64_t kChunk = 64 * kHistUnroll;
+    const int64_t waves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;  // wave-uniform
+    for (int64_t c0 = wave * kChunk; c0 < n; c0 += waves * kChunk) {
+        int64_t key[kHistUnroll];
+#pragma unroll
+        for (int u = 0; u < kHistUnroll; ++u) {
+            const int64_t i = c0 + u * 64 + lane;
+            key[u] = i < n ? keys[i] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < kHistUnroll; ++u) {
+            const bool ok = c0 + u * 64 + lane < n;
+            const uint64_t active = __ballot(ok);
+            if (active == 0ull) break;  // (wave-uniform) past the last key
+            for (int p = 0; p < passes; ++p) {
+                const int shift = begin_bit + p * kRadixBits;
+                const int bits = min(kRadixBits, end_bit - shift);
+                const uint32_t d = digit_of(key[u], shift, (1u << bits) - 1u);
+                const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);  // lane 0 is active when any lane is
+                if (__ballot(ok && d == d0) == active) {
+                    if (lane == 0) atomicAdd(&h[p][d0], (uint32_t)__popcll(active));
+                } else if (ok) {
+                    atomicAdd(&h[p][d], 1u);
+                }
             }
         }
     }
