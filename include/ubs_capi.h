@@ -414,6 +414,17 @@ int ubs_isect_offset_encode(int64_t n_isects, const int64_t *isect_ids, uint32_t
  * traffic). */
 size_t ubs_rasterize_workspace_bytes(uint32_t C, uint32_t N, int backward);
 
+/* The same workspace with room, after it, for the order in which the blend
+ * deals its 2x2 tile blocks to the GPU: heaviest first (by their list
+ * lengths), where ubs_rasterize_workspace_bytes gives raster order.  Any
+ * blend entry point given at least this many bytes computes that order from
+ * tile_offsets and uses it; results are the same as in raster order (the
+ * backward's float atomics add in another order: same within rounding).
+ * No counterpart in the reference (its launch grid is in raster order,
+ * rasterize_to_pixels_fwd.cu:235-236). */
+size_t ubs_rasterize_workspace_bytes_tiles(uint32_t C, uint32_t N, uint32_t tile_width, uint32_t tile_height,
+                                           int backward);
+
 /* means2d [C,N,2], conics [C,N,3], colors [C,N,channels], opacities [C,N],
  * betas [C,N], backgrounds [C,channels] or NULL, masks uint8 [C,th,tw] or
  * NULL, tile_offsets [C,th,tw], flatten_ids [n_isects]

@@ -500,6 +500,37 @@ def test_blend_packed_records_match_unpacked(device, C, channels, ts):
         assert rel_l2(host(b), host(a).astype(np.float64)) < 1e-5, n
 
 
+@pytest.mark.parametrize("C,channels,ts,mask", [(2, 3, 16, False), (1, 3, 8, True), (3, 4, 32, False),
+                                                (1, 7, 16, False)])
+def test_blend_tile_order_same_results(device, C, channels, ts, mask):
+    """Tile blocks dealt heaviest first (the default, the workspace of
+    ubs_rasterize_workspace_bytes_tiles) against raster order: the forward
+    bit for bit, the backward within the rounding of its atomics."""
+    W_, H_ = 232, 152
+    x = _raster_inputs(lambda **k: opaque_scene(2500, opacity=(0.2, 0.95), **k), W_, H_, C=C, ts=ts,
+                       channels=channels)
+    ins = [cu(x[k]) for k in ("m2", "cn", "col", "op", "be")]
+    g = np.random.Generator(np.random.PCG64(11))
+    masks = cu(g.uniform(0, 1, (C, x["th"], x["tw"])) < 0.8) if mask else None
+    bgs = cu(g.uniform(0, 1, (C, channels)).astype(np.float32))
+    res = {}
+    try:
+        for ordered in (False, True):
+            W.set_blend_tile_order(ordered)
+            fwd = W.raster_fwd_raw(*ins, bgs, masks, W_, H_, ts, cu(x["off"]), cu(x["flat"]))
+            vrc = cu(np.random.Generator(np.random.PCG64(8)).normal(0, 1, tuple(fwd[0].shape)).astype(np.float32))
+            vra = torch.ones_like(fwd[1])
+            bwd = W.raster_bwd_raw(*ins, bgs, masks, W_, H_, ts, cu(x["off"]), cu(x["flat"]), fwd[1], fwd[2],
+                                   fwd[3], vrc, vra)
+            res[ordered] = (fwd, bwd)
+    finally:
+        W.set_blend_tile_order(True)
+    for a, b in zip(res[False][0], res[True][0]):
+        assert torch.equal(a, b)
+    for a, b, n in zip(res[False][1], res[True][1], ("v_means2d", "v_conics", "v_colors", "v_opac", "v_betas")):
+        assert rel_l2(host(b), host(a).astype(np.float64)) < 1e-6, n
+
+
 @pytest.mark.parametrize("case", ["random", "dense_ties", "dense_4_bytes", "two_cams"])
 def test_segmented_depth_sort_matches_full_sort(device, case):
     """Every sort mode -- depth-first emission + onesweep on the camera/tile

@@ -80,14 +80,31 @@ MAP_DRIVER = r"""
 #include "ubs_raster.h"
 #include <cstdio>
 #include <cstdlib>
+#include <random>
 #include <vector>
+// argv: C tw th [seed]: with a seed, the blocks are dealt in a random order
+// (a permutation, as raster_block_order_kernel writes), plus one entry
+// past the block count, which must be refused
 int main(int argc, char **argv) {
     const uint32_t C = atoi(argv[1]), tw = atoi(argv[2]), th = atoi(argv[3]);
     const uint32_t total = C * tw * th, grid = ubs::xcd_grid(C, tw, th);
+    uint32_t nbx;
+    const uint32_t blocks = ubs::xcd_blocks(C, tw, th, nbx);
+    std::vector<uint32_t> order(blocks);
+    for (uint32_t k = 0; k < blocks; ++k) order[k] = k;
+    const bool ordered = argc > 4;
+    if (ordered) {
+        std::mt19937 rng(atoi(argv[4]));
+        std::shuffle(order.begin(), order.end(), rng);
+        uint32_t t;
+        std::vector<uint32_t> bad(blocks, blocks);  // every entry out of range
+        for (uint32_t b = 0; b < grid; ++b)
+            if (ubs::xcd_tile(b, C, tw, th, t, bad.data())) { printf("accepted a bad order entry\n"); return 1; }
+    }
     std::vector<int> seen(total, 0), xcd(total, -1);
     for (uint32_t b = 0; b < grid; ++b) {
         uint32_t t;
-        if (!ubs::xcd_tile(b, C, tw, th, t)) continue;
+        if (!ubs::xcd_tile(b, C, tw, th, t, ordered ? order.data() : nullptr)) continue;
         if (t >= total) { printf("out of range %u\n", t); return 1; }
         seen[t] += 1;
         xcd[t] = b % ubs::kNumXcd;
@@ -104,7 +121,9 @@ int main(int argc, char **argv) {
 @pytest.mark.skipif(shutil.which("hipcc") is None, reason="hipcc not on PATH")
 def test_workgroup_tile_map_is_a_balanced_bijection(tmp_path):
     """Every (camera, tile) is blended by exactly one workgroup of the launch
-    grid, and a centre-heavy image spreads evenly over the 8 XCDs."""
+    grid, in raster block order and in any other order of the blocks (an
+    order entry past the block count is refused), and a centre-heavy image
+    spreads evenly over the 8 XCDs."""
     import numpy as np
 
     src = tmp_path / "map.cpp"
@@ -116,6 +135,10 @@ def test_workgroup_tile_map_is_a_balanced_bijection(tmp_path):
     for C, tw, th in ((1, 120, 68), (1, 1, 1), (3, 5, 7), (2, 85, 64), (1, 16, 16), (8, 240, 135)):
         out = subprocess.run([str(exe), str(C), str(tw), str(th)], capture_output=True, text=True, timeout=300)
         assert out.returncode == 0, out.stdout
+        for seed in (1, 2):  # blocks dealt in another order (heaviest first on the GPU): still a bijection
+            o2 = subprocess.run([str(exe), str(C), str(tw), str(th), str(seed)], capture_output=True, text=True,
+                                timeout=300)
+            assert o2.returncode == 0, o2.stdout
         if (C, tw, th) == (1, 120, 68):
             xcd = np.array(out.stdout.split(), dtype=np.int64).reshape(th, tw)
             yy, xx = np.mgrid[0:th, 0:tw]

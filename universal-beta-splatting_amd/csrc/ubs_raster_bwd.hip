@@ -50,7 +50,8 @@ __global__ __launch_bounds__(TS *TS, UBS_BWD_WAVES_PER_EU) void raster_bwd_kerne
     const float *__restrict__ v_render_colors, const float *__restrict__ v_render_alphas,
     float *__restrict__ v_means2d, float *__restrict__ v_conics, float *__restrict__ v_colors,
     float *__restrict__ v_opacities, float *__restrict__ v_betas, const float4 *__restrict__ rec, int wide_rec,
-    float *__restrict__ vrec, const float *__restrict__ T_ckpt, unsigned long long *__restrict__ stats = nullptr) {
+    float *__restrict__ vrec, const float *__restrict__ T_ckpt, unsigned long long *__restrict__ stats = nullptr,
+    const uint32_t *__restrict__ order = nullptr) {
     constexpr int BS = TS * TS;
     constexpr int SB = TS / 8;
     constexpr int NW = BS / 64;
@@ -68,7 +69,7 @@ __global__ __launch_bounds__(TS *TS, UBS_BWD_WAVES_PER_EU) void raster_bwd_kerne
     __shared__ int32_t s_wmax[NW];
 
     uint32_t logical;
-    if (!xcd_tile(blockIdx.x, C, tw, th, logical)) return;
+    if (!xcd_tile(blockIdx.x, C, tw, th, logical, order)) return;
     const uint32_t cam = logical / (th * tw);
     const uint32_t tile = logical % (th * tw);
     const uint32_t ty = tile / tw, tx = tile % tw;
@@ -428,16 +429,16 @@ static int launch_bwd(bool f64t, uint32_t C, uint32_t N, int64_t n_isects, uint3
                       uint32_t tw, uint32_t th, const int32_t *offs, const int32_t *flat, const float *ra,
                       const int32_t *last, const float *Tf, const float *vrc, const float *vra, float *vm2,
                       float *vcon, float *vcol, float *vop, float *vbe, const float4 *rec, float *vrec,
-                      const float *ck, hipStream_t st) {
+                      const float *ck, const uint32_t *order, hipStream_t st) {
     const int wide = rec && raster_wide_records(channels, TS) ? 1 : 0;
     if (f64t)
         raster_bwd_kernel<TS, CDIM, true><<<xcd_grid(C, tw, th), TS * TS, 0, st>>>(
             C, N, n_isects, channels, coff, means2d, conics, colors, opac, betas, bg, masks, W, H, tw, th, offs, flat,
-            ra, last, Tf, vrc, vra, vm2, vcon, vcol, vop, vbe, rec, wide, vrec, ck);
+            ra, last, Tf, vrc, vra, vm2, vcon, vcol, vop, vbe, rec, wide, vrec, ck, nullptr, order);
     else
         raster_bwd_kernel<TS, CDIM, false><<<xcd_grid(C, tw, th), TS * TS, 0, st>>>(
             C, N, n_isects, channels, coff, means2d, conics, colors, opac, betas, bg, masks, W, H, tw, th, offs, flat,
-            ra, last, Tf, vrc, vra, vm2, vcon, vcol, vop, vbe, rec, wide, vrec, ck);
+            ra, last, Tf, vrc, vra, vm2, vcon, vcol, vop, vbe, rec, wide, vrec, ck, nullptr, order);
     UBS_LAUNCH_CHECK("raster_bwd_kernel");
     return UBS_OK;
 }
@@ -449,10 +450,10 @@ static int launch_bwd_chunk(int cd, bool f64t, uint32_t C, uint32_t N, int64_t n
                             uint32_t H, uint32_t tw, uint32_t th, const int32_t *offs, const int32_t *flat,
                             const float *ra, const int32_t *last, const float *Tf, const float *vrc,
                             const float *vra, float *vm2, float *vcon, float *vcol, float *vop, float *vbe,
-                            const float4 *rec, float *vrec, const float *ck, hipStream_t st) {
+                            const float4 *rec, float *vrec, const float *ck, const uint32_t *order, hipStream_t st) {
 #define UBS_BWD_ARGS                                                                                            \
     f64t, C, N, n_isects, channels, coff, means2d, conics, colors, opac, betas, bg, masks, W, H, tw, th, offs,  \
-        flat, ra, last, Tf, vrc, vra, vm2, vcon, vcol, vop, vbe, rec, vrec, ck, st
+        flat, ra, last, Tf, vrc, vra, vm2, vcon, vcol, vop, vbe, rec, vrec, ck, order, st
     switch (cd) {
     case 1: return launch_bwd<TS, 1>(UBS_BWD_ARGS);
     case 2: return launch_bwd<TS, 2>(UBS_BWD_ARGS);
@@ -518,6 +519,7 @@ static int raster_bwd_impl(uint32_t C, uint32_t N, int64_t n_isects, uint32_t ch
     }
     const float4 *rec = nullptr;
     float *vrec = nullptr;
+    const uint32_t *order = nullptr;
     const bool wide = packed && raster_wide_records(channels, tile_size);
     int e = wide ? UBS_OK : zero_async(v_colors, CN * channels * sizeof(float), st);
     if (packed) {
@@ -532,6 +534,13 @@ static int raster_bwd_impl(uint32_t C, uint32_t N, int64_t n_isects, uint32_t ch
                                                                        wide ? colors : nullptr, channels);
         UBS_LAUNCH_CHECK("raster_pack_kernel");
         rec = r;
+        const size_t base = raster_workspace_bytes(C, N, 1);
+        if (tile_offsets && workspace_bytes >= base + raster_order_bytes(C, tw, th)) {  // room for the block order
+            uint32_t *o = reinterpret_cast<uint32_t *>(static_cast<char *>(workspace) + base);
+            raster_block_order_kernel<<<1, kOrderThreads, 0, st>>>(C, tw, th, tile_offsets, n_isects, nullptr, o);
+            UBS_LAUNCH_CHECK("raster_block_order_kernel");
+            order = o;
+        }
     } else {
         if (!e) e = zero_async(v_means2d, CN * 2 * sizeof(float), st);
         if (!e) e = zero_async(v_conics, CN * 3 * sizeof(float), st);
@@ -559,17 +568,17 @@ static int raster_bwd_impl(uint32_t C, uint32_t N, int64_t n_isects, uint32_t ch
             e = launch_bwd_chunk<16>(cd, f64t, C, N, n_isects, channels, coff, means2d, conics, colors, opacities,
                                      betas, backgrounds, masks, W, H, tw, th, tile_offsets, flatten_ids,
                                      render_alphas, last_ids, T_final, v_render_colors, v_render_alphas, v_means2d,
-                                     v_conics, v_colors, v_opacities, v_betas, rec, vrec, T_ckpt, st);
+                                     v_conics, v_colors, v_opacities, v_betas, rec, vrec, T_ckpt, order, st);
         else if (tile_size == 8)
             e = launch_bwd_chunk<8>(cd, f64t, C, N, n_isects, channels, coff, means2d, conics, colors, opacities,
                                     betas, backgrounds, masks, W, H, tw, th, tile_offsets, flatten_ids,
                                     render_alphas, last_ids, T_final, v_render_colors, v_render_alphas, v_means2d,
-                                    v_conics, v_colors, v_opacities, v_betas, rec, vrec, T_ckpt, st);
+                                    v_conics, v_colors, v_opacities, v_betas, rec, vrec, T_ckpt, order, st);
         else
             e = launch_bwd_chunk<32>(cd, f64t, C, N, n_isects, channels, coff, means2d, conics, colors, opacities,
                                      betas, backgrounds, masks, W, H, tw, th, tile_offsets, flatten_ids,
                                      render_alphas, last_ids, T_final, v_render_colors, v_render_alphas, v_means2d,
-                                     v_conics, v_colors, v_opacities, v_betas, rec, vrec, T_ckpt, st);
+                                     v_conics, v_colors, v_opacities, v_betas, rec, vrec, T_ckpt, order, st);
         if (e) return e;
         coff += (uint32_t)cd;
     }

@@ -36,7 +36,8 @@ __global__ __launch_bounds__(TS *TS) void raster_fwd_kernel(
     const int32_t *__restrict__ tile_offsets, const int32_t *__restrict__ flatten_ids,
     float *__restrict__ render_colors, float *__restrict__ render_alphas, int32_t *__restrict__ last_ids,
     float *__restrict__ T_final, const float4 *__restrict__ rec, int wide_rec, float *__restrict__ T_ckpt,
-    unsigned long long *__restrict__ stats = nullptr, const int64_t *__restrict__ n_dev = nullptr) {
+    unsigned long long *__restrict__ stats = nullptr, const int64_t *__restrict__ n_dev = nullptr,
+    const uint32_t *__restrict__ order = nullptr) {
     // n_dev: the intersection count in device memory (at most n_isects, the
     // capacity of the lists), for callers that size the lists on the device
     if (n_dev) n_isects = min(max(*n_dev, (int64_t)0), n_isects);
@@ -54,7 +55,7 @@ __global__ __launch_bounds__(TS *TS) void raster_fwd_kernel(
     __shared__ __align__(16) uint16_t s_list[NW][BS];  // byte offsets (16 x entry) of the wave's primitives
 
     uint32_t logical;
-    if (!xcd_tile(blockIdx.x, C, tw, th, logical)) return;
+    if (!xcd_tile(blockIdx.x, C, tw, th, logical, order)) return;
     const uint32_t cam = logical / (th * tw);
     const uint32_t tile = logical % (th * tw);
     const uint32_t ty = tile / tw, tx = tile % tw;
@@ -265,10 +266,11 @@ static int launch_fwd(uint32_t C, uint32_t N, int64_t n_isects, uint32_t channel
                       const float *means2d, const float *conics, const float *colors, const float *opac,
                       const float *betas, const float *bg, const uint8_t *masks, uint32_t W, uint32_t H,
                       uint32_t tw, uint32_t th, const int32_t *offs, const int32_t *flat, float *rc, float *ra,
-                      int32_t *last, float *Tf, const float4 *rec, float *ck, const int64_t *n_dev, hipStream_t st) {
+                      int32_t *last, float *Tf, const float4 *rec, float *ck, const int64_t *n_dev,
+                      const uint32_t *order, hipStream_t st) {
     raster_fwd_kernel<TS, CDIM><<<xcd_grid(C, tw, th), TS * TS, 0, st>>>(
         C, N, n_isects, channels, coff, means2d, conics, colors, opac, betas, bg, masks, W, H, tw, th, offs, flat,
-        rc, ra, last, Tf, rec, rec && raster_wide_records(channels, TS) ? 1 : 0, ck, nullptr, n_dev);
+        rc, ra, last, Tf, rec, rec && raster_wide_records(channels, TS) ? 1 : 0, ck, nullptr, n_dev, order);
     UBS_LAUNCH_CHECK("raster_fwd_kernel");
     return UBS_OK;
 }
@@ -279,11 +281,11 @@ static int launch_fwd_chunk(int cd, uint32_t C, uint32_t N, int64_t n_isects, ui
                             const float *betas, const float *bg, const uint8_t *masks, uint32_t W, uint32_t H,
                             uint32_t tw, uint32_t th, const int32_t *offs, const int32_t *flat, float *rc,
                             float *ra, int32_t *last, float *Tf, const float4 *rec, float *ck, const int64_t *n_dev,
-                            hipStream_t st) {
+                            const uint32_t *order, hipStream_t st) {
 #define UBS_FWD_CASE(K)                                                                                     \
     case K:                                                                                                 \
         return launch_fwd<TS, K>(C, N, n_isects, channels, coff, means2d, conics, colors, opac, betas, bg,  \
-                                 masks, W, H, tw, th, offs, flat, rc, ra, last, Tf, rec, ck, n_dev, st);
+                                 masks, W, H, tw, th, offs, flat, rc, ra, last, Tf, rec, ck, n_dev, order, st);
     switch (cd) {
         UBS_FWD_CASE(1)
         UBS_FWD_CASE(2)
@@ -292,17 +294,17 @@ static int launch_fwd_chunk(int cd, uint32_t C, uint32_t N, int64_t n_isects, ui
     case 8:
         if constexpr (TS <= 16)
             return launch_fwd<TS, 8>(C, N, n_isects, channels, coff, means2d, conics, colors, opac, betas, bg,
-                                     masks, W, H, tw, th, offs, flat, rc, ra, last, Tf, rec, ck, n_dev, st);
+                                     masks, W, H, tw, th, offs, flat, rc, ra, last, Tf, rec, ck, n_dev, order, st);
         break;
     case 16:
         if constexpr (TS <= 16)
             return launch_fwd<TS, 16>(C, N, n_isects, channels, coff, means2d, conics, colors, opac, betas, bg,
-                                      masks, W, H, tw, th, offs, flat, rc, ra, last, Tf, rec, ck, n_dev, st);
+                                      masks, W, H, tw, th, offs, flat, rc, ra, last, Tf, rec, ck, n_dev, order, st);
         break;
     case 32:
         if constexpr (TS <= 16)
             return launch_fwd<TS, 32>(C, N, n_isects, channels, coff, means2d, conics, colors, opac, betas, bg,
-                                      masks, W, H, tw, th, offs, flat, rc, ra, last, Tf, rec, ck, n_dev, st);
+                                      masks, W, H, tw, th, offs, flat, rc, ra, last, Tf, rec, ck, n_dev, order, st);
         break;
     default:
         break;
@@ -342,6 +344,7 @@ static int raster_fwd_impl(uint32_t C, uint32_t N, int64_t n_isects, uint32_t ch
         }
     }
     const float4 *rec = nullptr;
+    const uint32_t *order = nullptr;
     if (workspace) {
         const size_t need = raster_workspace_bytes(C, N, 0);
         if (workspace_bytes < need) {
@@ -356,6 +359,12 @@ static int raster_fwd_impl(uint32_t C, uint32_t N, int64_t n_isects, uint32_t ch
                 channels);
             UBS_LAUNCH_CHECK("raster_pack_kernel");
             rec = static_cast<const float4 *>(workspace);
+            if (workspace_bytes >= need + raster_order_bytes(C, tw, th)) {  // room for the block order
+                uint32_t *o = reinterpret_cast<uint32_t *>(static_cast<char *>(workspace) + need);
+                raster_block_order_kernel<<<1, kOrderThreads, 0, st>>>(C, tw, th, tile_offsets, n_isects, n_dev, o);
+                UBS_LAUNCH_CHECK("raster_block_order_kernel");
+                order = o;
+            }
         }
     }
     uint32_t coff = 0;
@@ -365,15 +374,15 @@ static int raster_fwd_impl(uint32_t C, uint32_t N, int64_t n_isects, uint32_t ch
         if (tile_size == 16)
             e = launch_fwd_chunk<16>(cd, C, N, n_isects, channels, coff, means2d, conics, colors, opacities, betas,
                                      backgrounds, masks, W, H, tw, th, tile_offsets, flatten_ids, render_colors,
-                                     render_alphas, last_ids, T_final, rec, T_ckpt, n_dev, st);
+                                     render_alphas, last_ids, T_final, rec, T_ckpt, n_dev, order, st);
         else if (tile_size == 8)
             e = launch_fwd_chunk<8>(cd, C, N, n_isects, channels, coff, means2d, conics, colors, opacities, betas,
                                     backgrounds, masks, W, H, tw, th, tile_offsets, flatten_ids, render_colors,
-                                    render_alphas, last_ids, T_final, rec, T_ckpt, n_dev, st);
+                                    render_alphas, last_ids, T_final, rec, T_ckpt, n_dev, order, st);
         else
             e = launch_fwd_chunk<32>(cd, C, N, n_isects, channels, coff, means2d, conics, colors, opacities, betas,
                                      backgrounds, masks, W, H, tw, th, tile_offsets, flatten_ids, render_colors,
-                                     render_alphas, last_ids, T_final, rec, T_ckpt, n_dev, st);
+                                     render_alphas, last_ids, T_final, rec, T_ckpt, n_dev, order, st);
         if (e) return e;
         coff += (uint32_t)cd;
     }
@@ -430,6 +439,11 @@ extern "C" size_t ubs_transmittance_checkpoint_floats(uint32_t C, int64_t n_isec
 
 extern "C" size_t ubs_rasterize_workspace_bytes(uint32_t C, uint32_t N, int backward) {
     return raster_workspace_bytes(C, N, backward);
+}
+
+extern "C" size_t ubs_rasterize_workspace_bytes_tiles(uint32_t C, uint32_t N, uint32_t tile_width,
+                                                      uint32_t tile_height, int backward) {
+    return raster_workspace_bytes(C, N, backward) + raster_order_bytes(C, tile_width, tile_height);
 }
 
 #ifdef UBS_PROBE_BUILD

@@ -35,7 +35,7 @@ from torch import Tensor
 
 from .beta import BetaParams
 from .cuda._backend import lib
-from .cuda._wrapper import _call, camera_centers, ptr, stream_of
+from .cuda._wrapper import _call, blend_workspace_bytes, camera_centers, ptr, stream_of
 
 
 class _ViewScene(ctypes.Structure):
@@ -170,7 +170,7 @@ class ViewBatchGraph:
         ids, flat = self._sort(ids, flat, cap, self.info, 32, self.end_bit)
         offsets = torch.empty((S, self.th, self.tw), dtype=torch.int32, device=dev)
         _call("ubs_isect_offset_encode_dev", cap, ptr(self.info), ptr(ids), S, self.tw, self.th, ptr(offsets), st)
-        bws = int(L.ubs_rasterize_workspace_bytes(S, R, 0))
+        bws = blend_workspace_bytes(S, R, self.tw, self.th, False)
         bw = torch.empty((bws,), dtype=torch.uint8, device=dev)
         last = torch.empty((S, self.H, self.W), dtype=torch.int32, device=dev)
         _call("ubs_rasterize_to_pixels_fwd_dev", S, R, cap, ptr(self.info), 3, ptr(means2d), ptr(conics),

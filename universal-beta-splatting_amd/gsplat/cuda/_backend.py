@@ -88,6 +88,7 @@ _SIGNATURES = {
         _c_int, [_c_u32, _c_u32, _vp, _vp, _vp, _c_u32, _c_u32, _c_u32, _vp, _vp, _vp, _vp, _vp, _c_size, _vp]),
     "ubs_isect_offset_encode": (_c_int, [_c_i64, _vp, _c_u32, _c_u32, _c_u32, _vp, _vp]),
     "ubs_rasterize_workspace_bytes": (_c_size, [_c_u32, _c_u32, _c_int]),
+    "ubs_rasterize_workspace_bytes_tiles": (_c_size, [_c_u32, _c_u32, _c_u32, _c_u32, _c_int]),
     "ubs_rasterize_to_pixels_fwd": (
         _c_int, [_c_u32, _c_u32, _c_i64, _c_u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_u32, _c_u32, _c_u32,
                  _c_u32, _c_u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_size, _vp]),

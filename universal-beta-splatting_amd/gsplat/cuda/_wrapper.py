@@ -583,10 +583,31 @@ def set_blend_packed_records(enable: bool) -> None:
     _packed_records = bool(enable)
 
 
-def _blend_workspace(C: int, N: int, backward: bool, device):
+_tile_order = True
+
+
+def set_blend_tile_order(enable: bool) -> None:
+    """Deal the blend's tile blocks to the GPU heaviest first (default: the
+    workspace gets room for the order, ubs_rasterize_workspace_bytes_tiles) or
+    in raster order.  The forward's results are the same either way; the
+    backward's float atomics add in another order (same within rounding)."""
+    global _tile_order
+    _tile_order = bool(enable)
+
+
+def blend_workspace_bytes(C: int, N: int, tile_width: int, tile_height: int, backward: bool) -> int:
+    """Bytes of the blend workspace the wrappers pass (records, and the
+    block order unless set_blend_tile_order(False))."""
+    L = lib()
+    if _tile_order:
+        return int(L.ubs_rasterize_workspace_bytes_tiles(C, N, tile_width, tile_height, 1 if backward else 0))
+    return int(L.ubs_rasterize_workspace_bytes(C, N, 1 if backward else 0))
+
+
+def _blend_workspace(C: int, N: int, backward: bool, device, tile_width: int, tile_height: int):
     if not _packed_records or C * N == 0:
         return None, 0
-    nbytes = int(lib().ubs_rasterize_workspace_bytes(C, N, 1 if backward else 0))
+    nbytes = blend_workspace_bytes(C, N, tile_width, tile_height, backward)
     return torch.empty((nbytes,), dtype=torch.uint8, device=device), nbytes
 
 
@@ -619,7 +640,7 @@ def _raster_fwd(means2d, conics, colors, opacities, betas, backgrounds, masks, w
     last_ids = torch.empty((C, height, width), dtype=torch.int32, device=dev)
     T_final = torch.empty((C, height, width), dtype=torch.float32, device=dev)
     mp = _mask_ptr(masks)
-    ws, ws_bytes = _blend_workspace(C, N, False, dev)
+    ws, ws_bytes = _blend_workspace(C, N, False, dev, tw, th)
     if not checkpoint:
         _call("ubs_rasterize_to_pixels_fwd", C, N, flatten_ids.numel(), channels, ptr(means2d), ptr(conics),
               ptr(colors), ptr(opacities), ptr(betas), ptr(backgrounds), mp[0] if mp else None, width, height,
@@ -652,7 +673,7 @@ def raster_bwd_raw(means2d, conics, colors, opacities, betas, backgrounds, masks
     v_opacities = torch.empty_like(opacities)
     v_betas = torch.empty_like(betas)
     mp = _mask_ptr(masks)
-    ws, ws_bytes = _blend_workspace(C, N, True, means2d.device)
+    ws, ws_bytes = _blend_workspace(C, N, True, means2d.device, tw, th)
     if T_ckpt is None:
         _call("ubs_rasterize_to_pixels_bwd", C, N, flatten_ids.numel(), channels, ptr(means2d), ptr(conics),
               ptr(colors), ptr(opacities), ptr(betas), ptr(backgrounds), mp[0] if mp else None, width, height,
