@@ -94,13 +94,16 @@ __host__ __device__ inline size_t raster_order_bytes(uint32_t C, uint32_t tw, ui
 static __global__ __launch_bounds__(kOrderThreads) void raster_block_order_kernel(
     uint32_t C, uint32_t tw, uint32_t th, const int32_t *__restrict__ tile_offsets, int64_t n_isects,
     const int64_t *__restrict__ n_dev, uint32_t *__restrict__ order) {
+    constexpr uint32_t kCached = 8192;  // blocks whose work is kept in LDS between the passes
+    __shared__ uint32_t s_work[kCached];
     __shared__ uint32_t s_count[kOrderBuckets];
+    __shared__ uint32_t s_wave_sum[kOrderBuckets / 64];
     __shared__ uint32_t s_max;
     if (n_dev) n_isects = min(max(*n_dev, (int64_t)0), n_isects);
     uint32_t nbx;
     const uint32_t blocks = xcd_blocks(C, tw, th, nbx);
     const uint32_t per_cam = blocks / C, tiles = C * tw * th;
-    auto work = [&](uint32_t k) -> uint32_t {
+    auto work_of = [&](uint32_t k) -> uint32_t {
         const uint32_t cam = k / per_cam, kb = k % per_cam;
         const uint32_t x0 = (kb % nbx) * kXcdBlockX, y0 = (kb / nbx) * kXcdBlockY;
         uint32_t w = 0;
@@ -113,27 +116,44 @@ static __global__ __launch_bounds__(kOrderThreads) void raster_block_order_kerne
         }
         return w;
     };
-    for (uint32_t q = threadIdx.x; q < kOrderBuckets; q += kOrderThreads) s_count[q] = 0;
-    if (threadIdx.x == 0) s_max = 1;
+    auto work = [&](uint32_t k) -> uint32_t { return k < kCached ? s_work[k] : work_of(k); };
+    const uint32_t tid = threadIdx.x, lane = tid & 63;
+    for (uint32_t q = tid; q < kOrderBuckets; q += kOrderThreads) s_count[q] = 0;
+    if (tid == 0) s_max = 1;
     __syncthreads();
-    for (uint32_t k = threadIdx.x; k < blocks; k += kOrderThreads) atomicMax(&s_max, work(k));
-    __syncthreads();
-    const uint64_t wmax = s_max;
-    auto bucket = [&](uint32_t w) -> uint32_t {  // 255 = heaviest
-        return (uint32_t)min((uint64_t)w * kOrderBuckets / (wmax + 1), (uint64_t)kOrderBuckets - 1);
-    };
-    for (uint32_t k = threadIdx.x; k < blocks; k += kOrderThreads) atomicAdd(&s_count[bucket(work(k))], 1u);
-    __syncthreads();
-    if (threadIdx.x == 0) {  // counts -> first slot of each bucket, heaviest bucket at slot 0
-        uint32_t at = 0;
-        for (int q = (int)kOrderBuckets - 1; q >= 0; --q) {
-            const uint32_t c = s_count[q];
-            s_count[q] = at;
-            at += c;
-        }
+    for (uint32_t k = tid; k < blocks; k += kOrderThreads) {
+        const uint32_t w = work_of(k);
+        if (k < kCached) s_work[k] = w;
+        atomicMax(&s_max, w);
     }
     __syncthreads();
-    for (uint32_t k = threadIdx.x; k < blocks; k += kOrderThreads) order[atomicAdd(&s_count[bucket(work(k))], 1u)] = k;
+    const uint64_t wmax = s_max;
+    auto bucket = [&](uint32_t w) -> uint32_t {  // kOrderBuckets - 1 = heaviest
+        return (uint32_t)min((uint64_t)w * kOrderBuckets / (wmax + 1), (uint64_t)kOrderBuckets - 1);
+    };
+    for (uint32_t k = tid; k < blocks; k += kOrderThreads) atomicAdd(&s_count[bucket(work(k))], 1u);
+    __syncthreads();
+    // first slot of each bucket, heaviest bucket at slot 0: thread t scans
+    // bucket kOrderBuckets - 1 - t (a wave scan, then the earlier waves' sums)
+    uint32_t incl = 0, own = 0;
+    if (tid < kOrderBuckets) {
+        own = s_count[kOrderBuckets - 1 - tid];
+        incl = own;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t u = __shfl_up(incl, o, 64);
+            if ((int)lane >= o) incl += u;
+        }
+        if (lane == 63) s_wave_sum[tid >> 6] = incl;
+    }
+    __syncthreads();
+    if (tid < kOrderBuckets) {
+        uint32_t before = 0;
+        for (uint32_t w = 0; w < (tid >> 6); ++w) before += s_wave_sum[w];
+        s_count[kOrderBuckets - 1 - tid] = before + incl - own;
+    }
+    __syncthreads();
+    for (uint32_t k = tid; k < blocks; k += kOrderThreads) order[atomicAdd(&s_count[bucket(work(k))], 1u)] = k;
 }
 
 // Half extents of the sigma < 1 ellipse of conic [[a, b], [b, c]]:

@@ -59,6 +59,27 @@ constexpr uint32_t kSpinLimit = 1u << 22;
 #endif
 constexpr int kLookback = UBS_SORT_LOOKBACK;
 
+// Digit of pass p.  Byte-aligned (pass p: bits [8p, 8p + 8) of the range)
+// when passes may be skipped (the skip mask names depth bytes); otherwise the
+// range is split evenly, 14 bits as 7 + 7 rather than 8 + 6 (0.602 against
+// 0.606 ms for the isect stage at 1080p / 1M: the first pass writes runs of
+// equal digits twice as long).
+#ifndef UBS_SORT_BALANCED
+#define UBS_SORT_BALANCED 1
+#endif
+__host__ __device__ inline void pass_digit(int p, int begin_bit, int end_bit, bool balanced, int &shift, int &bits) {
+    const int total = end_bit - begin_bit;
+    if (UBS_SORT_BALANCED && balanced) {
+        const int passes = (total + kRadixBits - 1) / kRadixBits;
+        const int base = total / passes, extra = total % passes;
+        shift = begin_bit + p * base + (p < extra ? p : extra);
+        bits = base + (p < extra ? 1 : 0);
+    } else {
+        shift = begin_bit + p * kRadixBits;
+        bits = total - p * kRadixBits < kRadixBits ? total - p * kRadixBits : kRadixBits;
+    }
+}
+
 struct SortLayout {
     size_t status_off, hist_off, counter_off, lookback_off, total;
     int passes;
@@ -105,7 +126,7 @@ __device__ __forceinline__ int64_t sort_count(int64_t n_cap, const int64_t *n_de
 constexpr int kHistUnroll = UBS_SORT_HIST_UNROLL;
 
 __global__ void sort_hist_kernel(int64_t n_cap, const int64_t *__restrict__ n_dev, const int64_t *__restrict__ keys,
-                                 int begin_bit, int end_bit, int passes, uint32_t *__restrict__ hist) {
+                                 int begin_bit, int end_bit, int passes, int balanced, uint32_t *__restrict__ hist) {
     const int64_t n = sort_count(n_cap, n_dev);
     __shared__ uint32_t h[8][kRadix];
     for (int k = threadIdx.x; k < passes * kRadix; k += blockDim.x) h[k / kRadix][k % kRadix] = 0;
@@ -127,8 +148,8 @@ __global__ void sort_hist_kernel(int64_t n_cap, const int64_t *__restrict__ n_de
             const uint64_t active = __ballot(ok);
             if (active == 0ull) break;  // (wave-uniform) past the last key
             for (int p = 0; p < passes; ++p) {
-                const int shift = begin_bit + p * kRadixBits;
-                const int bits = min(kRadixBits, end_bit - shift);
+                int shift, bits;
+                pass_digit(p, begin_bit, end_bit, balanced != 0, shift, bits);
                 const uint32_t d = digit_of(key[u], shift, (1u << bits) - 1u);
                 const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);  // lane 0 is active when any lane is
                 if (__ballot(ok && d == d0) == active) {
@@ -445,7 +466,8 @@ static int sort_impl(int64_t n, const int64_t *n_dev, int64_t *keys, int64_t *ke
     uint32_t *counters = reinterpret_cast<uint32_t *>(ws + L.counter_off);
     uint32_t *lookback = reinterpret_cast<uint32_t *>(ws + L.lookback_off);
     const unsigned hist_blocks = (unsigned)std::min<int64_t>((n + 4095) / 4096, 2048);
-    sort_hist_kernel<<<hist_blocks, 256, 0, st>>>(n, n_dev, keys, begin_bit, end_bit, L.passes, hist);
+    sort_hist_kernel<<<hist_blocks, 256, 0, st>>>(n, n_dev, keys, begin_bit, end_bit, L.passes,
+                                                  skip_passes == 0 ? 1 : 0, hist);
     UBS_LAUNCH_CHECK("sort_hist_kernel");
     sort_hist_scan_kernel<<<L.passes, kRadix, 0, st>>>(hist);
     UBS_LAUNCH_CHECK("sort_hist_scan_kernel");
@@ -456,8 +478,8 @@ static int sort_impl(int64_t n, const int64_t *n_dev, int64_t *keys, int64_t *ke
         // a pass whose digit every key shares is the identity of a stable
         // sort: leaving it out changes nothing
         if ((skip_passes >> p) & 1u) continue;
-        const int shift = begin_bit + p * kRadixBits;
-        const int bits = std::min(kRadixBits, end_bit - shift);
+        int shift, bits;
+        pass_digit(p, begin_bit, end_bit, skip_passes == 0, shift, bits);
         auto *pass = n < kSmallSortN ? sort_pass_kernel<kItemsSmall> : sort_pass_kernel<kItems>;
         pass<<<(unsigned)L.tiles, kSortThreads, 0, st>>>(
             n, n_dev, kb[run & 1], kb[(run + 1) & 1], vb[run & 1], vb[(run + 1) & 1], shift, bits,
