@@ -415,7 +415,7 @@ int ubs_isect_offset_encode(int64_t n_isects, const int64_t *isect_ids, uint32_t
 size_t ubs_rasterize_workspace_bytes(uint32_t C, uint32_t N, int backward);
 
 /* The same workspace with room, after it, for the order in which the blend
- * deals its 2x2 tile blocks to the GPU: heaviest first (by their list
+ * deals its tiles to the GPU: heaviest first (by their list
  * lengths), where ubs_rasterize_workspace_bytes gives raster order.  Any
  * blend entry point given at least this many bytes computes that order from
  * tile_offsets and uses it; results are the same as in raster order (the
@@ -508,6 +508,28 @@ int ubs_rasterize_to_pixels_bwd_ckpt(uint32_t C, uint32_t N, int64_t n_isects, u
                                      float *v_means2d, float *v_conics, float *v_colors,
                                      float *v_opacities, float *v_betas, void *workspace,
                                      size_t workspace_bytes, void *stream);
+
+/* ubs_rasterize_to_pixels_bwd (T_ckpt NULL) or _bwd_ckpt that reuses the
+ * forward's packed primitive records and tile order instead of packing and
+ * ordering again: fwd_workspace is the workspace the forward of the same
+ * inputs was given, of ubs_rasterize_workspace_bytes_tiles(C, N, tile_width,
+ * tile_height, 0) bytes (read only); workspace then holds the packed
+ * gradients only, ubs_rasterize_workspace_bytes(C, N, 0) bytes.  Same
+ * results as the backward given one workspace.  No counterpart in the
+ * reference (its backward reads the separate arrays,
+ * rasterize_to_pixels_bwd.cu:138-151). */
+int ubs_rasterize_to_pixels_bwd_reuse(uint32_t C, uint32_t N, int64_t n_isects, uint32_t channels,
+                                      const float *means2d, const float *conics, const float *colors,
+                                      const float *opacities, const float *betas, const float *backgrounds,
+                                      const uint8_t *masks, uint32_t width, uint32_t height, uint32_t tile_size,
+                                      uint32_t tile_width, uint32_t tile_height, const int32_t *tile_offsets,
+                                      const int32_t *flatten_ids, const float *render_alphas,
+                                      const int32_t *last_ids, const float *T_final, const float *T_ckpt,
+                                      size_t ckpt_floats, const float *v_render_colors,
+                                      const float *v_render_alphas, float *v_means2d, float *v_conics,
+                                      float *v_colors, float *v_opacities, float *v_betas,
+                                      const void *fwd_workspace, size_t fwd_workspace_bytes, void *workspace,
+                                      size_t workspace_bytes, void *stream);
 
 /* ------------------------------------------------------------------------
  * Image loss of the training step: (1 - lambda) L1 + lambda (1 - SSIM).

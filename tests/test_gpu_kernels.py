@@ -531,6 +531,30 @@ def test_blend_tile_order_same_results(device, C, channels, ts, mask):
         assert rel_l2(host(b), host(a).astype(np.float64)) < 1e-6, n
 
 
+@pytest.mark.parametrize("C,channels,ts,ckpt", [(2, 3, 16, False), (1, 3, 16, True), (1, 7, 8, False),
+                                                (1, 40, 16, True)])
+def test_blend_backward_reuses_forward_workspace(device, C, channels, ts, ckpt):
+    """ubs_rasterize_to_pixels_bwd_reuse (the forward's packed records and
+    tile order, a gradient-only workspace) against the backward that packs
+    and orders again: same gradients within the rounding of its atomics, with
+    and without transmittance checkpoints, narrow and wide records."""
+    W_, H_ = 200, 136
+    x = _raster_inputs(lambda **k: opaque_scene(2000, opacity=(0.2, 0.95), **k), W_, H_, C=C, ts=ts,
+                       channels=channels)
+    ins = [cu(x[k]) for k in ("m2", "cn", "col", "op", "be")]
+    bgs = cu(np.random.Generator(np.random.PCG64(12)).uniform(0, 1, (C, channels)).astype(np.float32))
+    out = W._raster_fwd(*ins, bgs, None, W_, H_, ts, cu(x["off"]), cu(x["flat"]), ckpt)
+    rc, ra, last, Tf, T_ckpt, ws = out
+    assert ws is not None and (T_ckpt is not None) == ckpt
+    vrc = cu(np.random.Generator(np.random.PCG64(13)).normal(0, 1, tuple(rc.shape)).astype(np.float32))
+    vra = cu(np.random.Generator(np.random.PCG64(14)).normal(0, 1, tuple(ra.shape)).astype(np.float32))
+    args = (*ins, bgs, None, W_, H_, ts, cu(x["off"]), cu(x["flat"]), ra, last, Tf, vrc, vra)
+    plain = W.raster_bwd_raw(*args, T_ckpt=T_ckpt)
+    reuse = W.raster_bwd_raw(*args, T_ckpt=T_ckpt, fwd_workspace=ws)
+    for a, b, n in zip(plain, reuse, ("v_means2d", "v_conics", "v_colors", "v_opac", "v_betas")):
+        assert rel_l2(host(b), host(a).astype(np.float64)) < 1e-6, n
+
+
 @pytest.mark.parametrize("case", ["random", "dense_ties", "dense_4_bytes", "two_cams"])
 def test_segmented_depth_sort_matches_full_sort(device, case):
     """Every sort mode -- depth-first emission + onesweep on the camera/tile

@@ -82,24 +82,34 @@ MAP_DRIVER = r"""
 #include <cstdlib>
 #include <random>
 #include <vector>
-// argv: C tw th [seed]: with a seed, the blocks are dealt in a random order
-// (a permutation, as raster_block_order_kernel writes), plus one entry
-// past the block count, which must be refused
+// argv: C tw th [seed]: with a seed, each XCD runs its own tiles in a random
+// order (as raster_block_order_kernel writes them heaviest first), and an
+// order of nothing but padding must give no tile
 int main(int argc, char **argv) {
     const uint32_t C = atoi(argv[1]), tw = atoi(argv[2]), th = atoi(argv[3]);
     const uint32_t total = C * tw * th, grid = ubs::xcd_grid(C, tw, th);
-    uint32_t nbx;
-    const uint32_t blocks = ubs::xcd_blocks(C, tw, th, nbx);
-    std::vector<uint32_t> order(blocks);
-    for (uint32_t k = 0; k < blocks; ++k) order[k] = k;
+    // per XCD, the tiles raster order deals it, shuffled and padded with ~0u
+    const uint32_t slots = ubs::xcd_slots(C, tw, th);
+    std::vector<uint32_t> order((size_t)ubs::kNumXcd * slots, ~0u);
     const bool ordered = argc > 4;
     if (ordered) {
+        std::vector<std::vector<uint32_t>> mine(ubs::kNumXcd);
+        for (uint32_t b = 0; b < grid; ++b) {
+            uint32_t t;
+            if (!ubs::xcd_tile(b, C, tw, th, t)) continue;
+            if (ubs::xcd_of_tile(t, C, tw, th) != b % ubs::kNumXcd) { printf("xcd_of_tile disagrees\n"); return 1; }
+            mine[b % ubs::kNumXcd].push_back(t);
+        }
         std::mt19937 rng(atoi(argv[4]));
-        std::shuffle(order.begin(), order.end(), rng);
+        for (int x = 0; x < ubs::kNumXcd; ++x) {
+            if (mine[x].size() > slots) { printf("xcd %d holds %zu > %u tiles\n", x, mine[x].size(), slots); return 1; }
+            std::shuffle(mine[x].begin(), mine[x].end(), rng);
+            for (size_t j = 0; j < mine[x].size(); ++j) order[(size_t)x * slots + j] = mine[x][j];
+        }
         uint32_t t;
-        std::vector<uint32_t> bad(blocks, blocks);  // every entry out of range
+        std::vector<uint32_t> pad((size_t)ubs::kNumXcd * slots, ~0u);  // only padding: no tile at all
         for (uint32_t b = 0; b < grid; ++b)
-            if (ubs::xcd_tile(b, C, tw, th, t, bad.data())) { printf("accepted a bad order entry\n"); return 1; }
+            if (ubs::xcd_tile(b, C, tw, th, t, pad.data())) { printf("accepted a padding entry\n"); return 1; }
     }
     std::vector<int> seen(total, 0), xcd(total, -1);
     for (uint32_t b = 0; b < grid; ++b) {

@@ -24,9 +24,10 @@ __device__ __forceinline__ float beta_sigma(float a, float b2, float c, float dx
 // round-robin to the XCDs: every XCD gets an even share of an image whose
 // work is concentrated in its centre (contiguous bands per XCD left the
 // middle XCDs with 1.5x the average at 1080p), while the tiles of one block,
-// which share most of their primitives, stay in one L2.
+// which share most of their primitives, stay in one L2 (4x2, 2x4 and 4x4
+// blocks measured the same as 2x2 at 1080p / 1M).  With an order (below)
+// each XCD takes its own tiles heaviest first.
 constexpr int kNumXcd = 8;
-// (4x2, 2x4 and 4x4 blocks measured the same as 2x2 at 1080p / 1M)
 #ifndef UBS_XCD_BLOCK_X
 #define UBS_XCD_BLOCK_X 2
 #endif
@@ -49,22 +50,42 @@ __host__ __device__ inline uint32_t xcd_grid(uint32_t C, uint32_t tw, uint32_t t
     return (blocks + kNumXcd - 1) / kNumXcd * kNumXcd * kXcdBlockTiles;
 }
 
+// Tile slots per XCD: the most tiles any XCD is dealt.
+__host__ __device__ inline uint32_t xcd_slots(uint32_t C, uint32_t tw, uint32_t th) {
+    uint32_t nbx;
+    return (xcd_blocks(C, tw, th, nbx) + kNumXcd - 1) / kNumXcd * kXcdBlockTiles;
+}
+
+// XCD a tile is dealt to: that of its block.
+__host__ __device__ inline uint32_t xcd_of_tile(uint32_t t, uint32_t C, uint32_t tw, uint32_t th) {
+    uint32_t nbx;
+    const uint32_t blocks = xcd_blocks(C, tw, th, nbx);
+    const uint32_t per_cam = blocks / C, cam = t / (tw * th), rem = t % (tw * th);
+    const uint32_t k = cam * per_cam + (rem / tw) / kXcdBlockY * nbx + (rem % tw) / kXcdBlockX;
+    return k % kNumXcd;
+}
+
 // logical = cam * th * tw + ty * tw + tx of workgroup bid, or false.
-// order (may be null): the blocks in the order to deal them out (the k-th
-// block dealt is order[k]; raster_block_order_kernel puts the heaviest
-// first); null deals them in raster order.
+// order (may be null): per XCD, its tiles in the order to run them (the j-th
+// workgroup on XCD x runs tile order[x * xcd_slots + j]; entries past an
+// XCD's tiles are ~0u); raster_block_order_kernel writes them heaviest
+// first.  Null runs each XCD's blocks in raster order.
 __host__ __device__ inline bool xcd_tile(uint32_t bid, uint32_t C, uint32_t tw, uint32_t th, uint32_t &logical,
                                          const uint32_t *order = nullptr) {
+    if (order) {
+        const uint32_t slots = xcd_slots(C, tw, th), j = bid / kNumXcd;
+        if (j >= slots) return false;
+        const uint32_t t = order[(size_t)(bid % kNumXcd) * slots + j];
+        if (t >= C * tw * th) return false;  // a padding entry
+        logical = t;
+        return true;
+    }
     uint32_t nbx;
     const uint32_t blocks = xcd_blocks(C, tw, th, nbx);
     const uint32_t j = bid / kNumXcd;
-    uint32_t k = bid % kNumXcd + kNumXcd * (j / kXcdBlockTiles);  // block
-    const uint32_t r = j % kXcdBlockTiles;                        // tile inside it
+    const uint32_t k = bid % kNumXcd + kNumXcd * (j / kXcdBlockTiles);  // block
+    const uint32_t r = j % kXcdBlockTiles;                              // tile inside it
     if (k >= blocks) return false;
-    if (order) {
-        k = order[k];
-        if (k >= blocks) return false;  // (never, for an order the kernel below wrote)
-    }
     const uint32_t per_cam = blocks / C;
     const uint32_t cam = k / per_cam, kb = k % per_cam;
     const uint32_t tx = (kb % nbx) * kXcdBlockX + r % kXcdBlockX;
@@ -74,86 +95,115 @@ __host__ __device__ inline bool xcd_tile(uint32_t bid, uint32_t C, uint32_t tw, 
     return true;
 }
 
-// Heaviest blocks first.  A tile's blend time grows with its list, and the
-// lists of a centre-heavy image differ by ~3.5x between the mean and the
-// longest; dealt in raster order the heaviest blocks can start last and run
-// alone at the end of the launch.  One workgroup sorts the blocks by their
-// work (list entries + kTileCost per tile) into 256 buckets, heaviest bucket
-// first (longest-processing-time-first list scheduling, to the bucket width);
-// the order inside a bucket is whatever the atomics give (the blend's results
-// do not depend on the order its tiles run in).  n_dev as in the blend.
+// Heaviest tiles first, per XCD.  A tile's blend time grows with its list,
+// and the lists of a centre-heavy image differ by ~3.5x between the mean and
+// the longest; in raster order the heaviest tiles can start last and run
+// alone at the end of the launch.  One workgroup buckets every tile by its
+// work (list entries + kTileCost) into 256 buckets and writes, for each XCD,
+// the tiles its blocks hold, heaviest bucket first (longest-processing-time-
+// first list scheduling, to the bucket width, with each XCD keeping its own
+// tiles and so its L2 contents); the order inside a bucket is whatever the
+// atomics give (the blend's results do not depend on the order its tiles
+// run in).  n_dev as in the blend.
 constexpr uint32_t kOrderBuckets = 256;
 constexpr uint32_t kOrderThreads = 1024;
-constexpr uint32_t kTileCost = 64;
+#ifndef UBS_ORDER_TILE_COST
+#define UBS_ORDER_TILE_COST 64
+#endif
+constexpr uint32_t kTileCost = UBS_ORDER_TILE_COST;
 
 __host__ __device__ inline size_t raster_order_bytes(uint32_t C, uint32_t tw, uint32_t th) {
-    uint32_t nbx;
-    return ((size_t)xcd_blocks(C, tw, th, nbx) * sizeof(uint32_t) + 255) & ~(size_t)255;
+    return ((size_t)kNumXcd * xcd_slots(C, tw, th) * sizeof(uint32_t) + 255) & ~(size_t)255;
 }
 
 static __global__ __launch_bounds__(kOrderThreads) void raster_block_order_kernel(
     uint32_t C, uint32_t tw, uint32_t th, const int32_t *__restrict__ tile_offsets, int64_t n_isects,
     const int64_t *__restrict__ n_dev, uint32_t *__restrict__ order) {
-    constexpr uint32_t kCached = 8192;  // blocks whose work is kept in LDS between the passes
-    __shared__ uint32_t s_work[kCached];
-    __shared__ uint32_t s_count[kOrderBuckets];
-    __shared__ uint32_t s_wave_sum[kOrderBuckets / 64];
+    // the first kHeld x kOrderThreads tiles stay in registers between the
+    // passes (their offsets loaded together); tiles past them are read again
+    constexpr uint32_t kHeld = 8;
+    constexpr uint32_t kPerXcd = kOrderThreads / kNumXcd;     // 128 threads scan one XCD's buckets
+    constexpr uint32_t kPerThread = kOrderBuckets / kPerXcd;  // 2 buckets each
+    __shared__ uint32_t s_count[kNumXcd][kOrderBuckets];
+    __shared__ uint32_t s_total[kNumXcd];
+    __shared__ uint32_t s_wave_sum[kOrderThreads / 64];
     __shared__ uint32_t s_max;
     if (n_dev) n_isects = min(max(*n_dev, (int64_t)0), n_isects);
-    uint32_t nbx;
-    const uint32_t blocks = xcd_blocks(C, tw, th, nbx);
-    const uint32_t per_cam = blocks / C, tiles = C * tw * th;
-    auto work_of = [&](uint32_t k) -> uint32_t {
-        const uint32_t cam = k / per_cam, kb = k % per_cam;
-        const uint32_t x0 = (kb % nbx) * kXcdBlockX, y0 = (kb / nbx) * kXcdBlockY;
-        uint32_t w = 0;
-        for (uint32_t r = 0; r < kXcdBlockTiles; ++r) {
-            const uint32_t tx = x0 + r % kXcdBlockX, ty = y0 + r / kXcdBlockX;
-            if (tx >= tw || ty >= th) continue;
-            const uint32_t t = (cam * th + ty) * tw + tx;
-            const int64_t a = tile_offsets[t], b = t + 1 < tiles ? (int64_t)tile_offsets[t + 1] : n_isects;
-            w += kTileCost + (uint32_t)min(max(b - a, (int64_t)0), (int64_t)(1u << 24));
-        }
-        return w;
+    const uint32_t tiles = C * tw * th, slots = xcd_slots(C, tw, th);
+    auto work_of = [&](uint32_t t) -> uint32_t {
+        const int64_t a = tile_offsets[t], b = t + 1 < tiles ? (int64_t)tile_offsets[t + 1] : n_isects;
+        return kTileCost + (uint32_t)min(max(b - a, (int64_t)0), (int64_t)(1u << 24));
     };
-    auto work = [&](uint32_t k) -> uint32_t { return k < kCached ? s_work[k] : work_of(k); };
     const uint32_t tid = threadIdx.x, lane = tid & 63;
-    for (uint32_t q = tid; q < kOrderBuckets; q += kOrderThreads) s_count[q] = 0;
+    for (uint32_t q = tid; q < kNumXcd * kOrderBuckets; q += kOrderThreads) (&s_count[0][0])[q] = 0;
     if (tid == 0) s_max = 1;
-    __syncthreads();
-    for (uint32_t k = tid; k < blocks; k += kOrderThreads) {
-        const uint32_t w = work_of(k);
-        if (k < kCached) s_work[k] = w;
-        atomicMax(&s_max, w);
+    uint32_t held[kHeld];
+#pragma unroll
+    for (uint32_t u = 0; u < kHeld; ++u) {
+        const uint32_t t = tid + u * kOrderThreads;
+        held[u] = t < tiles ? work_of(t) : 0u;
     }
+    uint32_t mx = 0;
+#pragma unroll
+    for (uint32_t u = 0; u < kHeld; ++u) mx = max(mx, held[u]);
+    for (uint32_t t = tid + kHeld * kOrderThreads; t < tiles; t += kOrderThreads) mx = max(mx, work_of(t));
+    __syncthreads();  // (s_count and s_max initialised)
+    atomicMax(&s_max, mx);
     __syncthreads();
     const uint64_t wmax = s_max;
     auto bucket = [&](uint32_t w) -> uint32_t {  // kOrderBuckets - 1 = heaviest
         return (uint32_t)min((uint64_t)w * kOrderBuckets / (wmax + 1), (uint64_t)kOrderBuckets - 1);
     };
-    for (uint32_t k = tid; k < blocks; k += kOrderThreads) atomicAdd(&s_count[bucket(work(k))], 1u);
-    __syncthreads();
-    // first slot of each bucket, heaviest bucket at slot 0: thread t scans
-    // bucket kOrderBuckets - 1 - t (a wave scan, then the earlier waves' sums)
-    uint32_t incl = 0, own = 0;
-    if (tid < kOrderBuckets) {
-        own = s_count[kOrderBuckets - 1 - tid];
-        incl = own;
+    uint32_t key[kHeld];  // xcd << 8 | bucket of the held tiles
 #pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t u = __shfl_up(incl, o, 64);
-            if ((int)lane >= o) incl += u;
+    for (uint32_t u = 0; u < kHeld; ++u) {
+        const uint32_t t = tid + u * kOrderThreads;
+        key[u] = t < tiles ? (xcd_of_tile(t, C, tw, th) << 8) | bucket(held[u]) : 0u;
+        if (t < tiles) atomicAdd(&s_count[key[u] >> 8][key[u] & 255u], 1u);
+    }
+    for (uint32_t t = tid + kHeld * kOrderThreads; t < tiles; t += kOrderThreads)
+        atomicAdd(&s_count[xcd_of_tile(t, C, tw, th)][bucket(work_of(t))], 1u);
+    __syncthreads();
+    // first slot of each (XCD, bucket), heaviest bucket at slot 0: the 128
+    // threads of XCD x take its buckets two at a time from the heaviest down
+    const uint32_t x = tid / kPerXcd, i = tid % kPerXcd;
+    uint32_t c[kPerThread], own = 0;
+#pragma unroll
+    for (uint32_t u = 0; u < kPerThread; ++u) {
+        c[u] = s_count[x][kOrderBuckets - 1 - (kPerThread * i + u)];
+        own += c[u];
+    }
+    uint32_t incl = own;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t v = __shfl_up(incl, o, 64);
+        if ((int)lane >= o) incl += v;
+    }
+    if (lane == 63) s_wave_sum[tid >> 6] = incl;
+    __syncthreads();
+    uint32_t at = incl - own;
+    for (uint32_t w = x * (kPerXcd / 64); w < (tid >> 6); ++w) at += s_wave_sum[w];
+#pragma unroll
+    for (uint32_t u = 0; u < kPerThread; ++u) {
+        s_count[x][kOrderBuckets - 1 - (kPerThread * i + u)] = at;
+        at += c[u];
+    }
+    if (i == kPerXcd - 1) s_total[x] = at;  // the XCD's tile count
+    __syncthreads();
+#pragma unroll
+    for (uint32_t u = 0; u < kHeld; ++u) {
+        const uint32_t t = tid + u * kOrderThreads;
+        if (t < tiles) {
+            const uint32_t xt = key[u] >> 8;
+            order[(size_t)xt * slots + atomicAdd(&s_count[xt][key[u] & 255u], 1u)] = t;
         }
-        if (lane == 63) s_wave_sum[tid >> 6] = incl;
     }
-    __syncthreads();
-    if (tid < kOrderBuckets) {
-        uint32_t before = 0;
-        for (uint32_t w = 0; w < (tid >> 6); ++w) before += s_wave_sum[w];
-        s_count[kOrderBuckets - 1 - tid] = before + incl - own;
+    for (uint32_t t = tid + kHeld * kOrderThreads; t < tiles; t += kOrderThreads) {
+        const uint32_t xt = xcd_of_tile(t, C, tw, th);
+        order[(size_t)xt * slots + atomicAdd(&s_count[xt][bucket(work_of(t))], 1u)] = t;
     }
-    __syncthreads();
-    for (uint32_t k = tid; k < blocks; k += kOrderThreads) order[atomicAdd(&s_count[bucket(work(k))], 1u)] = k;
+    for (uint32_t q = tid; q < kNumXcd * slots; q += kOrderThreads)  // slots past each XCD's tiles
+        if (q % slots >= s_total[q / slots]) order[q] = ~0u;
 }
 
 // Half extents of the sigma < 1 ellipse of conic [[a, b], [b, c]]:

@@ -500,7 +500,7 @@ static int raster_bwd_impl(uint32_t C, uint32_t N, int64_t n_isects, uint32_t ch
                            const float *T_final, const float *v_render_colors, const float *v_render_alphas,
                            float *v_means2d, float *v_conics, float *v_colors, float *v_opacities, float *v_betas,
                            const float *T_ckpt, size_t ckpt_n, void *workspace, size_t workspace_bytes,
-                           void *stream) {
+                           void *stream, const void *fwd_ws = nullptr, size_t fwd_ws_bytes = 0) {
     hipStream_t st = as_stream(stream);
     UBS_REQUIRE(tile_size == 8 || tile_size == 16 || tile_size == 32, "tile_size must be 8, 16 or 32 (got %u)",
                 tile_size);
@@ -510,7 +510,15 @@ static int raster_bwd_impl(uint32_t C, uint32_t N, int64_t n_isects, uint32_t ch
     const uint64_t CN = (uint64_t)C * N;
     UBS_REQUIRE(CN == 0 || (v_means2d && v_conics && v_colors && v_opacities && v_betas), "null gradient output");
     const bool packed = workspace != nullptr && n_isects != 0 && (uint64_t)C * W * H != 0 && CN != 0;
-    if (workspace) {
+    if (fwd_ws) {  // the forward's records and order; this workspace holds the packed gradients only
+        UBS_REQUIRE(workspace, "a forward workspace needs a gradient workspace beside it");
+        const size_t need = raster_workspace_bytes(C, N, 0), need_fwd = need + raster_order_bytes(C, tw, th);
+        if (workspace_bytes < need || fwd_ws_bytes < need_fwd) {
+            set_error("blend workspaces of %zu (gradients) and %zu (forward) bytes, %zu and %zu needed",
+                      workspace_bytes, fwd_ws_bytes, need, need_fwd);
+            return UBS_ERR_WORKSPACE;
+        }
+    } else if (workspace) {
         const size_t need = raster_workspace_bytes(C, N, 1);
         if (workspace_bytes < need) {
             set_error("blend workspace of %zu bytes, %zu needed", workspace_bytes, need);
@@ -522,7 +530,15 @@ static int raster_bwd_impl(uint32_t C, uint32_t N, int64_t n_isects, uint32_t ch
     const uint32_t *order = nullptr;
     const bool wide = packed && raster_wide_records(channels, tile_size);
     int e = wide ? UBS_OK : zero_async(v_colors, CN * channels * sizeof(float), st);
-    if (packed) {
+    if (packed && fwd_ws) {
+        // the forward packed these records and wrote the tile order from the
+        // same inputs: only the gradient records are cleared here
+        vrec = static_cast<float *>(workspace);
+        if (!e) e = zero_async(vrec, CN * (wide ? kWideRecordFloats : kRecordFloats) * sizeof(float), st);
+        if (e) return e;
+        rec = static_cast<const float4 *>(fwd_ws);
+        order = reinterpret_cast<const uint32_t *>(static_cast<const char *>(fwd_ws) + raster_workspace_bytes(C, N, 0));
+    } else if (packed) {
         // records in the first half of the workspace, packed gradients in the
         // second; the scalar gradient tensors are written by the unpack below
         float4 *r = static_cast<float4 *>(workspace);
@@ -604,6 +620,24 @@ int ubs_rasterize_to_pixels_bwd(uint32_t C, uint32_t N, int64_t n_isects, uint32
                            H, tile_size, tw, th, tile_offsets, flatten_ids, render_alphas, last_ids, T_final,
                            v_render_colors, v_render_alphas, v_means2d, v_conics, v_colors, v_opacities, v_betas,
                            nullptr, 0, workspace, workspace_bytes, stream);
+}
+
+int ubs_rasterize_to_pixels_bwd_reuse(uint32_t C, uint32_t N, int64_t n_isects, uint32_t channels,
+                                      const float *means2d, const float *conics, const float *colors,
+                                      const float *opacities, const float *betas, const float *backgrounds,
+                                      const uint8_t *masks, uint32_t W, uint32_t H, uint32_t tile_size, uint32_t tw,
+                                      uint32_t th, const int32_t *tile_offsets, const int32_t *flatten_ids,
+                                      const float *render_alphas, const int32_t *last_ids, const float *T_final,
+                                      const float *T_ckpt, size_t ckpt_n, const float *v_render_colors,
+                                      const float *v_render_alphas, float *v_means2d, float *v_conics,
+                                      float *v_colors, float *v_opacities, float *v_betas, const void *fwd_workspace,
+                                      size_t fwd_workspace_bytes, void *workspace, size_t workspace_bytes,
+                                      void *stream) {
+    UBS_REQUIRE(fwd_workspace, "null forward workspace");
+    return raster_bwd_impl(C, N, n_isects, channels, means2d, conics, colors, opacities, betas, backgrounds, masks, W,
+                           H, tile_size, tw, th, tile_offsets, flatten_ids, render_alphas, last_ids, T_final,
+                           v_render_colors, v_render_alphas, v_means2d, v_conics, v_colors, v_opacities, v_betas,
+                           T_ckpt, ckpt_n, workspace, workspace_bytes, stream, fwd_workspace, fwd_workspace_bytes);
 }
 
 int ubs_rasterize_to_pixels_bwd_ckpt(uint32_t C, uint32_t N, int64_t n_isects, uint32_t channels,

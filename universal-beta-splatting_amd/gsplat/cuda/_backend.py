@@ -106,6 +106,10 @@ _SIGNATURES = {
         _c_int, [_c_u32, _c_u32, _c_i64, _c_u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_u32, _c_u32, _c_u32,
                  _c_u32, _c_u32, _vp, _vp, _vp, _vp, _vp, _vp, _c_size, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                  _c_size, _vp]),
+    "ubs_rasterize_to_pixels_bwd_reuse": (
+        _c_int, [_c_u32, _c_u32, _c_i64, _c_u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_u32, _c_u32, _c_u32,
+                 _c_u32, _c_u32, _vp, _vp, _vp, _vp, _vp, _vp, _c_size, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                 _c_size, _vp, _c_size, _vp]),
     "ubs_image_loss_workspace_bytes": (_c_size, [_c_i64, _c_i64, _c_i64, _c_i64]),
     "ubs_image_loss_fwd": (_c_int, [_c_i64, _c_i64, _c_i64, _c_i64, _vp, _vp, _c_f32, _vp, _vp, _c_size, _vp]),
     "ubs_image_loss_bwd": (_c_int, [_c_i64, _c_i64, _c_i64, _c_i64, _vp, _vp, _c_f32, _vp, _vp, _c_size, _vp, _vp]),

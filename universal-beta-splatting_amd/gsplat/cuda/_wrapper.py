@@ -623,11 +623,14 @@ def raster_fwd_ckpt_raw(means2d, conics, colors, opacities, betas, backgrounds, 
     """Blend forward that also stores the transmittance checkpoints:
     (render_colors, render_alphas, last_ids, T_final, T_ckpt)."""
     return _raster_fwd(means2d, conics, colors, opacities, betas, backgrounds, masks, width, height, tile_size,
-                       isect_offsets, flatten_ids, True)
+                       isect_offsets, flatten_ids, True)[:5]
 
 
 def _raster_fwd(means2d, conics, colors, opacities, betas, backgrounds, masks, width, height, tile_size,
                 isect_offsets, flatten_ids, checkpoint: bool):
+    """(render_colors, render_alphas, last_ids, T_final, T_ckpt or None, the
+    workspace: None, or the packed records and, with the tile order on, the
+    order, which the backward of the same inputs can reuse)."""
     require_cuda(means2d, conics, colors, opacities, betas, backgrounds, isect_offsets, flatten_ids)
     require_dtype(_F32, means2d, conics, colors, opacities, betas, backgrounds)
     require_dtype(_I32, isect_offsets, flatten_ids)
@@ -646,19 +649,23 @@ def _raster_fwd(means2d, conics, colors, opacities, betas, backgrounds, masks, w
               ptr(colors), ptr(opacities), ptr(betas), ptr(backgrounds), mp[0] if mp else None, width, height,
               tile_size, tw, th, ptr(isect_offsets), ptr(flatten_ids), ptr(render_colors), ptr(render_alphas),
               ptr(last_ids), ptr(T_final), ptr(ws), ws_bytes, stream_of(means2d))
-        return render_colors, render_alphas, last_ids, T_final, None
+        return render_colors, render_alphas, last_ids, T_final, None, ws
     n_ck = int(lib().ubs_transmittance_checkpoint_floats(C, flatten_ids.numel(), tw, th, tile_size))
     T_ckpt = torch.empty((n_ck,), dtype=torch.float32, device=dev)
     _call("ubs_rasterize_to_pixels_fwd_ckpt", C, N, flatten_ids.numel(), channels, ptr(means2d), ptr(conics),
           ptr(colors), ptr(opacities), ptr(betas), ptr(backgrounds), mp[0] if mp else None, width, height,
           tile_size, tw, th, ptr(isect_offsets), ptr(flatten_ids), ptr(render_colors), ptr(render_alphas),
           ptr(last_ids), ptr(T_final), ptr(T_ckpt), n_ck, ptr(ws), ws_bytes, stream_of(means2d))
-    return render_colors, render_alphas, last_ids, T_final, T_ckpt
+    return render_colors, render_alphas, last_ids, T_final, T_ckpt, ws
 
 
 def raster_bwd_raw(means2d, conics, colors, opacities, betas, backgrounds, masks, width, height, tile_size,
                    isect_offsets, flatten_ids, render_alphas, last_ids, T_final, v_render_colors,
-                   v_render_alphas, T_ckpt=None):
+                   v_render_alphas, T_ckpt=None, fwd_workspace=None):
+    """Blend backward.  fwd_workspace: the workspace the forward of the same
+    inputs returned (records and tile order, sized with the order): its
+    records and order are reused (ubs_rasterize_to_pixels_bwd_reuse) instead
+    of packed and computed again."""
     require_cuda(means2d, conics, colors, opacities, betas, backgrounds, isect_offsets, flatten_ids,
                  render_alphas, last_ids, T_final, v_render_colors, v_render_alphas)
     require_dtype(_F32, means2d, conics, colors, opacities, betas, backgrounds, render_alphas, T_final,
@@ -673,6 +680,19 @@ def raster_bwd_raw(means2d, conics, colors, opacities, betas, backgrounds, masks
     v_opacities = torch.empty_like(opacities)
     v_betas = torch.empty_like(betas)
     mp = _mask_ptr(masks)
+    if fwd_workspace is not None and _packed_records and C * N > 0:
+        if T_ckpt is not None:
+            require_cuda(T_ckpt)
+            require_dtype(_F32, T_ckpt)
+        gb = int(lib().ubs_rasterize_workspace_bytes(C, N, 0))
+        gws = torch.empty((gb,), dtype=torch.uint8, device=means2d.device)
+        _call("ubs_rasterize_to_pixels_bwd_reuse", C, N, flatten_ids.numel(), channels, ptr(means2d), ptr(conics),
+              ptr(colors), ptr(opacities), ptr(betas), ptr(backgrounds), mp[0] if mp else None, width, height,
+              tile_size, tw, th, ptr(isect_offsets), ptr(flatten_ids), ptr(render_alphas), ptr(last_ids),
+              ptr(T_final), ptr(T_ckpt), 0 if T_ckpt is None else T_ckpt.numel(), ptr(v_render_colors),
+              ptr(v_render_alphas), ptr(v_means2d), ptr(v_conics), ptr(v_colors), ptr(v_opacities), ptr(v_betas),
+              ptr(fwd_workspace), fwd_workspace.numel(), ptr(gws), gb, stream_of(means2d))
+        return v_means2d, v_conics, v_colors, v_opacities, v_betas
     ws, ws_bytes = _blend_workspace(C, N, True, means2d.device, tw, th)
     if T_ckpt is None:
         _call("ubs_rasterize_to_pixels_bwd", C, N, flatten_ids.numel(), channels, ptr(means2d), ptr(conics),
@@ -1059,12 +1079,14 @@ class _RasterizeToPixels(torch.autograd.Function):
     def forward(ctx, means2d, conics, colors, opacities, betas, backgrounds, masks, width, height, tile_size,
                 isect_offsets, flatten_ids):
         ckpt = _t_checkpoint and any(ctx.needs_input_grad[:5])
-        render_colors, render_alphas, last_ids, T_final, T_ckpt = _raster_fwd(
+        render_colors, render_alphas, last_ids, T_final, T_ckpt, ws = _raster_fwd(
             means2d, conics, colors, opacities, betas, backgrounds, masks, width, height, tile_size,
             isect_offsets, flatten_ids, ckpt)
         ctx.save_for_backward(means2d, conics, colors, opacities, betas, backgrounds, masks, isect_offsets,
                               flatten_ids, render_alphas, last_ids, T_final, T_ckpt)
         ctx.width, ctx.height, ctx.tile_size = width, height, tile_size
+        # the forward's packed records and tile order, for the backward to reuse
+        ctx.fwd_ws = ws if (ws is not None and _tile_order and any(ctx.needs_input_grad[:5])) else None
         return render_colors, render_alphas
 
     @staticmethod
@@ -1074,7 +1096,8 @@ class _RasterizeToPixels(torch.autograd.Function):
         v_means2d, v_conics, v_colors, v_opacities, v_betas = raster_bwd_raw(
             means2d, conics, colors, opacities, betas, backgrounds, masks, ctx.width, ctx.height,
             ctx.tile_size, isect_offsets, flatten_ids, render_alphas, last_ids, T_final,
-            v_render_colors.contiguous(), v_render_alphas.contiguous(), T_ckpt=T_ckpt)
+            v_render_colors.contiguous(), v_render_alphas.contiguous(), T_ckpt=T_ckpt, fwd_workspace=ctx.fwd_ws)
+        ctx.fwd_ws = None
         if ctx.needs_input_grad[5]:
             v_backgrounds = (v_render_colors * (1.0 - render_alphas).float()).sum(dim=(1, 2))
         else:
