@@ -531,6 +531,44 @@ def test_blend_tile_order_same_results(device, C, channels, ts, mask):
         assert rel_l2(host(b), host(a).astype(np.float64)) < 1e-6, n
 
 
+@pytest.mark.parametrize("C,W_,H_,ts", [(1, 232, 152, 16), (3, 200, 136, 8), (2, 1000, 600, 32)])
+def test_tile_order_is_per_xcd_heaviest_first(device, C, W_, H_, ts):
+    """The order the forward writes into its workspace, read back: for each
+    of the 8 XCDs exactly the tiles its 2x2 blocks hold (the XCD the raster
+    order gives them), heaviest first to the 256-bucket width of the order
+    kernel, then padding (~0)."""
+    x = _raster_inputs(lambda **k: opaque_scene(3000, **k), W_, H_, C=C, ts=ts)
+    ins = [cu(x[k]) for k in ("m2", "cn", "col", "op", "be")]
+    out = W._raster_fwd(*ins, None, None, W_, H_, ts, cu(x["off"]), cu(x["flat"]), False)
+    ws = out[5]
+    tw, th = x["tw"], x["th"]
+    L = W.lib()
+    base = int(L.ubs_rasterize_workspace_bytes(C, x["op"].shape[1], 0))
+    nbx, nby = (tw + 1) // 2, (th + 1) // 2
+    slots = (C * nbx * nby + 7) // 8 * 4
+    order = host(ws[base:base + 4 * 8 * slots].view(torch.int32)).view(np.uint32).reshape(8, slots)
+    tiles = C * tw * th
+    off = x["off"].reshape(-1).astype(np.int64)
+    cnt = np.diff(np.append(off, x["flat"].size))
+    work = cnt + 64
+    t = np.arange(tiles)
+    cam, rem = t // (tw * th), t % (tw * th)
+    xcd = (cam * nbx * nby + (rem // tw) // 2 * nbx + (rem % tw) // 2) % 8
+    width = (work.max() + 1) / 256.0
+    seen = np.zeros(tiles, np.int64)
+    for k in range(8):
+        lst = order[k]
+        real = lst[lst != 0xFFFFFFFF]
+        assert (lst[real.size:] == 0xFFFFFFFF).all(), k  # padding only after the tiles
+        assert (xcd[real] == k).all(), k
+        seen[real] += 1
+        assert set(real.tolist()) == set(np.nonzero(xcd == k)[0].tolist()), k
+        b = np.floor(work[real] * 256 / (work.max() + 1))
+        assert (np.diff(b) <= 0).all(), k  # heaviest bucket first
+    assert (seen == 1).all()
+    assert width > 0
+
+
 @pytest.mark.parametrize("C,channels,ts,ckpt", [(2, 3, 16, False), (1, 3, 16, True), (1, 7, 8, False),
                                                 (1, 40, 16, True)])
 def test_blend_backward_reuses_forward_workspace(device, C, channels, ts, ckpt):
