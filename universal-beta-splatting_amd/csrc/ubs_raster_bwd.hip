@@ -51,7 +51,7 @@ __global__ __launch_bounds__(TS *TS, UBS_BWD_WAVES_PER_EU) void raster_bwd_kerne
     float *__restrict__ v_means2d, float *__restrict__ v_conics, float *__restrict__ v_colors,
     float *__restrict__ v_opacities, float *__restrict__ v_betas, const float4 *__restrict__ rec, int wide_rec,
     float *__restrict__ vrec, const float *__restrict__ T_ckpt, unsigned long long *__restrict__ stats = nullptr,
-    const uint32_t *__restrict__ order = nullptr) {
+    const uint32_t *__restrict__ order = nullptr, unsigned long long *__restrict__ wg_times = nullptr) {
     constexpr int BS = TS * TS;
     constexpr int SB = TS / 8;
     constexpr int NW = BS / 64;
@@ -70,6 +70,9 @@ __global__ __launch_bounds__(TS *TS, UBS_BWD_WAVES_PER_EU) void raster_bwd_kerne
 
     uint32_t logical;
     if (!xcd_tile(blockIdx.x, C, tw, th, logical, order)) return;
+    // (measurement builds: the workgroup's start on the constant-rate clock)
+    unsigned long long t_start = 0;
+    if constexpr (STATS) t_start = __builtin_amdgcn_s_memrealtime();
     const uint32_t cam = logical / (th * tw);
     const uint32_t tile = logical % (th * tw);
     const uint32_t ty = tile / tw, tx = tile % tw;
@@ -417,7 +420,17 @@ __global__ __launch_bounds__(TS *TS, UBS_BWD_WAVES_PER_EU) void raster_bwd_kerne
             // steps, reduce-scatters and their lane-count classes are wave-uniform
             const unsigned long long tot =
                 (q == 0 || q == 3 || q >= 5) ? (unsigned long long)v[q] : wave_sum_u64(v[q]);
-            if (lane == 0 && tot) atomicAdd(stats + q, tot);
+            if (stats && lane == 0 && tot) atomicAdd(stats + q, tot);
+        }
+        if (wg_times) {  // start, end, tile and the hardware id of the workgroup
+            __syncthreads();
+            if (tid == 0) {
+                unsigned long long *o = wg_times + 4ull * blockIdx.x;
+                o[0] = t_start;
+                o[1] = __builtin_amdgcn_s_memrealtime();
+                o[2] = logical;
+                o[3] = (unsigned long long)__builtin_amdgcn_s_getreg((31 << 11) | 4);  // HW_ID
+            }
         }
     }
 }
@@ -680,6 +693,31 @@ __attribute__((visibility("default"))) int ubs_probe_raster_bwd_stats(
         C, N, n_isects, 3, 0, means2d, conics, colors, opacities, betas, nullptr, nullptr, W, H, tw, th,
         tile_offsets, flatten_ids, nullptr, last_ids, T_final, v_render_colors, v_render_alphas, nullptr, nullptr,
         v_colors, nullptr, nullptr, r, 1, vrec, nullptr, stats);
+    return (int)hipGetLastError();
+}
+
+// The same backward through `order` (null: raster order) writing per
+// workgroup its start, end, tile and hardware id into wg_times
+// [xcd_grid * 4] (zeroed); stats may be null (timing only).
+__attribute__((visibility("default"))) int ubs_probe_raster_bwd_times(
+    uint32_t C, uint32_t N, int64_t n_isects, const float *means2d, const float *conics, const float *colors,
+    const float *opacities, const float *betas, uint32_t W, uint32_t H, uint32_t tw, uint32_t th,
+    const int32_t *tile_offsets, const int32_t *flatten_ids, const int32_t *last_ids, const float *T_final,
+    const float *v_render_colors, const float *v_render_alphas, float *v_colors, void *workspace,
+    const uint32_t *order, unsigned long long *stats, unsigned long long *wg_times, void *stream) {
+    hipStream_t st = as_stream(stream);
+    const uint64_t CN = (uint64_t)C * N;
+    float4 *r = static_cast<float4 *>(workspace);
+    float *vrec = reinterpret_cast<float *>(static_cast<char *>(workspace) + raster_workspace_bytes(C, N, 0));
+    int e = zero_async(v_colors, CN * 3 * sizeof(float), st);
+    if (!e) e = zero_async(vrec, CN * kWideRecordFloats * sizeof(float), st);
+    if (e) return e;
+    raster_pack_kernel<<<grid_for((int64_t)CN), kThreads, 0, st>>>(CN, means2d, conics, opacities, betas, r,
+                                                                   colors, 3);
+    raster_bwd_kernel<16, 3, false, true><<<xcd_grid(C, tw, th), 256, 0, st>>>(
+        C, N, n_isects, 3, 0, means2d, conics, colors, opacities, betas, nullptr, nullptr, W, H, tw, th,
+        tile_offsets, flatten_ids, nullptr, last_ids, T_final, v_render_colors, v_render_alphas, nullptr, nullptr,
+        v_colors, nullptr, nullptr, r, 1, vrec, nullptr, stats, order, wg_times);
     return (int)hipGetLastError();
 }
 #endif

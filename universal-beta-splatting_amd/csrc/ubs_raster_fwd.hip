@@ -37,7 +37,7 @@ __global__ __launch_bounds__(TS *TS) void raster_fwd_kernel(
     float *__restrict__ render_colors, float *__restrict__ render_alphas, int32_t *__restrict__ last_ids,
     float *__restrict__ T_final, const float4 *__restrict__ rec, int wide_rec, float *__restrict__ T_ckpt,
     unsigned long long *__restrict__ stats = nullptr, const int64_t *__restrict__ n_dev = nullptr,
-    const uint32_t *__restrict__ order = nullptr) {
+    const uint32_t *__restrict__ order = nullptr, unsigned long long *__restrict__ wg_times = nullptr) {
     // n_dev: the intersection count in device memory (at most n_isects, the
     // capacity of the lists), for callers that size the lists on the device
     if (n_dev) n_isects = min(max(*n_dev, (int64_t)0), n_isects);
@@ -56,6 +56,9 @@ __global__ __launch_bounds__(TS *TS) void raster_fwd_kernel(
 
     uint32_t logical;
     if (!xcd_tile(blockIdx.x, C, tw, th, logical, order)) return;
+    // (measurement builds: the workgroup's start on the constant-rate clock)
+    unsigned long long t_start = 0;
+    if constexpr (STATS) t_start = __builtin_amdgcn_s_memrealtime();
     const uint32_t cam = logical / (th * tw);
     const uint32_t tile = logical % (th * tw);
     const uint32_t ty = tile / tw, tx = tile % tw;
@@ -245,7 +248,17 @@ __global__ __launch_bounds__(TS *TS) void raster_fwd_kernel(
         for (int q = 0; q < 5; ++q) {
             // steps are wave-uniform: one lane's count; the others sum over the wave
             const unsigned long long tot = q == 0 ? (unsigned long long)v[0] : wave_sum_u64(v[q]);
-            if (lane == 0 && tot) atomicAdd(stats + q, tot);
+            if (stats && lane == 0 && tot) atomicAdd(stats + q, tot);
+        }
+        if (wg_times) {  // start, end, tile and the hardware id of the workgroup
+            __syncthreads();
+            if (tid == 0) {
+                unsigned long long *o = wg_times + 4ull * blockIdx.x;
+                o[0] = t_start;
+                o[1] = __builtin_amdgcn_s_memrealtime();
+                o[2] = logical;
+                o[3] = (unsigned long long)__builtin_amdgcn_s_getreg((31 << 11) | 4);  // HW_ID
+            }
         }
     }
     if (inside) {
@@ -463,6 +476,28 @@ extern "C" __attribute__((visibility("default"))) int ubs_probe_raster_fwd_stats
         C, N, n_isects, 3, 0, means2d, conics, colors, opacities, betas, nullptr, nullptr, W, H, tw, th,
         tile_offsets, flatten_ids, render_colors, render_alphas, last_ids, T_final,
         static_cast<const float4 *>(workspace), 1, nullptr, stats);
+    return (int)hipGetLastError();
+}
+
+// The same forward dealing its tiles through `order` (null: raster order;
+// else an order as the library writes it) and writing per workgroup its
+// start and end on the constant-rate clock, its tile and its hardware id
+// into wg_times [xcd_grid * 4] (zeroed); the records in `workspace` are
+// packed here.  stats may be null (no counting atomics: timing only).
+extern "C" __attribute__((visibility("default"))) int ubs_probe_raster_fwd_times(
+    uint32_t C, uint32_t N, int64_t n_isects, const float *means2d, const float *conics, const float *colors,
+    const float *opacities, const float *betas, uint32_t W, uint32_t H, uint32_t tw, uint32_t th,
+    const int32_t *tile_offsets, const int32_t *flatten_ids, float *render_colors, float *render_alphas,
+    int32_t *last_ids, float *T_final, void *workspace, const uint32_t *order, unsigned long long *stats,
+    unsigned long long *wg_times, void *stream) {
+    hipStream_t st = as_stream(stream);
+    const uint64_t CN = (uint64_t)C * N;
+    raster_pack_kernel<<<grid_for((int64_t)CN), kThreads, 0, st>>>(CN, means2d, conics, opacities, betas,
+                                                                   static_cast<float4 *>(workspace), colors, 3);
+    raster_fwd_kernel<16, 3, true><<<xcd_grid(C, tw, th), 256, 0, st>>>(
+        C, N, n_isects, 3, 0, means2d, conics, colors, opacities, betas, nullptr, nullptr, W, H, tw, th,
+        tile_offsets, flatten_ids, render_colors, render_alphas, last_ids, T_final,
+        static_cast<const float4 *>(workspace), 1, nullptr, stats, nullptr, order, wg_times);
     return (int)hipGetLastError();
 }
 #endif
