@@ -70,7 +70,8 @@ def no_gc():
 
 # The blend backward that reuses the forward's packed records and tile order
 # (the autograd path's default) is the same stage as the plain backward.
-ENTRY_ALIASES = {"ubs_rasterize_to_pixels_bwd_reuse": "ubs_rasterize_to_pixels_bwd"}
+ENTRY_ALIASES = {"ubs_rasterize_to_pixels_bwd_reuse": "ubs_rasterize_to_pixels_bwd",
+                 "ubs_isect_tiles_emit_ordered_cap": "ubs_isect_tiles_emit_ordered"}
 
 
 def launch_durations(timer) -> dict:
@@ -429,7 +430,7 @@ def main():
             p.grad = None
         img, alpha, meta = render()
         if args.path == "fused":
-            loss = image_loss(img[0], tgt, args.lambda_dssim)
+            loss = image_loss(img, tgt[None], args.lambda_dssim)  # [1, H, W, 3]: no select in the graph
         else:  # the reference's torch ops
             loss = (img[0] - tgt).abs().mean()
         loss.backward()

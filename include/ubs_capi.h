@@ -444,7 +444,8 @@ int ubs_rasterize_to_pixels_fwd(uint32_t C, uint32_t N, int64_t n_isects, uint32
                                 void *workspace, size_t workspace_bytes, void *stream);
 
 /* Backward.  T_final may be NULL (then 1 - render_alphas is used, as the
- * reference does).  Outputs (zeroed here): v_means2d [C,N,2], v_conics
+ * reference does); v_render_alphas may be NULL (a zero gradient, without
+ * the caller filling one).  Outputs (zeroed here): v_means2d [C,N,2], v_conics
  * [C,N,3], v_colors [C,N,channels], v_opacities [C,N], v_betas [C,N].
  * workspace: NULL or ubs_rasterize_workspace_bytes(C, N, 1) bytes.
  * Replaces rasterize_to_pixels_bwd_tensor, rasterize_to_pixels_bwd.cu:278-479. */

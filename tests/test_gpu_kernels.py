@@ -224,6 +224,32 @@ def test_isect_unsorted_and_edges(device):
             assert_same(host(a), b, f"{n} sort={sort}")
 
 
+@pytest.mark.parametrize("mode", ["depth_first", "onesweep"])
+def test_sort_status_reaches_the_next_host_read(device, mode):
+    """A nonzero status word left by an earlier sort (what a look-back wait
+    that gave up leaves; set from the host here) makes the next host read of
+    the count info raise, then is cleared: the call after it gives the
+    oracle's lists again.  depth_first reads the info from pinned memory,
+    onesweep from the device."""
+    s = opaque_scene(3000, W=200, H=120, n_cams=1, seed=7)
+    r, m2, dp, _, _ = O.proj_fwd(s["means"], s["covars6"], None, None, s["viewmats"], s["Ks"], 200, 120)
+    tw, th = (200 + 15) // 16, (120 + 15) // 16
+    want = O.isect_tiles(m2, r, dp, 16, tw, th)
+    gm2, gr, gdp = cu(m2), cu(r), cu(dp)
+    before = W.isect_sort_mode()
+    W.set_isect_sort_mode(mode)
+    try:
+        W._late_status(gm2.device).fill_(1)
+        with pytest.raises(RuntimeError, match="look-back"):
+            W.isect_tiles(gm2, gr, gdp, 16, tw, th)
+        assert int(W._late_status(gm2.device).item()) == 0
+        got = W.isect_tiles(gm2, gr, gdp, 16, tw, th)
+    finally:
+        W.set_isect_sort_mode(before)
+    for a, b, n in zip(got, want, ("tpg", "ids", "flat")):
+        assert_same(host(a), b, f"{n} {mode}")
+
+
 @pytest.mark.parametrize("n", [0, 1, 63, 4095, 4096, 6143, 6144, 6145, 12289, 100003])
 @pytest.mark.parametrize("end_bit", [7, 33, 46, 64])
 def test_radix_sort_stable(device, n, end_bit):

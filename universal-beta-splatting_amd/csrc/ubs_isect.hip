@@ -154,7 +154,10 @@ __global__ void isect_cum_kernel(uint64_t total, const int32_t *__restrict__ tpg
     int64_t s = 0;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
-        v[k] = (base + k < total) ? tpg[order ? (uint64_t)order[base + k] : base + k] : 0;
+        // (an order entry outside [0, total), only from a depth sort that gave
+        // up, counts no tiles)
+        const uint64_t o = base + k < total ? (order ? (uint64_t)(uint32_t)order[base + k] : base + k) : total;
+        v[k] = o < total ? tpg[o] : 0;
         s += v[k];
     }
     int64_t tot;
@@ -190,7 +193,10 @@ __global__ void isect_ordered_sums_kernel(uint64_t total, const int32_t *__restr
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
         const uint64_t idx = base + (uint64_t)k * kThreads + threadIdx.x;
-        if (idx < total) s += tpg[order[idx]];
+        if (idx < total) {
+            const uint64_t o = (uint64_t)(uint32_t)order[idx];
+            if (o < total) s += tpg[o];
+        }
     }
     int64_t tot;
     block_exclusive_scan(s, lds_wave, tot);
@@ -222,8 +228,10 @@ __global__ __launch_bounds__(256) void isect_emit_kernel(uint32_t C, uint32_t N,
     const uint64_t pos0 = pos - (uint64_t)lane;
     if (pos0 >= CN) return;  // wave-uniform
     const bool valid = pos < CN;
-    const uint64_t idx = !valid ? 0 : order ? (uint64_t)order[pos] : pos;  // the flat (camera, primitive) id
-    const int32_t r = valid ? radii[idx] : 0;
+    uint64_t idx = !valid ? 0 : order ? (uint64_t)(uint32_t)order[pos] : pos;  // the flat (camera, primitive) id
+    const bool in_range = idx < CN;  // (false only for an order from a depth sort that gave up: no tiles)
+    if (!in_range) idx = 0;
+    const int32_t r = valid && in_range ? radii[idx] : 0;
     uint32_t x0 = 0, y0 = 0, x1 = 0, y1 = 0;
     if (r > 0) tile_box(means2d[idx * 2], means2d[idx * 2 + 1], r, ts, tw, th, x0, y0, x1, y1);
     const uint32_t bw = x1 - x0;
@@ -331,11 +339,13 @@ int ubs_isect_tiles_count_range(uint32_t C, uint32_t N, const float *means2d, co
     UBS_REQUIRE(ts > 0, "tile_size must be > 0");
     UBS_REQUIRE(bit_width_u32(tw * th) + bit_width_u32(C) <= 32, "camera and tile ids need more than 32 bits");
     UBS_REQUIRE(!range_dev || depths, "range_dev needs depths");
-    if (range_dev) {
-        int e = zero_async(range_dev, 2 * sizeof(uint32_t), st);
-        if (e) return e;
+    if (total == 0) {  // (otherwise the scan writes both)
+        if (range_dev) {
+            int e = zero_async(range_dev, 2 * sizeof(uint32_t), st);
+            if (e) return e;
+        }
+        return zero_async(n_isects_dev, sizeof(int64_t), st);
     }
-    if (total == 0) return zero_async(n_isects_dev, sizeof(int64_t), st);
     UBS_REQUIRE(total < (1ull << 31), "C*N must be < 2^31");
     UBS_REQUIRE(means2d && radii && tpg && cum, "null pointer");
     const uint64_t nb = (total + kScanTile - 1) / kScanTile;

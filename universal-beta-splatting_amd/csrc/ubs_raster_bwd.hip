@@ -107,7 +107,7 @@ __global__ __launch_bounds__(TS *TS, UBS_BWD_WAVES_PER_EU) void raster_bwd_kerne
         vrc[k] = inside ? v_render_colors[pix * channels + coff + k] : 0.f;
         if (backgrounds) bg_dot = __builtin_fmaf(backgrounds[cam * channels + coff + k], vrc[k], bg_dot);
     }
-    const float vra = (inside && first_chunk) ? v_render_alphas[pix] : 0.f;
+    const float vra = (inside && first_chunk && v_render_alphas) ? v_render_alphas[pix] : 0.f;  // null: zeros
     const float vra_bg = vra - bg_dot;  // d/d(T_final) of the pixel's outputs
     const float tf_vra = Tf * vra_bg;
     float bufv = 0.f;                   // sum over later primitives of c . v_render_color * alpha T
@@ -580,7 +580,7 @@ static int raster_bwd_impl(uint32_t C, uint32_t N, int64_t n_isects, uint32_t ch
     if (n_isects == 0 || (uint64_t)C * W * H == 0) return UBS_OK;
     UBS_REQUIRE((uint64_t)C * th * tw < (1ull << 31), "too many tiles");
     UBS_REQUIRE(means2d && conics && colors && opacities && betas && tile_offsets && flatten_ids && last_ids &&
-                    (render_alphas || T_final) && v_render_colors && v_render_alphas,
+                    (render_alphas || T_final) && v_render_colors,
                 "null pointer");
     if (T_ckpt) {
         const size_t need = ckpt_floats(C, n_isects, tw, th, tile_size);

@@ -188,7 +188,8 @@ def render_views(params: BetaParams, viewmats: Tensor, Ks: Tensor, width: int, h
                                                                                 tw, th)
     if isect_offsets is None:
         isect_offsets = isect_offset_encode(isect_ids, C, tw, th)
-    colors = params.rgb.expand(C, -1, -1)
+    # (one view: a view of the leaf, whose backward is no kernel; several: their gradients sum)
+    colors = params.rgb[None] if C == 1 else params.rgb.expand(C, -1, -1)
     render_colors, render_alphas = rasterize_to_pixels(
         means2d, conics, colors, opacities, betas, width, height, tile_size, isect_offsets, flatten_ids,
         backgrounds=backgrounds)

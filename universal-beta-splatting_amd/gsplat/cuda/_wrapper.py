@@ -322,7 +322,12 @@ def isect_read_info(states) -> list:
     and what was rendered from it, is invalid)."""
     if not states:
         return []
-    infos = (torch.stack([st.info for st in states]) if len(states) > 1 else states[0].info[None]).cpu()
+    if all(st.info_host is not None for st in states):
+        for st in states:
+            st.ready.synchronize()
+        infos = torch.stack([st.info_host for st in states])
+    else:
+        infos = (torch.stack([st.info for st in states]) if len(states) > 1 else states[0].info[None]).cpu()
     if bool((infos[:, 2] != 0).any()):
         for st in states:
             _late_status(st.info.device).zero_()
@@ -334,7 +339,7 @@ def isect_read_info(states) -> list:
 class IsectState:
     """The intersection stage up to the host read of n_isects (see isect_begin)."""
     __slots__ = ("means2d", "radii", "depths", "tile_size", "tile_width", "tile_height", "sort", "tpg", "cum",
-                 "info", "order", "C", "N")
+                 "info", "info_host", "ready", "order", "C", "N")
 
 
 def isect_begin(means2d: Tensor, radii: Tensor, depths: Tensor, tile_size: int, tile_width: int, tile_height: int,
@@ -343,7 +348,11 @@ def isect_begin(means2d: Tensor, radii: Tensor, depths: Tensor, tile_size: int, 
     depth range into ``state.info`` ([n_isects, packed range] int64, on the
     device) and, for the depth-first sort, the depth order of the pairs.
     Nothing waits for the host; several states can be read in one sync
-    before their isect_finish calls."""
+    before their isect_finish calls.  With the depth-first sort the info is
+    copied to pinned host memory before the depth sort is queued, so the
+    host read waits for the counts only and the depth sort runs while the
+    host queues the emission; that sort's status then joins the status
+    word the next host read reports."""
     C, N, _ = means2d.shape
     assert means2d.shape == (C, N, 2), means2d.size()
     assert radii.shape == (C, N), radii.size()
@@ -362,20 +371,29 @@ def isect_begin(means2d: Tensor, radii: Tensor, depths: Tensor, tile_size: int, 
     S.cum = torch.empty((C * N,), dtype=torch.int64, device=dev)
     # [n_isects, (~min depth bits, max depth bits) as two uint32, sort status],
     # read in one sync
-    S.info = torch.zeros((3,), dtype=torch.int64, device=dev)
-    S.info[2:3].add_(_late_status(dev))
+    if sort and C * N > 0:  # the count kernels write info[0] and info[1]
+        S.info = torch.empty((3,), dtype=torch.int64, device=dev)
+        S.info[2:3].copy_(_late_status(dev))
+    else:
+        S.info = torch.zeros((3,), dtype=torch.int64, device=dev)
+        S.info[2:3].add_(_late_status(dev))
     ws_bytes = int(L.ubs_isect_count_workspace_bytes(C, N))
     ws = torch.empty((ws_bytes,), dtype=torch.uint8, device=dev)
     _call("ubs_isect_tiles_count_range", C, N, ptr(means2d), ptr(radii), ptr(depths), tile_size, tile_width,
           tile_height, ptr(S.tpg), ptr(S.cum), ptr(S.info), _vp_at(S.info, 1) if sort else None, ptr(ws), ws_bytes,
           st)
-    S.order = None
+    S.order = S.info_host = S.ready = None
     if sort and _sort_mode == "depth_first" and C * N > 0:
-        # queued before the host read of n_isects, so it overlaps the round trip
+        # the info leaves for the host first; the depth sort and its scan
+        # run during the round trip
+        S.info_host = torch.empty((3,), dtype=torch.int64, pin_memory=True)
+        S.info_host.copy_(S.info, non_blocking=True)
+        S.ready = torch.cuda.Event()
+        S.ready.record(torch.cuda.current_stream(dev))
         dkeys = torch.empty((C * N,), dtype=torch.int64, device=dev)
         dvals = torch.empty((C * N,), dtype=torch.int32, device=dev)
         _call("ubs_isect_depth_keys", C, N, ptr(radii), ptr(depths), ptr(dkeys), ptr(dvals), st)
-        _, S.order = radix_sort_pairs(dkeys, dvals, 0, 32, status_out=S.info[2:3])
+        _, S.order = radix_sort_pairs(dkeys, dvals, 0, 32, status_out=_late_status(dev))
         S.cum = torch.empty((C * N,), dtype=torch.int64, device=dev)
         ows = int(L.ubs_isect_ordered_workspace_bytes(C, N))
         ow = torch.empty((ows,), dtype=torch.uint8, device=dev)
@@ -392,8 +410,10 @@ def isect_finish(S: IsectState, n_isects: int, packed: int):
     isect_ids = torch.empty((n_isects,), dtype=torch.int64, device=dev)
     flatten_ids = torch.empty((n_isects,), dtype=torch.int32, device=dev)
     if n_isects and S.order is not None:
-        _call("ubs_isect_tiles_emit_ordered", C, S.N, ptr(S.means2d), ptr(S.radii), ptr(S.depths), ptr(S.order),
-              ptr(S.cum), ts, tw, th, ptr(isect_ids), ptr(flatten_ids), st)
+        # capped at n_isects: the depth order may come from a sort whose
+        # status the host has not read yet
+        _call("ubs_isect_tiles_emit_ordered_cap", C, S.N, ptr(S.means2d), ptr(S.radii), ptr(S.depths),
+              ptr(S.order), ptr(S.cum), ts, tw, th, ptr(isect_ids), ptr(flatten_ids), n_isects, st)
     elif n_isects:
         _call("ubs_isect_tiles_emit", C, S.N, ptr(S.means2d), ptr(S.radii), ptr(S.depths), ptr(S.cum), ts, tw, th,
               ptr(isect_ids), ptr(flatten_ids), st)
@@ -671,7 +691,7 @@ def raster_bwd_raw(means2d, conics, colors, opacities, betas, backgrounds, masks
     require_dtype(_F32, means2d, conics, colors, opacities, betas, backgrounds, render_alphas, T_final,
                   v_render_colors, v_render_alphas)
     require_dtype(_I32, isect_offsets, flatten_ids, last_ids)
-    C, N = opacities.shape
+    C, N = opacities.shape  # (v_render_alphas None: a zero gradient)
     channels = colors.shape[-1]
     th, tw = isect_offsets.shape[1:3]
     v_means2d = torch.empty_like(means2d)
@@ -961,6 +981,7 @@ class _BetaProject(torch.autograd.Function):
                               cam_pos, timestamps, image_wh, radii, conics)
         ctx.width, ctx.height, ctx.eps2d, ctx.ortho = width, height, eps2d, ortho
         ctx.mark_non_differentiable(radii)
+        ctx.set_materialize_grads(False)  # the radii and depths gradients come back as None
         return radii, means2d, depths, conics, opacities, betas
 
     @staticmethod
@@ -1079,6 +1100,8 @@ class _RasterizeToPixels(torch.autograd.Function):
     def forward(ctx, means2d, conics, colors, opacities, betas, backgrounds, masks, width, height, tile_size,
                 isect_offsets, flatten_ids):
         ckpt = _t_checkpoint and any(ctx.needs_input_grad[:5])
+        # an output without a gradient comes back as None, not a filled zero tensor
+        ctx.set_materialize_grads(False)
         render_colors, render_alphas, last_ids, T_final, T_ckpt, ws = _raster_fwd(
             means2d, conics, colors, opacities, betas, backgrounds, masks, width, height, tile_size,
             isect_offsets, flatten_ids, ckpt)
@@ -1093,10 +1116,14 @@ class _RasterizeToPixels(torch.autograd.Function):
     def backward(ctx, v_render_colors, v_render_alphas):
         (means2d, conics, colors, opacities, betas, backgrounds, masks, isect_offsets, flatten_ids,
          render_alphas, last_ids, T_final, T_ckpt) = ctx.saved_tensors
+        if v_render_colors is None:
+            v_render_colors = torch.zeros((*render_alphas.shape[:3], colors.shape[-1]), dtype=torch.float32,
+                                          device=render_alphas.device)
         v_means2d, v_conics, v_colors, v_opacities, v_betas = raster_bwd_raw(
             means2d, conics, colors, opacities, betas, backgrounds, masks, ctx.width, ctx.height,
             ctx.tile_size, isect_offsets, flatten_ids, render_alphas, last_ids, T_final,
-            v_render_colors.contiguous(), v_render_alphas.contiguous(), T_ckpt=T_ckpt, fwd_workspace=ctx.fwd_ws)
+            v_render_colors.contiguous(), None if v_render_alphas is None else v_render_alphas.contiguous(),
+            T_ckpt=T_ckpt, fwd_workspace=ctx.fwd_ws)
         ctx.fwd_ws = None
         if ctx.needs_input_grad[5]:
             v_backgrounds = (v_render_colors * (1.0 - render_alphas).float()).sum(dim=(1, 2))
