@@ -128,12 +128,19 @@ __global__ __launch_bounds__(TS *TS, UBS_BWD_WAVES_PER_EU) void raster_bwd_kerne
     constexpr int NREG = RowTotals<NG2>::kRegs;
     constexpr int NPASS = (NREG + 3) / 4;
     const int my_j = lane & 3, my_bank = (lane >> 2) & 3;
-    int my_idx[NPASS];
+    // per pass: the lane's accumulator row (byte offset of s_acc[idx % NV][0],
+    // -1 for a lane without a total) and whether its total is the pair's
+    // second primitive's (then the entry offset t1 - t0 is added)
+    int my_row[NPASS];
+    uint32_t my_second[NPASS];
 #pragma unroll
     for (int p = 0; p < NPASS; ++p) {
         const int k = 4 * p + my_j;
         const int grp = k < NREG ? row_group<NG2>(k, my_bank) : -1;
-        my_idx[p] = grp >= 0 ? wave_total_index<2 * NV>(grp, lane >> 4) : -1;
+        const int idx = grp >= 0 ? wave_total_index<2 * NV>(grp, lane >> 4) : -1;
+        my_row[p] = idx >= 0 ? (idx % NV) * (BS + 1) * (int)sizeof(float) : -1;
+        my_second[p] = idx >= NV ? ~0u : 0u;
+        asm volatile("" : "+v"(my_second[p]));  // kept as a mask: one AND selects the offset
     }
 
     uint32_t st_steps = 0, st_live = 0, st_valid = 0, st_rs = 0, st_staged = 0, st_k4 = 0, st_k16 = 0;
@@ -285,10 +292,16 @@ __global__ __launch_bounds__(TS *TS, UBS_BWD_WAVES_PER_EU) void raster_bwd_kerne
         float pend_val[NPASS];
         auto add_pending = [&]() {
             if (!pend) return;
+            // byte offsets: the row's, the pair's first entry's, and for the
+            // second primitive's totals the distance to its entry
+            const uint32_t t0b = (uint32_t)pend_t0 * (uint32_t)sizeof(float);
+            const uint32_t db = (uint32_t)(pend_t1 - pend_t0) * (uint32_t)sizeof(float);
 #pragma unroll
             for (int p = 0; p < NPASS; ++p) {
-                const int idx = my_idx[p];
-                if (idx >= 0) atomicAdd(&s_acc[idx % NV][idx < NV ? pend_t0 : pend_t1], pend_val[p]);
+                if (my_row[p] >= 0) {
+                    const uint32_t off = (uint32_t)my_row[p] + t0b + (my_second[p] & db);
+                    atomicAdd(reinterpret_cast<float *>(reinterpret_cast<char *>(&s_acc[0][0]) + off), pend_val[p]);
+                }
             }
             pend = false;
         };
@@ -310,8 +323,8 @@ __global__ __launch_bounds__(TS *TS, UBS_BWD_WAVES_PER_EU) void raster_bwd_kerne
                 // (exact zeros), which keeps the loop body free of branches
                 const bool c0v = pixel_grads(r0, t0, v, 0, true);
                 const bool c1v = pixel_grads(r1, t1, v, NV, two);
-                const bool any = __ballot(c0v || c1v) != 0ull;
                 if constexpr (STATS) {
+                    const bool any = __ballot(c0v || c1v) != 0ull;
                     st_steps += two ? 2u : 1u;
                     st_live += (uint32_t)(t0 >= t_min) + (uint32_t)(two && t1 >= t_min);
                     st_valid += (uint32_t)c0v + (uint32_t)c1v;
@@ -321,7 +334,8 @@ __global__ __launch_bounds__(TS *TS, UBS_BWD_WAVES_PER_EU) void raster_bwd_kerne
                     st_k4 += (uint32_t)(any && k <= 4);
                     st_k16 += (uint32_t)(any && k <= 16);
                 }
-                if (!any) continue;
+                // (nearly every pair has a contribution in the wave; one without
+                // adds exact zeros, cheaper than testing every pair for it)
                 // wave totals, scattered over the rows and banks; one ds_add_f32
                 // per pass adds them into the primitives' accumulators
                 float tot[NREG];

@@ -52,7 +52,9 @@ __global__ __launch_bounds__(TS *TS) void raster_fwd_kernel(
     __shared__ float4 s_colv[kColVec ? BS : 1];
     __shared__ float s_col[kColVec ? 1 : CDIM][kColVec ? 1 : BS];
     __shared__ uint32_t s_mask[BS];
-    __shared__ __align__(16) uint16_t s_list[NW][BS];  // byte offsets (16 x entry) of the wave's primitives
+    // byte offsets (16 x entry) of the wave's primitives; 32-bit so that a
+    // pair of them is one 64-bit read with nothing to unpack
+    __shared__ __align__(16) uint32_t s_list[NW][BS];
 
     uint32_t logical;
     if (!xcd_tile(blockIdx.x, C, tw, th, logical, order)) return;
@@ -158,7 +160,7 @@ __global__ __launch_bounds__(TS *TS) void raster_fwd_kernel(
             const uint64_t bal = __ballot(hit);
             if (hit) {
                 const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-                s_list[wave][n_mine + __popcll(bal & lt)] = (uint16_t)(e * 16);
+                s_list[wave][n_mine + __popcll(bal & lt)] = (uint32_t)(e * 16);
             }
             n_mine += __popcll(bal);
         }
@@ -214,8 +216,8 @@ __global__ __launch_bounds__(TS *TS) void raster_fwd_kernel(
             // is blended: half the list reads, one LDS round trip per two steps
             int k = k0;
             for (; k + 1 < kend; k += 2) {
-                const uint32_t both = *reinterpret_cast<const uint32_t *>(&s_list[wave][k]);  // k is even
-                const int off0 = (int)(both & 0xffffu), off1 = (int)(both >> 16);
+                const uint2 both = *reinterpret_cast<const uint2 *>(&s_list[wave][k]);  // k is even
+                const int off0 = (int)both.x, off1 = (int)both.y;
                 float4 a1, a2, ac, b1, b2, bc;
                 rec_at(off0, a1, a2, ac);
                 rec_at(off1, b1, b2, bc);
