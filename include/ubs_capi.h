@@ -182,7 +182,8 @@ int ubs_fully_fused_projection_bwd(uint32_t C, uint32_t N, const float *means, c
  * [N,D-3], scale_raw [N,D], l_triangle [N,D(D-1)/2] (canonical rest pattern of
  * beta_model.py:69-73), opacity_raw [N,1], beta_raw [N,D-2]; quats [N,4] or
  * NULL (then the rotation is I + skew(l_triangle[:, :3])).
- * Per view: cam_pos [C,3] (camera centre), timestamps [C] (7-D; NULL -> 0),
+ * Per view: cam_pos [C,3] (camera centre; NULL -> taken from viewmats as
+ * ubs_camera_centers does), timestamps [C] (7-D; NULL -> 0),
  * viewmats [C,4,4], Ks [C,3,3]; image size width x height, or per view
  * image_wh int32 [C,2] when not NULL.
  * Outputs [C,N]: radii int32 (0 = culled; the other 2-D outputs of a culled
@@ -196,8 +197,14 @@ int ubs_beta_project_fwd(int64_t N, int64_t D, uint32_t C, const float *xyz, con
                          int ortho, int32_t *radii, float *means2d, float *depths, float *conics, float *opacities,
                          float *betas, void *stream);
 
+/* Camera centres [C,3] of world->camera viewmats [C,4,4] (row-major):
+ * -(R^T t), with the FMA order the fused projection uses when its cam_pos is
+ * NULL.  The reference takes the centre from its Camera object
+ * (scene/beta_model.py:676, viewpoint_camera.camera_center). */
+int ubs_camera_centers(uint32_t C, const float *viewmats, float *centres, void *stream);
+
 /* Backward: the saved radii and conics and the gradients [C,N] of means2d,
- * depths, conics, opacities and betas -> the leaf gradients (each written
+ * depths (NULL -> zeros), conics, opacities and betas -> the leaf gradients (each written
  * once, summed over the views): v_xyz [N,3], v_mean_rest [N,D-3],
  * v_scale_raw [N,D], v_l_triangle [N,D(D-1)/2], v_opacity_raw [N,1],
  * v_beta_raw [N,D-2], v_quats [N,4] (required when quats is given).

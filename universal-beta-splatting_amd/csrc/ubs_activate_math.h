@@ -46,6 +46,16 @@ __device__ __forceinline__ void quat_rot_vjp(const float (&q)[4], const float (&
 }
 
 // Unit query direction (xyz - cam) / |xyz - cam| (scene/beta_model.py:675-690).
+// Camera centre -(R^T t) of a row-major world->camera [4,4] matrix: one FMA
+// chain per coordinate, the same wherever the centre is taken (in the fused
+// projection or by ubs_camera_centers), so every path queries from the same
+// bits.
+__host__ __device__ inline void camera_centre(const float *vm, float (&cc)[3]) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        cc[k] = -__builtin_fmaf(vm[8 + k], vm[11], __builtin_fmaf(vm[4 + k], vm[7], vm[k] * vm[3]));
+}
+
 __device__ __forceinline__ void view_dir(float p0, float p1, float p2, const float *__restrict__ cam,
                                          float (&q)[3]) {
     const float d0 = p0 - cam[0], d1 = p1 - cam[1], d2 = p2 - cam[2];

@@ -81,14 +81,15 @@ __device__ __forceinline__ void beta_prim_setup(int64_t n, const float *__restri
     covar_fwd_body<D>(bp.L, bp.P, false, bp.Sig);
 }
 
-// Query of view c: unit direction from the camera centre (+ the view's time
-// for 7-D).
+// Query of view c: unit direction from the camera centre (cam_pos, or taken
+// from the view matrix when cam_pos is null) (+ the view's time for 7-D).
 template <int D>
 __device__ __forceinline__ void beta_query(const BetaPrim<D> &bp, const float *__restrict__ cam_pos,
-                                           const float *__restrict__ timestamps, uint32_t c,
-                                           float (&q)[D - 3]) {
-    float qd[3];
-    view_dir(bp.mean[0], bp.mean[1], bp.mean[2], cam_pos + (size_t)c * 3, qd);
+                                           const float *__restrict__ timestamps,
+                                           const float *__restrict__ viewmats, uint32_t c, float (&q)[D - 3]) {
+    float qd[3], cc[3];
+    if (!cam_pos) camera_centre(viewmats + (size_t)c * 16, cc);
+    view_dir(bp.mean[0], bp.mean[1], bp.mean[2], cam_pos ? cam_pos + (size_t)c * 3 : cc, qd);
     q[0] = qd[0];
     q[1] = qd[1];
     q[2] = qd[2];
@@ -109,7 +110,7 @@ __device__ __forceinline__ void beta_view_fwd(const BetaPrim<D> &bp, uint32_t c,
                                               float *__restrict__ opacities, float *__restrict__ betas) {
     constexpr int CQ = D - 3;
     float q[CQ];
-    beta_query<D>(bp, cam_pos, timestamps, c, q);
+    beta_query<D>(bp, cam_pos, timestamps, viewmats, c, q);
     CondRegs<CQ> st;
     cond_setup<CQ>(bp.mean, bp.Sig, bp.beta + 1, q, st);
     float m3[3], V9[9], oc;
@@ -261,7 +262,7 @@ __global__ __launch_bounds__(kThreads, D == 7 ? UBS_PROJ_BWD_WAVES_7 : UBS_PROJ_
         // a culled view has no 2-D record, so every gradient it could pass is 0
         if (radii[idx] <= 0) continue;
         float q[CQ];
-        beta_query<D>(bp, cam_pos, timestamps, c, q);
+        beta_query<D>(bp, cam_pos, timestamps, viewmats, c, q);
         CondRegs<CQ> st;
         cond_setup<CQ>(bp.mean, bp.Sig, bp.beta + 1, q, st);
         float m3[3], V9[9], oc;
@@ -269,7 +270,8 @@ __global__ __launch_bounds__(kThreads, D == 7 ? UBS_PROJ_BWD_WAVES_7 : UBS_PROJ_
         const int32_t Wc = image_wh ? image_wh[2 * c] : W, Hc = image_wh ? image_wh[2 * c + 1] : H;
         float vmean[3], vS[3][3], vR[3][3], vt[3];
         proj_bwd_body(m3, V9, 1, nullptr, nullptr, viewmats + (size_t)c * 16, Ks + (size_t)c * 9, 0u, Wc, Hc, eps2d,
-                      ortho != 0, conics + idx * 3, nullptr, v_means2d + idx * 2, v_depths[idx], v_conics + idx * 3,
+                      ortho != 0, conics + idx * 3, nullptr, v_means2d + idx * 2, v_depths ? v_depths[idx] : 0.f,
+                      v_conics + idx * 3,
                       nullptr, vmean, vS, vR, vt);
         // the projection reads the upper triangle of the conditioned
         // covariance, so its gradient is the folded upper triangle
@@ -358,6 +360,22 @@ __global__ __launch_bounds__(kThreads, D == 7 ? UBS_PROJ_BWD_WAVES_7 : UBS_PROJ_
 
 }  // namespace ubs
 
+namespace ubs {
+
+// Camera centres [C,3] of the view matrices, as the fused projection takes
+// them when it is given none.
+__global__ void camera_centers_kernel(uint32_t C, const float *__restrict__ viewmats, float *__restrict__ out) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    float cc[3];
+    camera_centre(viewmats + (size_t)c * 16, cc);
+    out[3 * c] = cc[0];
+    out[3 * c + 1] = cc[1];
+    out[3 * c + 2] = cc[2];
+}
+
+}  // namespace ubs
+
 using namespace ubs;
 
 extern "C" {
@@ -410,8 +428,8 @@ static int beta_project_fwd_impl(int64_t N, int64_t D, uint32_t C, const float *
     UBS_REQUIRE(D == 6 || D == 7, "the fused Beta projection is defined for D = 6 or 7 (got %lld)", (long long)D);
     UBS_REQUIRE(N >= 0 && (uint64_t)row_stride * C < (1ull << 31), "C*N must be < 2^31");
     if (N == 0 || C == 0) return UBS_OK;
-    UBS_REQUIRE(xyz && mean_rest && scale_raw && l_triangle && opacity_raw && beta_raw && cam_pos && viewmats &&
-                    Ks && radii && means2d && depths && conics && opacities && betas,
+    UBS_REQUIRE(xyz && mean_rest && scale_raw && l_triangle && opacity_raw && beta_raw && viewmats && Ks && radii &&
+                    means2d && depths && conics && opacities && betas,
                 "null pointer");
     hipStream_t st = as_stream(stream);
     const unsigned grid = grid_for(N);
@@ -432,6 +450,14 @@ static int beta_project_fwd_impl(int64_t N, int64_t D, uint32_t C, const float *
     return UBS_OK;
 }
 
+int ubs_camera_centers(uint32_t C, const float *viewmats, float *centres, void *stream) {
+    if (C == 0) return UBS_OK;
+    UBS_REQUIRE(viewmats && centres, "null pointer");
+    camera_centers_kernel<<<(C + 63) / 64, 64, 0, as_stream(stream)>>>(C, viewmats, centres);
+    UBS_LAUNCH_CHECK("camera_centers_kernel");
+    return UBS_OK;
+}
+
 int ubs_beta_project_fwd_table(uint32_t C, int64_t row_stride, const ubs_view_scene *table, const float *cam_pos,
                                const float *timestamps, const float *viewmats, const float *Ks,
                                const int32_t *image_wh, float eps2d, float near_plane, float far_plane,
@@ -439,8 +465,8 @@ int ubs_beta_project_fwd_table(uint32_t C, int64_t row_stride, const ubs_view_sc
                                float *opacities, float *betas, float *colors, void *stream) {
     UBS_REQUIRE(row_stride >= 0 && (uint64_t)row_stride * C < (1ull << 31), "C*row_stride must be < 2^31");
     if (row_stride == 0 || C == 0) return UBS_OK;
-    UBS_REQUIRE(table && cam_pos && viewmats && Ks && image_wh && radii && means2d && depths && conics && opacities &&
-                    betas && colors,
+    UBS_REQUIRE(table && viewmats && Ks && image_wh && radii && means2d && depths && conics && opacities && betas &&
+                    colors,
                 "null pointer");
     hipStream_t st = as_stream(stream);
     const unsigned grid = grid_for(row_stride);
@@ -471,8 +497,7 @@ int ubs_beta_project_bwd(int64_t N, int64_t D, uint32_t C, const float *xyz, con
     UBS_REQUIRE(xyz && mean_rest && scale_raw && l_triangle && opacity_raw && beta_raw && v_xyz && v_mean_rest &&
                     v_scale_raw && v_l_triangle && v_opacity_raw && v_beta_raw,
                 "null pointer");
-    UBS_REQUIRE(C == 0 || (cam_pos && viewmats && Ks && radii && conics && v_means2d && v_depths && v_conics &&
-                           v_opacities && v_betas),
+    UBS_REQUIRE(C == 0 || (viewmats && Ks && radii && conics && v_means2d && v_conics && v_opacities && v_betas),
                 "null pointer");
     hipStream_t st = as_stream(stream);
     const unsigned grid = grid_for(N);

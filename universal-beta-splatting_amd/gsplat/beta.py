@@ -113,9 +113,9 @@ def rest_indices(D: int, device) -> tuple:
 
 
 def camera_center(viewmat: Tensor) -> Tensor:
-    """World position of a world->camera [4,4] row-major matrix."""
-    R, t = viewmat[:3, :3], viewmat[:3, 3]
-    return -(R.transpose(0, 1) @ t)
+    """World position of a world->camera [4,4] row-major matrix (the bits
+    the fused projection takes it with)."""
+    return camera_centers(viewmat[None])[0]
 
 
 def conditioned(params: BetaParams, viewmat: Tensor, timestamp: Optional[float] = None, rest=None):
@@ -174,9 +174,7 @@ def render_views(params: BetaParams, viewmats: Tensor, Ks: Tensor, width: int, h
         raise NotImplementedError("Only implemented for 6D or 7D query")
     C = viewmats.shape[0]
     assert viewmats.shape == (C, 4, 4) and Ks.shape == (C, 3, 3), (viewmats.shape, Ks.shape)
-    with torch.no_grad():
-        if cam_pos is None:
-            cam_pos = camera_centers(viewmats.detach().to(torch.float32))
+    if cam_pos is not None:  # (None: the projection takes each view's centre itself)
         cam_pos = cam_pos.detach().to(torch.float32).contiguous()
     radii, means2d, depths, conics, opacities, betas = beta_project(
         params.xyz, params.mean_rest, params.scale_raw, params.l_triangle, params.opacity_raw, params.beta_raw,
@@ -219,16 +217,16 @@ def render_batch(params: BetaParams, viewmats: Tensor, Ks: Tensor, sizes, timest
     sizes = [(int(w), int(h)) for w, h in sizes]
     assert len(sizes) == C, (len(sizes), C)
     dev = viewmats.device
-    with torch.no_grad():
-        if cam_pos is None:
-            cam_pos = camera_centers(viewmats.detach().to(torch.float32))
+    if cam_pos is not None:  # (None: the projection takes each view's centre itself)
         cam_pos = cam_pos.detach().to(torch.float32).contiguous()
     # project the views ordered by size group, so that each group's
     # projected rows are one contiguous slice [c0:c1] of the [C,N,*] outputs
     order = sorted(range(C), key=lambda c: (sizes[c], c))
     if order != list(range(C)):
         perm = torch.tensor(order, dtype=torch.long, device=dev)
-        viewmats, Ks, cam_pos = viewmats.index_select(0, perm), Ks.index_select(0, perm), cam_pos.index_select(0, perm)
+        viewmats, Ks = viewmats.index_select(0, perm), Ks.index_select(0, perm)
+        if cam_pos is not None:
+            cam_pos = cam_pos.index_select(0, perm)
         if timestamps is not None:
             timestamps = timestamps.index_select(0, perm)
         if backgrounds is not None:

@@ -54,6 +54,7 @@ _SIGNATURES = {
                                                        _c_int] + [_vp] * 7),
     "ubs_beta_project_bwd": (
         _c_int, [_c_i64, _c_i64, _c_u32] + [_vp] * 11 + [_c_i32, _c_i32, _vp, _c_f32, _c_int] + [_vp] * 15),
+    "ubs_camera_centers": (_c_int, [_c_u32, _vp, _vp, _vp]),
     "ubs_isect_count_workspace_bytes": (_c_size, [_c_u32, _c_u32]),
     "ubs_isect_tiles_count": (
         _c_int, [_c_u32, _c_u32, _vp, _vp, _c_u32, _c_u32, _c_u32, _vp, _vp, _vp, _vp, _c_size, _vp]),

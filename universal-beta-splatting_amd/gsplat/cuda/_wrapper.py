@@ -122,10 +122,18 @@ def rot_scale_l_triangle_to_covar(
         rot.contiguous(), scale.contiguous(), l_triangle.contiguous(), rest_i, rest_j, spatial_block)
 
 
+@torch.no_grad()
 def camera_centers(viewmats: Tensor) -> Tensor:
     """World positions [C,3] of world->camera [C,4,4] matrices, -(R^T t) per
-    view (the camera centre BetaModel's query is taken from)."""
-    return torch.stack([-(vm[:3, :3].transpose(0, 1) @ vm[:3, 3]) for vm in viewmats])
+    view (the camera centre BetaModel's query is taken from), one launch; the
+    same bits as the fused projection's own centre when it is given none."""
+    viewmats = viewmats.detach().to(torch.float32).contiguous()
+    require_cuda(viewmats)
+    C = viewmats.size(0)
+    assert viewmats.shape == (C, 4, 4), viewmats.shape
+    out = torch.empty((C, 3), dtype=torch.float32, device=viewmats.device)
+    _call("ubs_camera_centers", C, ptr(viewmats), ptr(out), stream_of(viewmats))
+    return out
 
 
 def beta_project(
@@ -140,7 +148,7 @@ def beta_project(
     Ks: Tensor,  # [C, 3, 3]
     width: int,
     height: int,
-    cam_pos: Optional[Tensor] = None,  # [C, 3]; camera_centers(viewmats) when None
+    cam_pos: Optional[Tensor] = None,  # [C, 3]; camera_centers(viewmats), taken in the kernel, when None
     timestamps: Optional[Tensor] = None,  # [C] (7-D); 0 when None
     image_sizes: Optional[Tensor] = None,  # int32 [C, 2] (W, H) per view, else width x height
     eps2d: float = 0.3,
@@ -164,13 +172,10 @@ def beta_project(
     D = scale_raw.shape[1]
     if D not in (6, 7):
         raise NotImplementedError("Only implemented for 6D or 7D query")
-    if cam_pos is None:
-        with torch.no_grad():
-            cam_pos = camera_centers(viewmats.detach().float())
     return _BetaProject.apply(
         xyz.contiguous(), mean_rest.contiguous(), scale_raw.contiguous(), l_triangle.contiguous(),
         opacity_raw.contiguous(), beta_raw.contiguous(), None if quats is None else quats.contiguous(),
-        viewmats.contiguous(), Ks.contiguous(), cam_pos.detach().contiguous(),
+        viewmats.contiguous(), Ks.contiguous(), None if cam_pos is None else cam_pos.detach().contiguous(),
         None if timestamps is None else timestamps.detach().float().contiguous(),
         None if image_sizes is None else image_sizes.to(torch.int32).contiguous(), int(width), int(height),
         float(eps2d), float(near_plane), float(far_plane), float(radius_clip), bool(ortho))
@@ -922,7 +927,7 @@ def beta_project_fwd_raw(xyz, mean_rest, scale_raw, l_triangle, opacity_raw, bet
     assert opacity_raw.shape == (N, 1) and beta_raw.shape == (N, D - 2), (opacity_raw.shape, beta_raw.shape)
     assert quats is None or quats.shape == (N, 4), quats.shape
     assert viewmats.shape == (C, 4, 4) and Ks.shape == (C, 3, 3), (viewmats.shape, Ks.shape)
-    assert cam_pos.shape == (C, 3), cam_pos.shape
+    assert cam_pos is None or cam_pos.shape == (C, 3), cam_pos.shape
     assert timestamps is None or timestamps.shape == (C,), timestamps.shape
     assert image_wh is None or image_wh.shape == (C, 2), image_wh.shape
     dev, dt = xyz.device, xyz.dtype
@@ -944,7 +949,7 @@ def beta_project_bwd_raw(xyz, mean_rest, scale_raw, l_triangle, opacity_raw, bet
                          v_conics, v_opacities, v_betas):
     """ubs_beta_project_bwd: (v_xyz, v_mean_rest, v_scale_raw, v_l_triangle, v_opacity_raw, v_beta_raw,
     v_quats or None)."""
-    grads = [g.contiguous() for g in (v_means2d, v_depths, v_conics, v_opacities, v_betas)]
+    grads = [None if g is None else g.contiguous() for g in (v_means2d, v_depths, v_conics, v_opacities, v_betas)]
     require_cuda(xyz, mean_rest, scale_raw, l_triangle, opacity_raw, beta_raw, quats, cam_pos, timestamps,
                  viewmats, Ks, image_wh, radii, conics, *grads)
     require_dtype(_F32, xyz, mean_rest, scale_raw, l_triangle, opacity_raw, beta_raw, quats, cam_pos, timestamps,
@@ -996,7 +1001,7 @@ class _BetaProject(torch.autograd.Function):
         grads = beta_project_bwd_raw(
             xyz, mean_rest, scale_raw, l_triangle, opacity_raw, beta_raw, quats, cam_pos, timestamps, viewmats, Ks,
             ctx.width, ctx.height, image_wh, ctx.eps2d, ctx.ortho, radii, conics, g(v_means2d, (C, N, 2)),
-            g(v_depths, (C, N)), g(v_conics, (C, N, 3)), g(v_opacities, (C, N)), g(v_betas, (C, N)))
+            v_depths, g(v_conics, (C, N, 3)), g(v_opacities, (C, N)), g(v_betas, (C, N)))  # (None: zeros)
         out = [gr if need else None for gr, need in zip(grads, ctx.needs_input_grad[:7])]
         return (*out, None, None, None, None, None, None, None, None, None, None, None, None)
 
