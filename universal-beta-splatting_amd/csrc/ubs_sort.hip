@@ -176,25 +176,14 @@ __global__ void sort_hist_kernel(int64_t n_cap, const int64_t *__restrict__ n_de
     }
 }
 
-// Exclusive scan of each pass's 256 digit counts (one workgroup per pass).
-__global__ void sort_hist_scan_kernel(uint32_t *__restrict__ hist) {
-    __shared__ uint32_t s[kRadix];
-    uint32_t *h = hist + (size_t)blockIdx.x * kRadix;
-    const int d = threadIdx.x;
-    s[d] = h[d];
-    __syncthreads();
-    uint32_t acc = 0;
-    for (int k = 0; k < d; ++k) acc += s[k];
-    h[d] = acc;
-}
-
 template <int ITEMS>
 __global__ __launch_bounds__(kSortThreads) void sort_pass_kernel(
     int64_t n_cap, const int64_t *__restrict__ n_dev, const int64_t *__restrict__ keys_in, int64_t *__restrict__ keys_out,
     const int32_t *__restrict__ vals_in, int32_t *__restrict__ vals_out, int shift, int bits,
-    const uint32_t *__restrict__ digit_base, uint32_t *__restrict__ tile_counter, uint32_t *lookback,
+    const uint32_t *__restrict__ digit_count, uint32_t *__restrict__ tile_counter, uint32_t *lookback,
     uint32_t *status) {
     __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_wave_tot[kSortWaves];
     __shared__ uint32_t s_wave_cnt[kSortWaves][kRadix];
     __shared__ uint32_t s_digit_start[kRadix];
     __shared__ uint32_t s_out_base[kRadix];
@@ -207,6 +196,17 @@ __global__ __launch_bounds__(kSortThreads) void sort_pass_kernel(
     const int64_t n = sort_count(n_cap, n_dev);
     if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
     for (int k = tid; k < kSortWaves * kRadix; k += kSortThreads) (&s_wave_cnt[0][0])[k] = 0;
+    // the pass's exclusive digit offsets from its digit counts, scanned by
+    // every tile (256 counts; no launch of its own): thread d takes digit d
+    // (kSortThreads == kRadix)
+    const uint32_t my_cnt = digit_count[tid];
+    uint32_t incl = my_cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t u = (uint32_t)__shfl_up((int)incl, o, 64);
+        if (lane >= o) incl += u;
+    }
+    if (lane == 63) s_wave_tot[wave] = incl;
     __syncthreads();
     const int64_t tile = s_tile;
     const int64_t base = tile * TILE;
@@ -214,13 +214,14 @@ __global__ __launch_bounds__(kSortThreads) void sort_pass_kernel(
     // past the last key leave (tile ids are handed out in order, so no tile
     // that a real one waits on is among them)
     if (base >= n) return;
+    uint32_t my_base = incl - my_cnt;
+    for (int w = 0; w < wave; ++w) my_base += s_wave_tot[w];
 
     // a pass whose digit every key shares (a constant exponent byte of the
     // depths) is the identity of a stable sort: the tile is copied through
-    // (the exclusive digit offsets show it: one digit spans all n keys)
+    // (one digit counts all n keys)
     {
-        const uint32_t next = tid + 1 < kRadix ? digit_base[tid + 1] : (uint32_t)n;
-        if (__syncthreads_or(next - digit_base[tid] == (uint32_t)n)) {
+        if (__syncthreads_or(my_cnt == (uint32_t)n)) {
             const int64_t end = min(base + (int64_t)TILE, n);
             for (int64_t j = base + tid; j < end; j += kSortThreads) {
                 keys_out[j] = keys_in[j];
@@ -361,7 +362,7 @@ __global__ __launch_bounds__(kSortThreads) void sort_pass_kernel(
             __hip_atomic_store(lookback + (size_t)tile * kRadix + d, kFlagPrefix | ((excl + tot) & kValueMask),
                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        s_out_base[d] = digit_base[d] + excl - s_digit_start[d];
+        s_out_base[d] = my_base + excl - s_digit_start[d];
     }
     __syncthreads();
 
@@ -479,8 +480,6 @@ static int sort_impl(int64_t n, const int64_t *n_dev, int64_t *keys, int64_t *ke
     sort_hist_kernel<<<hist_blocks, 256, 0, st>>>(n, n_dev, keys, begin_bit, end_bit, L.passes,
                                                   skip_passes == 0 ? 1 : 0, hist);
     UBS_LAUNCH_CHECK("sort_hist_kernel");
-    sort_hist_scan_kernel<<<L.passes, kRadix, 0, st>>>(hist);
-    UBS_LAUNCH_CHECK("sort_hist_scan_kernel");
     int64_t *kb[2] = {keys, keys_alt};
     int32_t *vb[2] = {vals, vals_alt};
     int run = 0;  // passes launched so far: the data sits in buffer run & 1
