@@ -73,7 +73,9 @@ def no_gc():
 # the blend entry points, timed with HIP events inside the timed steps
 BLEND_ENTRIES = ("ubs_rasterize_to_pixels_fwd", "ubs_rasterize_to_pixels_bwd", "ubs_rasterize_to_pixels_bwd_reuse")
 ENTRY_ALIASES = {"ubs_rasterize_to_pixels_bwd_reuse": "ubs_rasterize_to_pixels_bwd",
-                 "ubs_isect_tiles_emit_ordered_cap": "ubs_isect_tiles_emit_ordered"}
+                 "ubs_isect_tiles_emit_ordered_cap": "ubs_isect_tiles_emit_ordered",
+                 "ubs_radix_sort_pairs_flag": "ubs_radix_sort_pairs",
+                 "ubs_isect_tiles_count_info": "ubs_isect_tiles_count_range"}
 
 
 def launch_durations(timer) -> dict:
@@ -108,7 +110,7 @@ def algorithmic_bytes(name: str, c: dict, n_isect: int, C: int = 1) -> float:
         if name == "ubs_radix_sort_pairs" and mode == "segmented":
             return n_isect * (8.0 + 24.0 * tile_passes)
         passes = math.ceil(end_bit / 8)
-        if name.endswith("_skip"):  # passes over depth bytes every key shares are not run
+        if name.endswith("_skip") or mode == "onesweep":  # passes over depth bytes every key shares are not run
             passes -= c.get("sort_skipped", 0)
         return n_isect * (8.0 + 24.0 * passes)
     if name == "ubs_isect_offset_encode":

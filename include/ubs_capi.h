@@ -294,6 +294,16 @@ int ubs_isect_tiles_count_range(uint32_t C, uint32_t N, const float *means2d, co
                                 int64_t *n_isects_dev, uint32_t *range_dev, void *workspace,
                                 size_t workspace_bytes, void *stream);
 
+/* The same pass writing the three int64 words the host reads in one copy:
+ * info[0] = n_isects, info[1] = the depth range as above (two uint32),
+ * info[2] = *status_in (a status word the caller accumulates; NULL -> 0).
+ * No counterpart in the reference. */
+int ubs_isect_tiles_count_info(uint32_t C, uint32_t N, const float *means2d, const int32_t *radii,
+                               const float *depths, uint32_t tile_size, uint32_t tile_width,
+                               uint32_t tile_height, int32_t *tiles_per_gauss, int64_t *cum_tiles,
+                               int64_t *info, const int64_t *status_in, void *workspace,
+                               size_t workspace_bytes, void *stream);
+
 /* Bits the (cam, tile) prefix of an isect id occupies: 32 + this = end_bit. */
 uint32_t ubs_isect_prefix_bits(uint32_t C, uint32_t tile_width, uint32_t tile_height);
 
@@ -316,6 +326,14 @@ int ubs_radix_sort_pairs(int64_t n, int64_t *keys, int64_t *keys_alt, int32_t *v
 int ubs_radix_sort_pairs_skip(int64_t n, int64_t *keys, int64_t *keys_alt, int32_t *vals,
                               int32_t *vals_alt, int begin_bit, int end_bit, uint32_t skip_passes,
                               void *workspace, size_t workspace_bytes, int *result_in_alt, void *stream);
+/* ubs_radix_sort_pairs_skip that also sets *flag_dev = 1 (device memory,
+ * never cleared by the sort) when a look-back wait gives up, so a caller can
+ * collect the status of several sorts in one word and read it with a later
+ * host read.  No counterpart in the reference. */
+int ubs_radix_sort_pairs_flag(int64_t n, int64_t *keys, int64_t *keys_alt, int32_t *vals,
+                              int32_t *vals_alt, int begin_bit, int end_bit, uint32_t skip_passes,
+                              void *workspace, size_t workspace_bytes, int *result_in_alt, uint32_t *flag_dev,
+                              void *stream);
 
 /* Depth-first emission: the same intersection entries in an order that a
  * stable sort on the camera/tile bits alone ([32, 32 + tile_bits +

@@ -109,7 +109,9 @@ __global__ void isect_count_kernel(uint64_t total, const float *__restrict__ mea
 // Pass 1b: exclusive scan of the block sums in one block; the grand total
 // goes to *n_isects, and the depth range of all blocks to range[0..1].
 __global__ void isect_scan_blocks_kernel(int64_t nb, int64_t *__restrict__ block_sums, int64_t *__restrict__ n_isects,
-                                         const uint32_t *__restrict__ block_range, uint32_t *__restrict__ range) {
+                                         const uint32_t *__restrict__ block_range, uint32_t *__restrict__ range,
+                                         const int64_t *__restrict__ status_in = nullptr,
+                                         int64_t *__restrict__ status_out = nullptr) {
     __shared__ int64_t lds_wave[kThreads / 64];
     __shared__ int64_t carry;
     __shared__ uint32_t lds_max[kThreads / 64];
@@ -141,6 +143,7 @@ __global__ void isect_scan_blocks_kernel(int64_t nb, int64_t *__restrict__ block
         __syncthreads();
     }
     if (threadIdx.x == 0) *n_isects = carry;
+    if (status_out && threadIdx.x == 0) *status_out = status_in ? *status_in : 0;
 }
 
 // Pass 1c: int64 inclusive cumulative sum (blocked: each thread owns 4
@@ -329,10 +332,32 @@ int ubs_isect_tiles_count(uint32_t C, uint32_t N, const float *means2d, const in
                                        workspace, ws_bytes, stream);
 }
 
+static int count_impl(uint32_t C, uint32_t N, const float *means2d, const int32_t *radii, const float *depths,
+                      uint32_t ts, uint32_t tw, uint32_t th, int32_t *tpg, int64_t *cum, int64_t *n_isects_dev,
+                      uint32_t *range_dev, void *workspace, size_t ws_bytes, void *stream,
+                      const int64_t *status_in, int64_t *status_out);
+
 int ubs_isect_tiles_count_range(uint32_t C, uint32_t N, const float *means2d, const int32_t *radii,
                                 const float *depths, uint32_t ts, uint32_t tw, uint32_t th, int32_t *tpg,
                                 int64_t *cum, int64_t *n_isects_dev, uint32_t *range_dev, void *workspace,
                                 size_t ws_bytes, void *stream) {
+    return count_impl(C, N, means2d, radii, depths, ts, tw, th, tpg, cum, n_isects_dev, range_dev, workspace,
+                      ws_bytes, stream, nullptr, nullptr);
+}
+
+int ubs_isect_tiles_count_info(uint32_t C, uint32_t N, const float *means2d, const int32_t *radii,
+                               const float *depths, uint32_t ts, uint32_t tw, uint32_t th, int32_t *tpg,
+                               int64_t *cum, int64_t *info, const int64_t *status_in, void *workspace,
+                               size_t ws_bytes, void *stream) {
+    UBS_REQUIRE(info && depths, "null pointer");
+    return count_impl(C, N, means2d, radii, depths, ts, tw, th, tpg, cum, info,
+                      reinterpret_cast<uint32_t *>(info + 1), workspace, ws_bytes, stream, status_in, info + 2);
+}
+
+static int count_impl(uint32_t C, uint32_t N, const float *means2d, const int32_t *radii, const float *depths,
+                      uint32_t ts, uint32_t tw, uint32_t th, int32_t *tpg, int64_t *cum, int64_t *n_isects_dev,
+                      uint32_t *range_dev, void *workspace, size_t ws_bytes, void *stream,
+                      const int64_t *status_in, int64_t *status_out) {
     hipStream_t st = as_stream(stream);
     const uint64_t total = (uint64_t)C * N;
     UBS_REQUIRE(n_isects_dev, "n_isects_dev is null");
@@ -343,6 +368,12 @@ int ubs_isect_tiles_count_range(uint32_t C, uint32_t N, const float *means2d, co
         if (range_dev) {
             int e = zero_async(range_dev, 2 * sizeof(uint32_t), st);
             if (e) return e;
+        }
+        if (status_out) {  // the status word, by the scan kernel's copy
+            isect_scan_blocks_kernel<<<1, kThreads, 0, st>>>(0, nullptr, n_isects_dev, nullptr, nullptr, status_in,
+                                                             status_out);
+            UBS_LAUNCH_CHECK("isect_scan_blocks_kernel");
+            return UBS_OK;
         }
         return zero_async(n_isects_dev, sizeof(int64_t), st);
     }
@@ -358,7 +389,8 @@ int ubs_isect_tiles_count_range(uint32_t C, uint32_t N, const float *means2d, co
     const float *dp = range_dev ? depths : nullptr;
     isect_count_kernel<<<(unsigned)nb, kThreads, 0, st>>>(total, means2d, radii, dp, ts, tw, th, tpg, bs, br);
     UBS_LAUNCH_CHECK("isect_count_kernel");
-    isect_scan_blocks_kernel<<<1, kThreads, 0, st>>>((int64_t)nb, bs, n_isects_dev, br, range_dev);
+    isect_scan_blocks_kernel<<<1, kThreads, 0, st>>>((int64_t)nb, bs, n_isects_dev, br, range_dev, status_in,
+                                                     status_out);
     UBS_LAUNCH_CHECK("isect_scan_blocks_kernel");
     isect_cum_kernel<<<(unsigned)nb, kThreads, 0, st>>>(total, tpg, bs, cum, nullptr);
     UBS_LAUNCH_CHECK("isect_cum_kernel");

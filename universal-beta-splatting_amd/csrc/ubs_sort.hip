@@ -181,7 +181,7 @@ __global__ __launch_bounds__(kSortThreads) void sort_pass_kernel(
     int64_t n_cap, const int64_t *__restrict__ n_dev, const int64_t *__restrict__ keys_in, int64_t *__restrict__ keys_out,
     const int32_t *__restrict__ vals_in, int32_t *__restrict__ vals_out, int shift, int bits,
     const uint32_t *__restrict__ digit_count, uint32_t *__restrict__ tile_counter, uint32_t *lookback,
-    uint32_t *status) {
+    uint32_t *status, uint32_t *flag) {
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_wave_tot[kSortWaves];
     __shared__ uint32_t s_wave_cnt[kSortWaves][kRadix];
@@ -354,6 +354,7 @@ __global__ __launch_bounds__(kSortThreads) void sort_pass_kernel(
                 if (wait) {
                     if (++spins > kSpinLimit) {
                         __hip_atomic_store(status, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        if (flag) __hip_atomic_store(flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         break;
                     }
                     __builtin_amdgcn_s_sleep(1);
@@ -434,13 +435,21 @@ int ubs_radix_sort_pairs(int64_t n, int64_t *keys, int64_t *keys_alt, int32_t *v
 
 static int sort_impl(int64_t n, const int64_t *n_dev, int64_t *keys, int64_t *keys_alt, int32_t *vals,
                      int32_t *vals_alt, int begin_bit, int end_bit, uint32_t skip_passes, void *workspace,
-                     size_t ws_bytes, int *result_in_alt, void *stream);
+                     size_t ws_bytes, int *result_in_alt, void *stream, uint32_t *flag = nullptr);
 
 int ubs_radix_sort_pairs_skip(int64_t n, int64_t *keys, int64_t *keys_alt, int32_t *vals, int32_t *vals_alt,
                               int begin_bit, int end_bit, uint32_t skip_passes, void *workspace, size_t ws_bytes,
                               int *result_in_alt, void *stream) {
     return sort_impl(n, nullptr, keys, keys_alt, vals, vals_alt, begin_bit, end_bit, skip_passes, workspace,
                      ws_bytes, result_in_alt, stream);
+}
+
+int ubs_radix_sort_pairs_flag(int64_t n, int64_t *keys, int64_t *keys_alt, int32_t *vals, int32_t *vals_alt,
+                              int begin_bit, int end_bit, uint32_t skip_passes, void *workspace, size_t ws_bytes,
+                              int *result_in_alt, uint32_t *flag_dev, void *stream) {
+    UBS_REQUIRE(flag_dev, "flag_dev is null");
+    return sort_impl(n, nullptr, keys, keys_alt, vals, vals_alt, begin_bit, end_bit, skip_passes, workspace,
+                     ws_bytes, result_in_alt, stream, flag_dev);
 }
 
 int ubs_radix_sort_pairs_dev(int64_t n_cap, const int64_t *n_dev, int64_t *keys, int64_t *keys_alt, int32_t *vals,
@@ -453,7 +462,7 @@ int ubs_radix_sort_pairs_dev(int64_t n_cap, const int64_t *n_dev, int64_t *keys,
 
 static int sort_impl(int64_t n, const int64_t *n_dev, int64_t *keys, int64_t *keys_alt, int32_t *vals,
                      int32_t *vals_alt, int begin_bit, int end_bit, uint32_t skip_passes, void *workspace,
-                     size_t ws_bytes, int *result_in_alt, void *stream) {
+                     size_t ws_bytes, int *result_in_alt, void *stream, uint32_t *flag) {
     hipStream_t st = as_stream(stream);
     UBS_REQUIRE(result_in_alt, "result_in_alt is null");
     UBS_REQUIRE(begin_bit >= 0 && end_bit <= 64 && begin_bit <= end_bit, "bad bit range [%d, %d)", begin_bit,
@@ -492,7 +501,7 @@ static int sort_impl(int64_t n, const int64_t *n_dev, int64_t *keys, int64_t *ke
         auto *pass = n < kSmallSortN ? sort_pass_kernel<kItemsSmall> : sort_pass_kernel<kItems>;
         pass<<<(unsigned)L.tiles, kSortThreads, 0, st>>>(
             n, n_dev, kb[run & 1], kb[(run + 1) & 1], vb[run & 1], vb[(run + 1) & 1], shift, bits,
-            hist + (size_t)p * kRadix, counters + p, lookback + (size_t)p * L.tiles * kRadix, status);
+            hist + (size_t)p * kRadix, counters + p, lookback + (size_t)p * L.tiles * kRadix, status, flag);
         UBS_LAUNCH_CHECK("sort_pass_kernel");
         ++run;
     }

@@ -306,10 +306,11 @@ def isect_tiles_offsets(means2d: Tensor, radii: Tensor, depths: Tensor, tile_siz
     return isect_finish(state, n_isects, packed)
 
 
-# Per device: the sum of the status words of the isect-id sorts that ran after
-# the last host read (nonzero = a look-back wait gave up and that sort's order
-# is invalid).  The next isect_begin folds it into its info, so the host read
-# that already sizes the next step reports it.
+# Per device: the status of the isect-id sorts that ran after the last host
+# read (nonzero = a look-back wait gave up and that sort's order is invalid;
+# the sorts set it, ubs_radix_sort_pairs_flag).  The next isect_begin's
+# count kernels copy it into its info, so the host read that already sizes
+# the next step reports it.
 _late_sort_status = {}
 
 
@@ -375,18 +376,12 @@ def isect_begin(means2d: Tensor, radii: Tensor, depths: Tensor, tile_size: int, 
     S.tpg = torch.empty((C, N), dtype=torch.int32, device=dev)
     S.cum = torch.empty((C * N,), dtype=torch.int64, device=dev)
     # [n_isects, (~min depth bits, max depth bits) as two uint32, sort status],
-    # read in one sync
-    if sort and C * N > 0:  # the count kernels write info[0] and info[1]
-        S.info = torch.empty((3,), dtype=torch.int64, device=dev)
-        S.info[2:3].copy_(_late_status(dev))
-    else:
-        S.info = torch.zeros((3,), dtype=torch.int64, device=dev)
-        S.info[2:3].add_(_late_status(dev))
+    # written by the count kernels and read in one sync
+    S.info = torch.empty((3,), dtype=torch.int64, device=dev)
     ws_bytes = int(L.ubs_isect_count_workspace_bytes(C, N))
     ws = torch.empty((ws_bytes,), dtype=torch.uint8, device=dev)
-    _call("ubs_isect_tiles_count_range", C, N, ptr(means2d), ptr(radii), ptr(depths), tile_size, tile_width,
-          tile_height, ptr(S.tpg), ptr(S.cum), ptr(S.info), _vp_at(S.info, 1) if sort else None, ptr(ws), ws_bytes,
-          st)
+    _call("ubs_isect_tiles_count_info", C, N, ptr(means2d), ptr(radii), ptr(depths), tile_size, tile_width,
+          tile_height, ptr(S.tpg), ptr(S.cum), ptr(S.info), ptr(_late_status(dev)), ptr(ws), ws_bytes, st)
     S.order = S.info_host = S.ready = None
     if sort and _sort_mode == "depth_first" and C * N > 0:
         # the info leaves for the host first; the depth sort and its scan
@@ -472,7 +467,8 @@ def radix_sort_pairs(keys: Tensor, values: Tensor, begin_bit: int, end_bit: int,
     the 8-bit pass over bits [begin_bit + 8p, +8), which the caller knows
     every key shares (the pass would be the identity).  check_status reads
     the sort's status word now (a host sync); status_out (int64 [1] on the
-    device) instead gets it added, without a sync, for a later read."""
+    device) instead is set to 1 if the sort gives up (never cleared), without
+    a sync, for a later read."""
     require_cuda(keys, values)
     assert keys.dtype == torch.int64 and values.dtype == torch.int32
     n = keys.numel()
@@ -485,15 +481,17 @@ def radix_sort_pairs(keys: Tensor, values: Tensor, begin_bit: int, end_bit: int,
     import ctypes
 
     sel = ctypes.c_int(0)
-    if skip_passes:
+    if status_out is not None:
+        # a give-up sets the low word of status_out (int64, little-endian) to 1
+        assert status_out.dtype == torch.int64 and status_out.is_contiguous()
+        _call("ubs_radix_sort_pairs_flag", n, ptr(keys), ptr(keys_alt), ptr(values), ptr(vals_alt), begin_bit,
+              end_bit, int(skip_passes), ptr(ws), ws_bytes, ctypes.byref(sel), ptr(status_out), st)
+    elif skip_passes:
         _call("ubs_radix_sort_pairs_skip", n, ptr(keys), ptr(keys_alt), ptr(values), ptr(vals_alt), begin_bit,
               end_bit, int(skip_passes), ptr(ws), ws_bytes, ctypes.byref(sel), st)
     else:
         _call("ubs_radix_sort_pairs", n, ptr(keys), ptr(keys_alt), ptr(values), ptr(vals_alt), begin_bit,
               end_bit, ptr(ws), ws_bytes, ctypes.byref(sel), st)
-    if status_out is not None and n > 0:
-        # the status word (uint32 at offset 0, followed by zeroed bytes) as int64
-        status_out.add_(ws[:8].view(torch.int64))
     if check_status:
         status = ctypes.c_int(0)
         L.check(L.ubs_radix_sort_status(ptr(ws), ctypes.byref(status), st), "radix_sort_status")
