@@ -106,16 +106,48 @@ __global__ void isect_count_kernel(uint64_t total, const float *__restrict__ mea
     }
 }
 
-// Pass 1b: exclusive scan of the block sums in one block; the grand total
-// goes to *n_isects, and the depth range of all blocks to range[0..1].
-__global__ void isect_scan_blocks_kernel(int64_t nb, int64_t *__restrict__ block_sums, int64_t *__restrict__ n_isects,
-                                         const uint32_t *__restrict__ block_range, uint32_t *__restrict__ range,
-                                         const int64_t *__restrict__ status_in = nullptr,
-                                         int64_t *__restrict__ status_out = nullptr) {
+// Pass 1b: int64 inclusive cumulative sum (blocked: each thread owns 4
+// consecutive elements).  Each block takes its offset by adding the block
+// sums before it itself (at most nb int64 reads, from L2), so no scan launch
+// runs between the passes; block 0 also writes the grand total to *n_isects
+// (when given), the depth range of all blocks to range[0..1] (when given)
+// and *status_in (0 when null) to *status_out (when given).
+__global__ void isect_cum_kernel(uint64_t total, const int32_t *__restrict__ tpg,
+                                 const int64_t *__restrict__ block_sums, int64_t nb, int64_t *__restrict__ cum,
+                                 const int32_t *__restrict__ order, int64_t *__restrict__ n_isects,
+                                 const uint32_t *__restrict__ block_range, uint32_t *__restrict__ range,
+                                 const int64_t *__restrict__ status_in, int64_t *__restrict__ status_out) {
     __shared__ int64_t lds_wave[kThreads / 64];
-    __shared__ int64_t carry;
+    __shared__ int64_t lds_off[kThreads / 64];
     __shared__ uint32_t lds_max[kThreads / 64];
-    if (range) {  // grid-uniform
+    const bool first = blockIdx.x == 0;
+    // the block's offset; block 0 sums all nb block sums: the grand total
+    const int64_t upto = first && n_isects ? nb : (int64_t)blockIdx.x;
+    int64_t part = 0;
+    for (int64_t i = threadIdx.x; i < upto; i += kThreads) part += block_sums[i];
+    int64_t red;
+    block_exclusive_scan(part, lds_off, red);
+    const int64_t offset = first ? 0 : red;
+    if (first && n_isects && threadIdx.x == 0) *n_isects = red;
+    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
+    int32_t v[kScanItems];
+    int64_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        // (an order entry outside [0, total), only from a depth sort that gave
+        // up, counts no tiles)
+        const uint64_t o = base + k < total ? (order ? (uint64_t)(uint32_t)order[base + k] : base + k) : total;
+        v[k] = o < total ? tpg[o] : 0;
+        s += v[k];
+    }
+    int64_t tot;
+    int64_t run = block_exclusive_scan(s, lds_wave, tot) + offset;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        run += v[k];
+        if (base + k < total) cum[base + k] = run;
+    }
+    if (first && range) {  // block-uniform
         uint32_t lo_n = 0u, hi = 0u;
         for (int64_t i = threadIdx.x; i < nb; i += kThreads) {
             lo_n = max(lo_n, block_range[2 * i]);
@@ -129,47 +161,7 @@ __global__ void isect_scan_blocks_kernel(int64_t nb, int64_t *__restrict__ block
             range[1] = hi;
         }
     }
-    if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
-    for (int64_t start = 0; start < nb; start += kThreads) {
-        const int64_t i = start + threadIdx.x;
-        const int64_t v = i < nb ? block_sums[i] : 0;
-        int64_t tot;
-        const int64_t ex = block_exclusive_scan(v, lds_wave, tot);
-        const int64_t c0 = carry;
-        if (i < nb) block_sums[i] = c0 + ex;
-        __syncthreads();
-        if (threadIdx.x == 0) carry = c0 + tot;
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) *n_isects = carry;
-    if (status_out && threadIdx.x == 0) *status_out = status_in ? *status_in : 0;
-}
-
-// Pass 1c: int64 inclusive cumulative sum (blocked: each thread owns 4
-// consecutive elements).
-__global__ void isect_cum_kernel(uint64_t total, const int32_t *__restrict__ tpg,
-                                 const int64_t *__restrict__ block_offsets, int64_t *__restrict__ cum,
-                                 const int32_t *__restrict__ order) {
-    __shared__ int64_t lds_wave[kThreads / 64];
-    const uint64_t base = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanItems;
-    int32_t v[kScanItems];
-    int64_t s = 0;
-#pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        // (an order entry outside [0, total), only from a depth sort that gave
-        // up, counts no tiles)
-        const uint64_t o = base + k < total ? (order ? (uint64_t)(uint32_t)order[base + k] : base + k) : total;
-        v[k] = o < total ? tpg[o] : 0;
-        s += v[k];
-    }
-    int64_t tot;
-    int64_t run = block_exclusive_scan(s, lds_wave, tot) + block_offsets[blockIdx.x];
-#pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        run += v[k];
-        if (base + k < total) cum[base + k] = run;
-    }
+    if (first && status_out && threadIdx.x == 0) *status_out = status_in ? *status_in : 0;
 }
 
 // Depth-first emission (an order of the same entries that a stable sort on
@@ -364,18 +356,11 @@ static int count_impl(uint32_t C, uint32_t N, const float *means2d, const int32_
     UBS_REQUIRE(ts > 0, "tile_size must be > 0");
     UBS_REQUIRE(bit_width_u32(tw * th) + bit_width_u32(C) <= 32, "camera and tile ids need more than 32 bits");
     UBS_REQUIRE(!range_dev || depths, "range_dev needs depths");
-    if (total == 0) {  // (otherwise the scan writes both)
-        if (range_dev) {
-            int e = zero_async(range_dev, 2 * sizeof(uint32_t), st);
-            if (e) return e;
-        }
-        if (status_out) {  // the status word, by the scan kernel's copy
-            isect_scan_blocks_kernel<<<1, kThreads, 0, st>>>(0, nullptr, n_isects_dev, nullptr, nullptr, status_in,
-                                                             status_out);
-            UBS_LAUNCH_CHECK("isect_scan_blocks_kernel");
-            return UBS_OK;
-        }
-        return zero_async(n_isects_dev, sizeof(int64_t), st);
+    if (total == 0) {  // the totals (0), the range (0, 0) and the status word
+        isect_cum_kernel<<<1, kThreads, 0, st>>>(0, nullptr, nullptr, 0, nullptr, nullptr, n_isects_dev, nullptr,
+                                                 range_dev, status_in, status_out);
+        UBS_LAUNCH_CHECK("isect_cum_kernel");
+        return UBS_OK;
     }
     UBS_REQUIRE(total < (1ull << 31), "C*N must be < 2^31");
     UBS_REQUIRE(means2d && radii && tpg && cum, "null pointer");
@@ -389,10 +374,8 @@ static int count_impl(uint32_t C, uint32_t N, const float *means2d, const int32_
     const float *dp = range_dev ? depths : nullptr;
     isect_count_kernel<<<(unsigned)nb, kThreads, 0, st>>>(total, means2d, radii, dp, ts, tw, th, tpg, bs, br);
     UBS_LAUNCH_CHECK("isect_count_kernel");
-    isect_scan_blocks_kernel<<<1, kThreads, 0, st>>>((int64_t)nb, bs, n_isects_dev, br, range_dev, status_in,
-                                                     status_out);
-    UBS_LAUNCH_CHECK("isect_scan_blocks_kernel");
-    isect_cum_kernel<<<(unsigned)nb, kThreads, 0, st>>>(total, tpg, bs, cum, nullptr);
+    isect_cum_kernel<<<(unsigned)nb, kThreads, 0, st>>>(total, tpg, bs, (int64_t)nb, cum, nullptr, n_isects_dev,
+                                                        range_dev ? br : nullptr, range_dev, status_in, status_out);
     UBS_LAUNCH_CHECK("isect_cum_kernel");
     return UBS_OK;
 }
@@ -444,9 +427,8 @@ int ubs_isect_scan_ordered(uint32_t C, uint32_t N, const int32_t *tpg, const int
     int64_t *bs = reinterpret_cast<int64_t *>(workspace);
     isect_ordered_sums_kernel<<<(unsigned)nb, kThreads, 0, st>>>(total, tpg, order, bs);
     UBS_LAUNCH_CHECK("isect_ordered_sums_kernel");
-    isect_scan_blocks_kernel<<<1, kThreads, 0, st>>>((int64_t)nb, bs, bs + nb, nullptr, nullptr);
-    UBS_LAUNCH_CHECK("isect_scan_blocks_kernel");
-    isect_cum_kernel<<<(unsigned)nb, kThreads, 0, st>>>(total, tpg, bs, cum, order);
+    isect_cum_kernel<<<(unsigned)nb, kThreads, 0, st>>>(total, tpg, bs, (int64_t)nb, cum, order, nullptr, nullptr,
+                                                        nullptr, nullptr, nullptr);
     UBS_LAUNCH_CHECK("isect_cum_kernel");
     return UBS_OK;
 }
