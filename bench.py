@@ -71,8 +71,11 @@ def no_gc():
 # The blend backward that reuses the forward's packed records and tile order
 # (the autograd path's default) is the same stage as the plain backward.
 # the blend entry points, timed with HIP events inside the timed steps
-BLEND_ENTRIES = ("ubs_rasterize_to_pixels_fwd", "ubs_rasterize_to_pixels_bwd", "ubs_rasterize_to_pixels_bwd_reuse")
+BLEND_ENTRIES = ("ubs_rasterize_to_pixels_fwd", "ubs_rasterize_to_pixels_fwd_prepacked", "ubs_rasterize_to_pixels_bwd",
+                 "ubs_rasterize_to_pixels_bwd_reuse")
 ENTRY_ALIASES = {"ubs_rasterize_to_pixels_bwd_reuse": "ubs_rasterize_to_pixels_bwd",
+                 "ubs_rasterize_to_pixels_fwd_prepacked": "ubs_rasterize_to_pixels_fwd",
+                 "ubs_beta_project_fwd_rec": "ubs_beta_project_fwd",
                  "ubs_isect_tiles_emit_ordered_cap": "ubs_isect_tiles_emit_ordered",
                  "ubs_radix_sort_pairs_flag": "ubs_radix_sort_pairs",
                  "ubs_isect_tiles_count_info": "ubs_isect_tiles_count_range"}

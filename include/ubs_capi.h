@@ -197,6 +197,20 @@ int ubs_beta_project_fwd(int64_t N, int64_t D, uint32_t C, const float *xyz, con
                          int ortho, int32_t *radii, float *means2d, float *depths, float *conics, float *opacities,
                          float *betas, void *stream);
 
+/* ubs_beta_project_fwd that also writes, for every visible entry c N + n,
+ * the blend's 64-byte record (means2d, conic, opacity, beta_0, rgb[n] [N,3])
+ * into records (the first ubs_rasterize_workspace_bytes(C, N, 0) bytes of
+ * the blend's workspace), for ubs_rasterize_to_pixels_fwd_prepacked.  No
+ * counterpart in the reference. */
+int ubs_beta_project_fwd_rec(int64_t N, int64_t D, uint32_t C, const float *xyz, const float *mean_rest,
+                             const float *scale_raw, const float *l_triangle, const float *opacity_raw,
+                             const float *beta_raw, const float *quats, const float *cam_pos,
+                             const float *timestamps, const float *viewmats, const float *Ks, int32_t width,
+                             int32_t height, const int32_t *image_wh, float eps2d, float near_plane,
+                             float far_plane, float radius_clip, int ortho, int32_t *radii, float *means2d,
+                             float *depths, float *conics, float *opacities, float *betas, const float *rgb,
+                             void *records, void *stream);
+
 /* Camera centres [C,3] of world->camera viewmats [C,4,4] (row-major):
  * -(R^T t), with the FMA order the fused projection uses when its cam_pos is
  * NULL.  The reference takes the centre from its Camera object
@@ -467,6 +481,19 @@ int ubs_rasterize_to_pixels_fwd(uint32_t C, uint32_t N, int64_t n_isects, uint32
                                 const int32_t *flatten_ids, float *render_colors,
                                 float *render_alphas, int32_t *last_ids, float *T_final,
                                 void *workspace, size_t workspace_bytes, void *stream);
+
+/* ubs_rasterize_to_pixels_fwd whose workspace already holds the packed
+ * records (3 colour channels, tile_size 8 or 16: the 64-byte ones) of every
+ * visible entry, written by ubs_beta_project_fwd_rec: the packing pass is
+ * left out.  No counterpart in the reference. */
+int ubs_rasterize_to_pixels_fwd_prepacked(uint32_t C, uint32_t N, int64_t n_isects, uint32_t channels,
+                                          const float *means2d, const float *conics, const float *colors,
+                                          const float *opacities, const float *betas, const float *backgrounds,
+                                          const uint8_t *masks, uint32_t W, uint32_t H, uint32_t tile_size,
+                                          uint32_t tw, uint32_t th, const int32_t *tile_offsets,
+                                          const int32_t *flatten_ids, float *render_colors, float *render_alphas,
+                                          int32_t *last_ids, float *T_final, void *workspace,
+                                          size_t workspace_bytes, void *stream);
 
 /* Backward.  T_final may be NULL (then 1 - render_alphas is used, as the
  * reference does); v_render_alphas may be NULL (a zero gradient, without

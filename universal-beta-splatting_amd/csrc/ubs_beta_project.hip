@@ -107,7 +107,9 @@ __device__ __forceinline__ void beta_view_fwd(const BetaPrim<D> &bp, uint32_t c,
                                               float eps2d, float near_plane, float far_plane, float radius_clip,
                                               int ortho, int32_t *__restrict__ radii, float *__restrict__ means2d,
                                               float *__restrict__ depths, float *__restrict__ conics,
-                                              float *__restrict__ opacities, float *__restrict__ betas) {
+                                              float *__restrict__ opacities, float *__restrict__ betas,
+                                              const float *__restrict__ rgb_n = nullptr,
+                                              float4 *__restrict__ rec = nullptr) {
     constexpr int CQ = D - 3;
     float q[CQ];
     beta_query<D>(bp, cam_pos, timestamps, viewmats, c, q);
@@ -131,6 +133,11 @@ __device__ __forceinline__ void beta_view_fwd(const BetaPrim<D> &bp, uint32_t c,
     conics[idx * 3] = o.conic[0];
     conics[idx * 3 + 1] = o.conic[1];
     conics[idx * 3 + 2] = o.conic[2];
+    if (rec) {  // the blend's 64-byte record of the entry (what its packing writes)
+        rec[4 * idx] = make_float4(o.m2[0], o.m2[1], o.conic[0], o.conic[1]);
+        rec[4 * idx + 1] = make_float4(o.conic[2], oc, bp.beta[0], 0.f);
+        rec[4 * idx + 2] = make_float4(rgb_n[0], rgb_n[1], rgb_n[2], 0.f);
+    }
 }
 
 template <int D, bool QUAT>
@@ -142,15 +149,17 @@ __global__ __launch_bounds__(kThreads) void beta_project_fwd_kernel(
     int32_t W, int32_t H, const int32_t *__restrict__ image_wh, float eps2d, float near_plane, float far_plane,
     float radius_clip, int ortho, int32_t *__restrict__ radii, float *__restrict__ means2d,
     float *__restrict__ depths, float *__restrict__ conics, float *__restrict__ opacities,
-    float *__restrict__ betas, int64_t row_stride) {
+    float *__restrict__ betas, int64_t row_stride, const float *__restrict__ rgb = nullptr,
+    float4 *__restrict__ rec = nullptr) {
     const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= N) return;
     BetaPrim<D> bp;
     beta_prim_setup<D, QUAT>(n, xyz, mean_rest, scale_raw, ltri, opacity_raw, beta_raw, quats, bp);
+    const float *rgb_n = rec ? rgb + 3 * n : nullptr;
     for (uint32_t c = 0; c < C; ++c)
         beta_view_fwd<D>(bp, c, (int64_t)c * row_stride + n, cam_pos, timestamps, viewmats, Ks, W, H, image_wh,
                          eps2d, near_plane, far_plane, radius_clip, ortho, radii, means2d, depths, conics,
-                         opacities, betas);
+                         opacities, betas, rgb_n, rec);
 }
 
 // Views of several resident scenes in one batch: view c projects the scene
@@ -387,7 +396,7 @@ static int beta_project_fwd_impl(int64_t N, int64_t D, uint32_t C, const float *
                                  int32_t height, const int32_t *image_wh, float eps2d, float near_plane,
                                  float far_plane, float radius_clip, int ortho, int32_t *radii, float *means2d,
                                  float *depths, float *conics, float *opacities, float *betas, int64_t row_stride,
-                                 void *stream);
+                                 void *stream, const float *rgb = nullptr, void *records = nullptr);
 
 int ubs_beta_project_fwd(int64_t N, int64_t D, uint32_t C, const float *xyz, const float *mean_rest,
                          const float *scale_raw, const float *l_triangle, const float *opacity_raw,
@@ -400,6 +409,21 @@ int ubs_beta_project_fwd(int64_t N, int64_t D, uint32_t C, const float *xyz, con
                                  cam_pos, timestamps, viewmats, Ks, width, height, image_wh, eps2d, near_plane,
                                  far_plane, radius_clip, ortho, radii, means2d, depths, conics, opacities, betas, N,
                                  stream);
+}
+
+int ubs_beta_project_fwd_rec(int64_t N, int64_t D, uint32_t C, const float *xyz, const float *mean_rest,
+                             const float *scale_raw, const float *l_triangle, const float *opacity_raw,
+                             const float *beta_raw, const float *quats, const float *cam_pos, const float *timestamps,
+                             const float *viewmats, const float *Ks, int32_t width, int32_t height,
+                             const int32_t *image_wh, float eps2d, float near_plane, float far_plane,
+                             float radius_clip, int ortho, int32_t *radii, float *means2d, float *depths,
+                             float *conics, float *opacities, float *betas, const float *rgb, void *records,
+                             void *stream) {
+    UBS_REQUIRE(records && rgb, "null pointer");
+    return beta_project_fwd_impl(N, D, C, xyz, mean_rest, scale_raw, l_triangle, opacity_raw, beta_raw, quats,
+                                 cam_pos, timestamps, viewmats, Ks, width, height, image_wh, eps2d, near_plane,
+                                 far_plane, radius_clip, ortho, radii, means2d, depths, conics, opacities, betas, N,
+                                 stream, rgb, records);
 }
 
 int ubs_beta_project_fwd_rows(int64_t N, int64_t D, uint32_t C, const float *xyz, const float *mean_rest,
@@ -424,8 +448,9 @@ static int beta_project_fwd_impl(int64_t N, int64_t D, uint32_t C, const float *
                                  int32_t height, const int32_t *image_wh, float eps2d, float near_plane,
                                  float far_plane, float radius_clip, int ortho, int32_t *radii, float *means2d,
                                  float *depths, float *conics, float *opacities, float *betas, int64_t row_stride,
-                                 void *stream) {
+                                 void *stream, const float *rgb, void *records) {
     UBS_REQUIRE(D == 6 || D == 7, "the fused Beta projection is defined for D = 6 or 7 (got %lld)", (long long)D);
+    UBS_REQUIRE(!records || (rgb && row_stride == N), "records need rgb and row_stride == N");
     UBS_REQUIRE(N >= 0 && (uint64_t)row_stride * C < (1ull << 31), "C*N must be < 2^31");
     if (N == 0 || C == 0) return UBS_OK;
     UBS_REQUIRE(xyz && mean_rest && scale_raw && l_triangle && opacity_raw && beta_raw && viewmats && Ks && radii &&
@@ -437,7 +462,7 @@ static int beta_project_fwd_impl(int64_t N, int64_t D, uint32_t C, const float *
     beta_project_fwd_kernel<DD, QQ><<<grid, kThreads, 0, st>>>(                                                  \
         N, C, xyz, mean_rest, scale_raw, l_triangle, opacity_raw, beta_raw, quats, cam_pos, timestamps, viewmats, \
         Ks, width, height, image_wh, eps2d, near_plane, far_plane, radius_clip, ortho, radii, means2d, depths,    \
-        conics, opacities, betas, row_stride)
+        conics, opacities, betas, row_stride, rgb, static_cast<float4 *>(records))
     if (D == 6) {
         if (quats) UBS_BP_FWD(6, true);
         else UBS_BP_FWD(6, false);

@@ -339,7 +339,8 @@ static int raster_fwd_impl(uint32_t C, uint32_t N, int64_t n_isects, uint32_t ch
                            uint32_t tile_size, uint32_t tw, uint32_t th, const int32_t *tile_offsets,
                            const int32_t *flatten_ids, float *render_colors, float *render_alphas,
                            int32_t *last_ids, float *T_final, float *T_ckpt, size_t ckpt_n, void *workspace,
-                           size_t workspace_bytes, void *stream, const int64_t *n_dev = nullptr) {
+                           size_t workspace_bytes, void *stream, const int64_t *n_dev = nullptr,
+                           bool prepacked = false) {
     hipStream_t st = as_stream(stream);
     UBS_REQUIRE(tile_size == 8 || tile_size == 16 || tile_size == 32, "tile_size must be 8, 16 or 32 (got %u)",
                 tile_size);
@@ -367,12 +368,15 @@ static int raster_fwd_impl(uint32_t C, uint32_t N, int64_t n_isects, uint32_t ch
             return UBS_ERR_WORKSPACE;
         }
         const uint64_t CN = (uint64_t)C * N;
+        const bool wide = raster_wide_records(channels, tile_size);
+        UBS_REQUIRE(!prepacked || (wide && channels == 3), "prepacked records are the wide 3-channel ones");
         if (CN && n_isects) {
-            const bool wide = raster_wide_records(channels, tile_size);
-            raster_pack_kernel<<<grid_for((int64_t)CN), kThreads, 0, st>>>(
-                CN, means2d, conics, opacities, betas, static_cast<float4 *>(workspace), wide ? colors : nullptr,
-                channels);
-            UBS_LAUNCH_CHECK("raster_pack_kernel");
+            if (!prepacked) {
+                raster_pack_kernel<<<grid_for((int64_t)CN), kThreads, 0, st>>>(
+                    CN, means2d, conics, opacities, betas, static_cast<float4 *>(workspace), wide ? colors : nullptr,
+                    channels);
+                UBS_LAUNCH_CHECK("raster_pack_kernel");
+            }
             rec = static_cast<const float4 *>(workspace);
             if (workspace_bytes >= need + raster_order_bytes(C, tw, th)) {  // room for the block order
                 uint32_t *o = reinterpret_cast<uint32_t *>(static_cast<char *>(workspace) + need);
@@ -415,6 +419,18 @@ extern "C" int ubs_rasterize_to_pixels_fwd(uint32_t C, uint32_t N, int64_t n_ise
     return raster_fwd_impl(C, N, n_isects, channels, means2d, conics, colors, opacities, betas, backgrounds, masks, W,
                            H, tile_size, tw, th, tile_offsets, flatten_ids, render_colors, render_alphas, last_ids,
                            T_final, nullptr, 0, workspace, workspace_bytes, stream);
+}
+
+extern "C" int ubs_rasterize_to_pixels_fwd_prepacked(
+    uint32_t C, uint32_t N, int64_t n_isects, uint32_t channels, const float *means2d, const float *conics,
+    const float *colors, const float *opacities, const float *betas, const float *backgrounds, const uint8_t *masks,
+    uint32_t W, uint32_t H, uint32_t tile_size, uint32_t tw, uint32_t th, const int32_t *tile_offsets,
+    const int32_t *flatten_ids, float *render_colors, float *render_alphas, int32_t *last_ids, float *T_final,
+    void *workspace, size_t workspace_bytes, void *stream) {
+    UBS_REQUIRE(workspace, "prepacked records need the workspace that holds them");
+    return raster_fwd_impl(C, N, n_isects, channels, means2d, conics, colors, opacities, betas, backgrounds, masks, W,
+                           H, tile_size, tw, th, tile_offsets, flatten_ids, render_colors, render_alphas, last_ids,
+                           T_final, nullptr, 0, workspace, workspace_bytes, stream, nullptr, true);
 }
 
 extern "C" int ubs_rasterize_to_pixels_fwd_dev(uint32_t C, uint32_t N, int64_t n_cap, const int64_t *n_dev,

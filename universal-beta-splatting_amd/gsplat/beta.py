@@ -24,6 +24,7 @@ from .cuda._wrapper import (
     beta_activations,
     beta_project,
     camera_centers,
+    prepacked_blend_workspace,
     cond_mean_convariance_opacity,
     isect_begin,
     isect_finish,
@@ -176,12 +177,15 @@ def render_views(params: BetaParams, viewmats: Tensor, Ks: Tensor, width: int, h
     assert viewmats.shape == (C, 4, 4) and Ks.shape == (C, 3, 3), (viewmats.shape, Ks.shape)
     if cam_pos is not None:  # (None: the projection takes each view's centre itself)
         cam_pos = cam_pos.detach().to(torch.float32).contiguous()
+    tw = math.ceil(width / float(tile_size))
+    th = math.ceil(height / float(tile_size))
+    # the projection also writes the blend's per-entry records (no packing pass)
+    ws = prepacked_blend_workspace(C, params.xyz.shape[0], params.rgb.shape[-1], tile_size, tw, th,
+                                   params.xyz.device)
     radii, means2d, depths, conics, opacities, betas = beta_project(
         params.xyz, params.mean_rest, params.scale_raw, params.l_triangle, params.opacity_raw, params.beta_raw,
         params.quats, viewmats, Ks, width, height, cam_pos=cam_pos, timestamps=timestamps, eps2d=eps2d,
-        near_plane=near_plane, far_plane=far_plane, radius_clip=radius_clip)
-    tw = math.ceil(width / float(tile_size))
-    th = math.ceil(height / float(tile_size))
+        near_plane=near_plane, far_plane=far_plane, radius_clip=radius_clip, rgb=params.rgb, records=ws)
     tiles_per_gauss, isect_ids, flatten_ids, isect_offsets = isect_tiles_offsets(means2d, radii, depths, tile_size,
                                                                                 tw, th)
     if isect_offsets is None:
@@ -190,7 +194,7 @@ def render_views(params: BetaParams, viewmats: Tensor, Ks: Tensor, width: int, h
     colors = params.rgb[None] if C == 1 else params.rgb.expand(C, -1, -1)
     render_colors, render_alphas = rasterize_to_pixels(
         means2d, conics, colors, opacities, betas, width, height, tile_size, isect_offsets, flatten_ids,
-        backgrounds=backgrounds)
+        backgrounds=backgrounds, prepacked_workspace=ws)
     meta = dict(radii=radii, means2d=means2d, depths=depths, conics=conics, opacities=opacities, betas=betas,
                 tile_width=tw, tile_height=th, tiles_per_gauss=tiles_per_gauss, isect_ids=isect_ids,
                 flatten_ids=flatten_ids, isect_offsets=isect_offsets, width=width, height=height,

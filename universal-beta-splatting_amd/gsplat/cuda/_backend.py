@@ -52,6 +52,9 @@ _SIGNATURES = {
     "ubs_beta_project_fwd": (
         _c_int, [_c_i64, _c_i64, _c_u32] + [_vp] * 11 + [_c_i32, _c_i32, _vp, _c_f32, _c_f32, _c_f32, _c_f32,
                                                        _c_int] + [_vp] * 7),
+    "ubs_beta_project_fwd_rec": (
+        _c_int, [_c_i64, _c_i64, _c_u32] + [_vp] * 11 + [_c_i32, _c_i32, _vp, _c_f32, _c_f32, _c_f32, _c_f32,
+                                                       _c_int] + [_vp] * 9),
     "ubs_beta_project_bwd": (
         _c_int, [_c_i64, _c_i64, _c_u32] + [_vp] * 11 + [_c_i32, _c_i32, _vp, _c_f32, _c_int] + [_vp] * 15),
     "ubs_camera_centers": (_c_int, [_c_u32, _vp, _vp, _vp]),
@@ -95,6 +98,9 @@ _SIGNATURES = {
     "ubs_rasterize_workspace_bytes": (_c_size, [_c_u32, _c_u32, _c_int]),
     "ubs_rasterize_workspace_bytes_tiles": (_c_size, [_c_u32, _c_u32, _c_u32, _c_u32, _c_int]),
     "ubs_rasterize_to_pixels_fwd": (
+        _c_int, [_c_u32, _c_u32, _c_i64, _c_u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_u32, _c_u32, _c_u32,
+                 _c_u32, _c_u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_size, _vp]),
+    "ubs_rasterize_to_pixels_fwd_prepacked": (
         _c_int, [_c_u32, _c_u32, _c_i64, _c_u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_u32, _c_u32, _c_u32,
                  _c_u32, _c_u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_size, _vp]),
     "ubs_rasterize_to_pixels_fwd_dev": (

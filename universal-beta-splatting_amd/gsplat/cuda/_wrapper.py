@@ -156,6 +156,8 @@ def beta_project(
     far_plane: float = 1e10,
     radius_clip: float = 0.0,
     ortho: bool = False,
+    rgb: Optional[Tensor] = None,  # [N, 3], with records
+    records: Optional[Tensor] = None,  # a prepacked_blend_workspace: the blend's records are written too
 ) -> Tuple[Tensor, Tensor, Tensor, Tensor, Tensor, Tensor]:
     """Fused Beta projection of a 6-D or 7-D scene for C views, from the leaf
     parameters (pre-activation, BetaModel's layout) to the per-view 2-D
@@ -178,7 +180,8 @@ def beta_project(
         viewmats.contiguous(), Ks.contiguous(), None if cam_pos is None else cam_pos.detach().contiguous(),
         None if timestamps is None else timestamps.detach().float().contiguous(),
         None if image_sizes is None else image_sizes.to(torch.int32).contiguous(), int(width), int(height),
-        float(eps2d), float(near_plane), float(far_plane), float(radius_clip), bool(ortho))
+        float(eps2d), float(near_plane), float(far_plane), float(radius_clip), bool(ortho),
+        None if records is None else rgb.detach().contiguous(), records)
 
 
 def fully_fused_projection(
@@ -527,9 +530,12 @@ def rasterize_to_pixels(
     flatten_ids: Tensor,  # [n_isects]
     backgrounds: Optional[Tensor] = None,  # [C, channels]
     masks: Optional[Tensor] = None,  # [C, tile_height, tile_width]
+    prepacked_workspace: Optional[Tensor] = None,
 ) -> Tuple[Tensor, Tensor]:
     """Front-to-back Beta blending: (render_colors [C,H,W,channels],
-    render_alphas [C,H,W,1]).  reference: _wrapper.py:366-491"""
+    render_alphas [C,H,W,1]).  reference: _wrapper.py:366-491
+    prepacked_workspace: a prepacked_blend_workspace whose records the fused
+    projection wrote (beta_project(records=...)) for these inputs."""
     C = isect_offsets.size(0)
     N = means2d.size(1)
     assert means2d.shape == (C, N, 2), means2d.shape
@@ -554,7 +560,7 @@ def rasterize_to_pixels(
     render_colors, render_alphas = _RasterizeToPixels.apply(
         means2d.contiguous(), conics.contiguous(), colors.contiguous(), opacities.contiguous(),
         betas.contiguous(), backgrounds, masks, image_width, image_height, tile_size,
-        isect_offsets.contiguous(), flatten_ids.contiguous())
+        isect_offsets.contiguous(), flatten_ids.contiguous(), prepacked_workspace)
     return render_colors, render_alphas
 
 
@@ -627,6 +633,18 @@ def blend_workspace_bytes(C: int, N: int, tile_width: int, tile_height: int, bac
     return int(L.ubs_rasterize_workspace_bytes(C, N, 1 if backward else 0))
 
 
+def prepacked_blend_workspace(C: int, N: int, channels: int, tile_size: int, tile_width: int, tile_height: int,
+                              device):
+    """A blend workspace for the fused projection to write the blend's
+    records into (beta_project(records=...)), then handed to
+    rasterize_to_pixels(prepacked_workspace=...); None where the blend packs
+    them itself (not 3 channels, 32-pixel tiles, transmittance checkpoints,
+    packed records off)."""
+    if channels != 3 or tile_size not in (8, 16) or _t_checkpoint:
+        return None
+    return _blend_workspace(C, N, False, device, tile_width, tile_height)[0]
+
+
 def _blend_workspace(C: int, N: int, backward: bool, device, tile_width: int, tile_height: int):
     if not _packed_records or C * N == 0:
         return None, 0
@@ -650,7 +668,7 @@ def raster_fwd_ckpt_raw(means2d, conics, colors, opacities, betas, backgrounds, 
 
 
 def _raster_fwd(means2d, conics, colors, opacities, betas, backgrounds, masks, width, height, tile_size,
-                isect_offsets, flatten_ids, checkpoint: bool):
+                isect_offsets, flatten_ids, checkpoint: bool, prepacked=None):
     """(render_colors, render_alphas, last_ids, T_final, T_ckpt or None, the
     workspace: None, or the packed records and, with the tile order on, the
     order, which the backward of the same inputs can reuse)."""
@@ -666,9 +684,14 @@ def _raster_fwd(means2d, conics, colors, opacities, betas, backgrounds, masks, w
     last_ids = torch.empty((C, height, width), dtype=torch.int32, device=dev)
     T_final = torch.empty((C, height, width), dtype=torch.float32, device=dev)
     mp = _mask_ptr(masks)
-    ws, ws_bytes = _blend_workspace(C, N, False, dev, tw, th)
+    if prepacked is not None and not checkpoint:  # records written by the fused projection
+        ws, ws_bytes = prepacked, prepacked.numel()
+    else:
+        prepacked = None
+        ws, ws_bytes = _blend_workspace(C, N, False, dev, tw, th)
     if not checkpoint:
-        _call("ubs_rasterize_to_pixels_fwd", C, N, flatten_ids.numel(), channels, ptr(means2d), ptr(conics),
+        entry = "ubs_rasterize_to_pixels_fwd_prepacked" if prepacked is not None else "ubs_rasterize_to_pixels_fwd"
+        _call(entry, C, N, flatten_ids.numel(), channels, ptr(means2d), ptr(conics),
               ptr(colors), ptr(opacities), ptr(betas), ptr(backgrounds), mp[0] if mp else None, width, height,
               tile_size, tw, th, ptr(isect_offsets), ptr(flatten_ids), ptr(render_colors), ptr(render_alphas),
               ptr(last_ids), ptr(T_final), ptr(ws), ws_bytes, stream_of(means2d))
@@ -911,7 +934,8 @@ def projection_bwd_raw(means, covars, quats, scales, viewmats, Ks, width, height
 
 
 def beta_project_fwd_raw(xyz, mean_rest, scale_raw, l_triangle, opacity_raw, beta_raw, quats, cam_pos, timestamps,
-                         viewmats, Ks, width, height, image_wh, eps2d, near_plane, far_plane, radius_clip, ortho):
+                         viewmats, Ks, width, height, image_wh, eps2d, near_plane, far_plane, radius_clip, ortho,
+                         rgb=None, records=None):
     """ubs_beta_project_fwd: (radii, means2d, depths, conics, opacities, betas), each [C,N,...]."""
     require_cuda(xyz, mean_rest, scale_raw, l_triangle, opacity_raw, beta_raw, quats, cam_pos, timestamps,
                  viewmats, Ks, image_wh)
@@ -935,10 +959,17 @@ def beta_project_fwd_raw(xyz, mean_rest, scale_raw, l_triangle, opacity_raw, bet
     conics = torch.empty((C, N, 3), dtype=dt, device=dev)
     opacities = torch.empty((C, N), dtype=dt, device=dev)
     betas = torch.empty((C, N), dtype=dt, device=dev)
-    _call("ubs_beta_project_fwd", N, D, C, ptr(xyz), ptr(mean_rest), ptr(scale_raw), ptr(l_triangle),
-          ptr(opacity_raw), ptr(beta_raw), ptr(quats), ptr(cam_pos), ptr(timestamps), ptr(viewmats), ptr(Ks),
-          width, height, ptr(image_wh), eps2d, near_plane, far_plane, radius_clip, int(ortho), ptr(radii),
-          ptr(means2d), ptr(depths), ptr(conics), ptr(opacities), ptr(betas), stream_of(xyz))
+    args = (N, D, C, ptr(xyz), ptr(mean_rest), ptr(scale_raw), ptr(l_triangle), ptr(opacity_raw), ptr(beta_raw),
+            ptr(quats), ptr(cam_pos), ptr(timestamps), ptr(viewmats), ptr(Ks), width, height, ptr(image_wh), eps2d,
+            near_plane, far_plane, radius_clip, int(ortho), ptr(radii), ptr(means2d), ptr(depths), ptr(conics),
+            ptr(opacities), ptr(betas))
+    if records is not None:  # the blend's records too (rgb [N,3])
+        require_cuda(rgb, records)
+        require_dtype(_F32, rgb)
+        assert rgb.shape == (N, 3), rgb.shape
+        _call("ubs_beta_project_fwd_rec", *args, ptr(rgb), ptr(records), stream_of(xyz))
+    else:
+        _call("ubs_beta_project_fwd", *args, stream_of(xyz))
     return radii, means2d, depths, conics, opacities, betas
 
 
@@ -976,10 +1007,11 @@ def beta_project_bwd_raw(xyz, mean_rest, scale_raw, l_triangle, opacity_raw, bet
 class _BetaProject(torch.autograd.Function):
     @staticmethod
     def forward(ctx, xyz, mean_rest, scale_raw, l_triangle, opacity_raw, beta_raw, quats, viewmats, Ks, cam_pos,
-                timestamps, image_wh, width, height, eps2d, near_plane, far_plane, radius_clip, ortho):
+                timestamps, image_wh, width, height, eps2d, near_plane, far_plane, radius_clip, ortho, rgb=None,
+                records=None):
         radii, means2d, depths, conics, opacities, betas = beta_project_fwd_raw(
             xyz, mean_rest, scale_raw, l_triangle, opacity_raw, beta_raw, quats, cam_pos, timestamps, viewmats, Ks,
-            width, height, image_wh, eps2d, near_plane, far_plane, radius_clip, ortho)
+            width, height, image_wh, eps2d, near_plane, far_plane, radius_clip, ortho, rgb, records)
         ctx.save_for_backward(xyz, mean_rest, scale_raw, l_triangle, opacity_raw, beta_raw, quats, viewmats, Ks,
                               cam_pos, timestamps, image_wh, radii, conics)
         ctx.width, ctx.height, ctx.eps2d, ctx.ortho = width, height, eps2d, ortho
@@ -1001,7 +1033,7 @@ class _BetaProject(torch.autograd.Function):
             ctx.width, ctx.height, image_wh, ctx.eps2d, ctx.ortho, radii, conics, g(v_means2d, (C, N, 2)),
             v_depths, g(v_conics, (C, N, 3)), g(v_opacities, (C, N)), g(v_betas, (C, N)))  # (None: zeros)
         out = [gr if need else None for gr, need in zip(grads, ctx.needs_input_grad[:7])]
-        return (*out, None, None, None, None, None, None, None, None, None, None, None, None)
+        return (*out, None, None, None, None, None, None, None, None, None, None, None, None, None, None)
 
 
 class _BetaActivations(torch.autograd.Function):
@@ -1101,13 +1133,13 @@ class _FullyFusedProjection(torch.autograd.Function):
 class _RasterizeToPixels(torch.autograd.Function):
     @staticmethod
     def forward(ctx, means2d, conics, colors, opacities, betas, backgrounds, masks, width, height, tile_size,
-                isect_offsets, flatten_ids):
+                isect_offsets, flatten_ids, prepacked=None):
         ckpt = _t_checkpoint and any(ctx.needs_input_grad[:5])
         # an output without a gradient comes back as None, not a filled zero tensor
         ctx.set_materialize_grads(False)
         render_colors, render_alphas, last_ids, T_final, T_ckpt, ws = _raster_fwd(
             means2d, conics, colors, opacities, betas, backgrounds, masks, width, height, tile_size,
-            isect_offsets, flatten_ids, ckpt)
+            isect_offsets, flatten_ids, ckpt, prepacked)
         ctx.save_for_backward(means2d, conics, colors, opacities, betas, backgrounds, masks, isect_offsets,
                               flatten_ids, render_alphas, last_ids, T_final, T_ckpt)
         ctx.width, ctx.height, ctx.tile_size = width, height, tile_size
@@ -1133,4 +1165,4 @@ class _RasterizeToPixels(torch.autograd.Function):
         else:
             v_backgrounds = None
         return (v_means2d, v_conics, v_colors, v_opacities, v_betas, v_backgrounds,
-                None, None, None, None, None, None)
+                None, None, None, None, None, None, None)
