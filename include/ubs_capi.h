@@ -310,13 +310,15 @@ int ubs_isect_tiles_count_range(uint32_t C, uint32_t N, const float *means2d, co
 
 /* The same pass writing the three int64 words the host reads in one copy:
  * info[0] = n_isects, info[1] = the depth range as above (two uint32),
- * info[2] = *status_in (a status word the caller accumulates; NULL -> 0).
- * No counterpart in the reference. */
+ * info[2] = *status_in (a status word the caller accumulates; NULL -> 0);
+ * with depth_keys / depth_vals (both or neither) it also writes what
+ * ubs_isect_depth_keys writes, for the depth-first emission's sort.  No
+ * counterpart in the reference. */
 int ubs_isect_tiles_count_info(uint32_t C, uint32_t N, const float *means2d, const int32_t *radii,
                                const float *depths, uint32_t tile_size, uint32_t tile_width,
                                uint32_t tile_height, int32_t *tiles_per_gauss, int64_t *cum_tiles,
-                               int64_t *info, const int64_t *status_in, void *workspace,
-                               size_t workspace_bytes, void *stream);
+                               int64_t *info, const int64_t *status_in, int64_t *depth_keys,
+                               int32_t *depth_vals, void *workspace, size_t workspace_bytes, void *stream);
 
 /* Bits the (cam, tile) prefix of an isect id occupies: 32 + this = end_bit. */
 uint32_t ubs_isect_prefix_bits(uint32_t C, uint32_t tile_width, uint32_t tile_height);

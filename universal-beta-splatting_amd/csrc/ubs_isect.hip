@@ -72,7 +72,8 @@ __device__ __forceinline__ uint32_t block_max_u32(uint32_t v, uint32_t *lds_wave
 __global__ void isect_count_kernel(uint64_t total, const float *__restrict__ means2d,
                                    const int32_t *__restrict__ radii, const float *__restrict__ depths,
                                    uint32_t ts, uint32_t tw, uint32_t th, int32_t *__restrict__ tpg,
-                                   int64_t *__restrict__ block_sums, uint32_t *__restrict__ block_range) {
+                                   int64_t *__restrict__ block_sums, uint32_t *__restrict__ block_range,
+                                   int64_t *__restrict__ dkeys = nullptr, int32_t *__restrict__ dvals = nullptr) {
     __shared__ int64_t lds_wave[kThreads / 64];
     __shared__ uint32_t lds_max[kThreads / 64];
     const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
@@ -89,6 +90,10 @@ __global__ void isect_count_kernel(uint64_t total, const float *__restrict__ mea
                 const uint32_t b = __float_as_uint(depths[idx]);
                 lo_n = max(lo_n, ~b);
                 hi = max(hi, b);
+            }
+            if (dkeys) {  // the depth-first emission's sort input, as isect_depth_keys_kernel
+                dkeys[idx] = radii[idx] > 0 ? (int64_t)__float_as_uint(depths[idx]) : (int64_t)0xFFFFFFFFu;
+                dvals[idx] = (int32_t)idx;
             }
         }
     }
@@ -327,7 +332,8 @@ int ubs_isect_tiles_count(uint32_t C, uint32_t N, const float *means2d, const in
 static int count_impl(uint32_t C, uint32_t N, const float *means2d, const int32_t *radii, const float *depths,
                       uint32_t ts, uint32_t tw, uint32_t th, int32_t *tpg, int64_t *cum, int64_t *n_isects_dev,
                       uint32_t *range_dev, void *workspace, size_t ws_bytes, void *stream,
-                      const int64_t *status_in, int64_t *status_out);
+                      const int64_t *status_in, int64_t *status_out, int64_t *dkeys = nullptr,
+                      int32_t *dvals = nullptr);
 
 int ubs_isect_tiles_count_range(uint32_t C, uint32_t N, const float *means2d, const int32_t *radii,
                                 const float *depths, uint32_t ts, uint32_t tw, uint32_t th, int32_t *tpg,
@@ -339,17 +345,19 @@ int ubs_isect_tiles_count_range(uint32_t C, uint32_t N, const float *means2d, co
 
 int ubs_isect_tiles_count_info(uint32_t C, uint32_t N, const float *means2d, const int32_t *radii,
                                const float *depths, uint32_t ts, uint32_t tw, uint32_t th, int32_t *tpg,
-                               int64_t *cum, int64_t *info, const int64_t *status_in, void *workspace,
-                               size_t ws_bytes, void *stream) {
+                               int64_t *cum, int64_t *info, const int64_t *status_in, int64_t *depth_keys,
+                               int32_t *depth_vals, void *workspace, size_t ws_bytes, void *stream) {
     UBS_REQUIRE(info && depths, "null pointer");
+    UBS_REQUIRE(!depth_keys == !depth_vals, "depth_keys and depth_vals go together");
     return count_impl(C, N, means2d, radii, depths, ts, tw, th, tpg, cum, info,
-                      reinterpret_cast<uint32_t *>(info + 1), workspace, ws_bytes, stream, status_in, info + 2);
+                      reinterpret_cast<uint32_t *>(info + 1), workspace, ws_bytes, stream, status_in, info + 2,
+                      depth_keys, depth_vals);
 }
 
 static int count_impl(uint32_t C, uint32_t N, const float *means2d, const int32_t *radii, const float *depths,
                       uint32_t ts, uint32_t tw, uint32_t th, int32_t *tpg, int64_t *cum, int64_t *n_isects_dev,
                       uint32_t *range_dev, void *workspace, size_t ws_bytes, void *stream,
-                      const int64_t *status_in, int64_t *status_out) {
+                      const int64_t *status_in, int64_t *status_out, int64_t *dkeys, int32_t *dvals) {
     hipStream_t st = as_stream(stream);
     const uint64_t total = (uint64_t)C * N;
     UBS_REQUIRE(n_isects_dev, "n_isects_dev is null");
@@ -372,7 +380,9 @@ static int count_impl(uint32_t C, uint32_t N, const float *means2d, const int32_
     int64_t *bs = reinterpret_cast<int64_t *>(workspace);
     uint32_t *br = reinterpret_cast<uint32_t *>(bs + nb);
     const float *dp = range_dev ? depths : nullptr;
-    isect_count_kernel<<<(unsigned)nb, kThreads, 0, st>>>(total, means2d, radii, dp, ts, tw, th, tpg, bs, br);
+    UBS_REQUIRE(!dkeys || depths, "depth keys need depths");
+    isect_count_kernel<<<(unsigned)nb, kThreads, 0, st>>>(total, means2d, radii, dp ? dp : (dkeys ? depths : nullptr),
+                                                         ts, tw, th, tpg, bs, br, dkeys, dvals);
     UBS_LAUNCH_CHECK("isect_count_kernel");
     isect_cum_kernel<<<(unsigned)nb, kThreads, 0, st>>>(total, tpg, bs, (int64_t)nb, cum, nullptr, n_isects_dev,
                                                         range_dev ? br : nullptr, range_dev, status_in, status_out);

@@ -383,19 +383,21 @@ def isect_begin(means2d: Tensor, radii: Tensor, depths: Tensor, tile_size: int, 
     S.info = torch.empty((3,), dtype=torch.int64, device=dev)
     ws_bytes = int(L.ubs_isect_count_workspace_bytes(C, N))
     ws = torch.empty((ws_bytes,), dtype=torch.uint8, device=dev)
+    depth_first = sort and _sort_mode == "depth_first" and C * N > 0
+    # the depth-first sort's input (depth bits, flat ids), written by the count pass
+    dkeys = torch.empty((C * N,), dtype=torch.int64, device=dev) if depth_first else None
+    dvals = torch.empty((C * N,), dtype=torch.int32, device=dev) if depth_first else None
     _call("ubs_isect_tiles_count_info", C, N, ptr(means2d), ptr(radii), ptr(depths), tile_size, tile_width,
-          tile_height, ptr(S.tpg), ptr(S.cum), ptr(S.info), ptr(_late_status(dev)), ptr(ws), ws_bytes, st)
+          tile_height, ptr(S.tpg), ptr(S.cum), ptr(S.info), ptr(_late_status(dev)), ptr(dkeys), ptr(dvals), ptr(ws),
+          ws_bytes, st)
     S.order = S.info_host = S.ready = None
-    if sort and _sort_mode == "depth_first" and C * N > 0:
+    if depth_first:
         # the info leaves for the host first; the depth sort and its scan
         # run during the round trip
         S.info_host = torch.empty((3,), dtype=torch.int64, pin_memory=True)
         S.info_host.copy_(S.info, non_blocking=True)
         S.ready = torch.cuda.Event()
         S.ready.record(torch.cuda.current_stream(dev))
-        dkeys = torch.empty((C * N,), dtype=torch.int64, device=dev)
-        dvals = torch.empty((C * N,), dtype=torch.int32, device=dev)
-        _call("ubs_isect_depth_keys", C, N, ptr(radii), ptr(depths), ptr(dkeys), ptr(dvals), st)
         _, S.order = radix_sort_pairs(dkeys, dvals, 0, 32, status_out=_late_status(dev))
         S.cum = torch.empty((C * N,), dtype=torch.int64, device=dev)
         ows = int(L.ubs_isect_ordered_workspace_bytes(C, N))
