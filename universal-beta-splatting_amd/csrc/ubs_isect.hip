@@ -278,30 +278,45 @@ __global__ __launch_bounds__(256) void isect_emit_kernel(uint32_t C, uint32_t N,
 // reference: isect_tiles.cu:287-333
 // n_dev (may be null): the count in device memory, clamped to n_cap (the
 // grid's size); a device count of 0 zero-fills the offsets.
+constexpr int kOffItems = 4;  // consecutive ids per thread (two 16-byte loads)
+
 __global__ void isect_offsets_kernel(int64_t n_cap, const int64_t *__restrict__ n_dev,
                                      const int64_t *__restrict__ ids, uint32_t tile_bits, int64_t n_tiles,
                                      int64_t total_tiles, int32_t *__restrict__ offsets) {
     const int64_t n = n_dev ? min(max(*n_dev, (int64_t)0), n_cap) : n_cap;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (n == 0) {
-        for (int64_t k = i; k < total_tiles; k += (int64_t)gridDim.x * blockDim.x) offsets[k] = 0;
+        for (int64_t k = t; k < total_tiles; k += (int64_t)gridDim.x * blockDim.x) offsets[k] = 0;
         return;
     }
-    if (i >= n) return;
+    const int64_t i0 = t * kOffItems;
+    if (i0 >= n) return;
     const uint64_t mask = (1ull << tile_bits) - 1ull;
-    const uint64_t hi = (uint64_t)ids[i] >> 32;
-    const int64_t cur = (int64_t)(hi >> tile_bits) * n_tiles + (int64_t)(hi & mask);
-    if (i == 0) {
-        for (int64_t k = 0; k <= cur && k < total_tiles; ++k) offsets[k] = 0;
+    auto flat = [&](int64_t id) {  // the (camera, tile) index of an isect id
+        const uint64_t hi = (uint64_t)id >> 32;
+        return (int64_t)(hi >> tile_bits) * n_tiles + (int64_t)(hi & mask);
+    };
+    int64_t v[kOffItems];
+    if (i0 + kOffItems <= n) {
+        const longlong2 a = reinterpret_cast<const longlong2 *>(ids + i0)[0];
+        const longlong2 b = reinterpret_cast<const longlong2 *>(ids + i0)[1];
+        v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+    } else {
+#pragma unroll
+        for (int j = 0; j < kOffItems; ++j) v[j] = i0 + j < n ? ids[i0 + j] : 0;
     }
-    if (i == n - 1) {
-        for (int64_t k = cur + 1; k < total_tiles; ++k) offsets[k] = (int32_t)n;
-    }
-    if (i > 0) {
-        const uint64_t hp = (uint64_t)ids[i - 1] >> 32;
-        const int64_t prev = (int64_t)(hp >> tile_bits) * n_tiles + (int64_t)(hp & mask);
-        if (prev == cur) return;
+    // entry i starts every (camera, tile) after its predecessor's up to its own
+    // (entry 0: from the first); the last entry ends the rest with n
+    int64_t prev = i0 > 0 ? flat(ids[i0 - 1]) : -1;
+#pragma unroll
+    for (int j = 0; j < kOffItems; ++j) {
+        const int64_t i = i0 + j;
+        if (i >= n) break;
+        const int64_t cur = flat(v[j]);
         for (int64_t k = prev + 1; k <= cur && k < total_tiles; ++k) offsets[k] = (int32_t)i;
+        if (i == n - 1)
+            for (int64_t k = cur + 1; k < total_tiles; ++k) offsets[k] = (int32_t)n;
+        prev = cur;
     }
 }
 
@@ -485,8 +500,8 @@ int ubs_isect_offset_encode(int64_t n, const int64_t *ids, uint32_t C, uint32_t 
     if (n == 0) return zero_async(offsets, (size_t)total_tiles * sizeof(int32_t), st);
     UBS_REQUIRE(ids, "null pointer");
     UBS_REQUIRE(n < (1ll << 31), "n_isects must be < 2^31");
-    isect_offsets_kernel<<<grid_for(n), kThreads, 0, st>>>(n, nullptr, ids, bit_width_u32((uint32_t)n_tiles),
-                                                           n_tiles, total_tiles, offsets);
+    isect_offsets_kernel<<<grid_for((n + kOffItems - 1) / kOffItems), kThreads, 0, st>>>(
+        n, nullptr, ids, bit_width_u32((uint32_t)n_tiles), n_tiles, total_tiles, offsets);
     UBS_LAUNCH_CHECK("isect_offsets_kernel");
     return UBS_OK;
 }
@@ -498,7 +513,7 @@ int ubs_isect_offset_encode_dev(int64_t n_cap, const int64_t *n_dev, const int64
     if (total_tiles == 0) return UBS_OK;
     UBS_REQUIRE(offsets && n_dev && (ids || n_cap == 0), "null pointer");
     UBS_REQUIRE(n_cap >= 0 && n_cap < (1ll << 31), "n_cap must be in [0, 2^31)");
-    isect_offsets_kernel<<<grid_for(std::max<int64_t>(n_cap, 1)), kThreads, 0, st>>>(
+    isect_offsets_kernel<<<grid_for(std::max<int64_t>((n_cap + kOffItems - 1) / kOffItems, 1)), kThreads, 0, st>>>(
         n_cap, n_dev, ids, bit_width_u32((uint32_t)n_tiles), n_tiles, total_tiles, offsets);
     UBS_LAUNCH_CHECK("isect_offsets_kernel");
     return UBS_OK;
