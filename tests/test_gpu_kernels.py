@@ -662,3 +662,29 @@ def test_segmented_depth_sort_matches_full_sort(device, case):
         assert_same(host(flat), want_flat, f"flat {mode}")
         if offs is not None:  # offsets handed out on the way are those of the final ids
             assert_same(host(offs), O.isect_offset_encode(want_ids, C, tw, th), f"offsets {mode}")
+
+
+# ------------------------------------------------------------- camera centres
+def test_camera_centers_match_the_fused_projection(device):
+    """ubs_camera_centers gives -(R^T t) (float64 reference, float32
+    tolerance), and the fused projection given no cam_pos takes the same
+    centres: every output equals the one with cam_pos = camera_centers(...)
+    bit for bit, for 3 views of a 6-D scene."""
+    from gsplat import beta
+
+    vms, Ks, _ = scenes.make_cameras(3, 200, 120, seed=4, radius=(3.0, 5.0))
+    want = np.stack([-(v[:3, :3].astype(np.float64).T @ v[:3, 3].astype(np.float64)) for v in vms])
+    got = host(W.camera_centers(cu(vms)))
+    assert np.abs(got - want).max() <= 1e-5 * np.abs(want).max()
+    params = beta.BetaParams.from_scene(scenes.make_scene(3000, 6, seed=5), torch.device(device))
+    args = (params.xyz, params.mean_rest, params.scale_raw, params.l_triangle, params.opacity_raw, params.beta_raw,
+            params.quats, cu(vms), cu(Ks), 200, 120)
+    with torch.no_grad():
+        a = W.beta_project(*args)
+        b = W.beta_project(*args, cam_pos=W.camera_centers(cu(vms)))
+    for x, y, n in zip(a, b, ("radii", "means2d", "depths", "conics", "opacities", "betas")):
+        vis = host(a[0]) > 0
+        xa, ya = host(x), host(y)
+        if n != "radii":  # (culled entries' 2-D outputs are left untouched)
+            xa, ya = xa[vis], ya[vis]
+        assert_same(xa, ya, n)
