@@ -77,6 +77,7 @@ ENTRY_ALIASES = {"ubs_rasterize_to_pixels_bwd_reuse": "ubs_rasterize_to_pixels_b
                  "ubs_rasterize_to_pixels_fwd_prepacked": "ubs_rasterize_to_pixels_fwd",
                  "ubs_beta_project_fwd_rec": "ubs_beta_project_fwd",
                  "ubs_isect_tiles_emit_ordered_cap": "ubs_isect_tiles_emit_ordered",
+                 "ubs_isect_tiles_emit_ordered_rec": "ubs_isect_tiles_emit_ordered",
                  "ubs_radix_sort_pairs_flag": "ubs_radix_sort_pairs",
                  "ubs_isect_tiles_count_info": "ubs_isect_tiles_count_range"}
 

@@ -312,13 +312,25 @@ int ubs_isect_tiles_count_range(uint32_t C, uint32_t N, const float *means2d, co
  * info[0] = n_isects, info[1] = the depth range as above (two uint32),
  * info[2] = *status_in (a status word the caller accumulates; NULL -> 0);
  * with depth_keys / depth_vals (both or neither) it also writes what
- * ubs_isect_depth_keys writes, for the depth-first emission's sort.  No
- * counterpart in the reference. */
+ * ubs_isect_depth_keys writes, for the depth-first emission's sort; with
+ * pair_records (float4 [C*N], may be NULL) the pair's (x, y, radius bits,
+ * depth) for ubs_isect_tiles_emit_ordered_rec.  No counterpart in the
+ * reference. */
 int ubs_isect_tiles_count_info(uint32_t C, uint32_t N, const float *means2d, const int32_t *radii,
                                const float *depths, uint32_t tile_size, uint32_t tile_width,
                                uint32_t tile_height, int32_t *tiles_per_gauss, int64_t *cum_tiles,
                                int64_t *info, const int64_t *status_in, int64_t *depth_keys,
-                               int32_t *depth_vals, void *workspace, size_t workspace_bytes, void *stream);
+                               int32_t *depth_vals, void *pair_records, void *workspace,
+                               size_t workspace_bytes, void *stream);
+
+/* ubs_isect_tiles_emit_ordered_cap reading each pair's centre, radius and
+ * depth from the 16-byte pair records ubs_isect_tiles_count_info wrote (one
+ * gather per pair in depth order instead of three).  No counterpart in the
+ * reference. */
+int ubs_isect_tiles_emit_ordered_rec(uint32_t C, uint32_t N, const void *pair_records, const int32_t *order,
+                                     const int64_t *cum, uint32_t tile_size, uint32_t tile_width,
+                                     uint32_t tile_height, int64_t *isect_ids, int32_t *flatten_ids,
+                                     int64_t cap, void *stream);
 
 /* Bits the (cam, tile) prefix of an isect id occupies: 32 + this = end_bit. */
 uint32_t ubs_isect_prefix_bits(uint32_t C, uint32_t tile_width, uint32_t tile_height);

@@ -73,7 +73,8 @@ __global__ void isect_count_kernel(uint64_t total, const float *__restrict__ mea
                                    const int32_t *__restrict__ radii, const float *__restrict__ depths,
                                    uint32_t ts, uint32_t tw, uint32_t th, int32_t *__restrict__ tpg,
                                    int64_t *__restrict__ block_sums, uint32_t *__restrict__ block_range,
-                                   int64_t *__restrict__ dkeys = nullptr, int32_t *__restrict__ dvals = nullptr) {
+                                   int64_t *__restrict__ dkeys = nullptr, int32_t *__restrict__ dvals = nullptr,
+                                   float4 *__restrict__ prec = nullptr) {
     __shared__ int64_t lds_wave[kThreads / 64];
     __shared__ uint32_t lds_max[kThreads / 64];
     const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
@@ -95,6 +96,9 @@ __global__ void isect_count_kernel(uint64_t total, const float *__restrict__ mea
                 dkeys[idx] = radii[idx] > 0 ? (int64_t)__float_as_uint(depths[idx]) : (int64_t)0xFFFFFFFFu;
                 dvals[idx] = (int32_t)idx;
             }
+            if (prec)  // what the emission reads of the pair, in one 16-byte record
+                prec[idx] = make_float4(means2d[idx * 2], means2d[idx * 2 + 1], __int_as_float(radii[idx]),
+                                        depths[idx]);
         }
     }
     int64_t tot;
@@ -221,7 +225,8 @@ __global__ __launch_bounds__(256) void isect_emit_kernel(uint32_t C, uint32_t N,
                                                          int64_t *__restrict__ isect_ids,
                                                          int32_t *__restrict__ flatten_ids,
                                                          const int32_t *__restrict__ order,
-                                                         int64_t cap = INT64_MAX) {
+                                                         int64_t cap = INT64_MAX,
+                                                         const float4 *__restrict__ prec = nullptr) {
     const uint64_t CN = (uint64_t)C * N;
     const uint64_t pos = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;  // place in emission order
     const int lane = (int)(threadIdx.x & 63);
@@ -231,9 +236,12 @@ __global__ __launch_bounds__(256) void isect_emit_kernel(uint32_t C, uint32_t N,
     uint64_t idx = !valid ? 0 : order ? (uint64_t)(uint32_t)order[pos] : pos;  // the flat (camera, primitive) id
     const bool in_range = idx < CN;  // (false only for an order from a depth sort that gave up: no tiles)
     if (!in_range) idx = 0;
-    const int32_t r = valid && in_range ? radii[idx] : 0;
+    // the pair's centre, radius and depth: one 16-byte record, or three arrays
+    const float4 pr = prec && valid && in_range ? prec[idx] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const int32_t r = !(valid && in_range) ? 0 : prec ? __float_as_int(pr.z) : radii[idx];
     uint32_t x0 = 0, y0 = 0, x1 = 0, y1 = 0;
-    if (r > 0) tile_box(means2d[idx * 2], means2d[idx * 2 + 1], r, ts, tw, th, x0, y0, x1, y1);
+    if (r > 0)
+        tile_box(prec ? pr.x : means2d[idx * 2], prec ? pr.y : means2d[idx * 2 + 1], r, ts, tw, th, x0, y0, x1, y1);
     const uint32_t bw = x1 - x0;
     const uint32_t cnt = r > 0 ? (y1 - y0) * bw : 0u;
     // exclusive prefix of the counts over the wave
@@ -247,7 +255,8 @@ __global__ __launch_bounds__(256) void isect_emit_kernel(uint32_t C, uint32_t N,
     const uint32_t total = __shfl(inc, 63, 64);
     const int64_t wave_start = pos0 == 0 ? 0 : cum[pos0 - 1];
     const int64_t cid = valid ? (int64_t)(idx / N) : 0;
-    const int64_t key_hi = (cid << (32 + tile_bits)) | (int64_t)__float_as_int(valid ? depths[idx] : 0.f);
+    const float dep = !valid ? 0.f : prec ? pr.w : depths[idx];
+    const int64_t key_hi = (cid << (32 + tile_bits)) | (int64_t)__float_as_int(dep);
     const int32_t fid = (int32_t)idx;
     const uint32_t rounds = (total + 63u) / 64u;
     for (uint32_t it = 0; it < rounds; ++it) {  // every lane runs every round: the shuffles see all lanes
@@ -348,7 +357,7 @@ static int count_impl(uint32_t C, uint32_t N, const float *means2d, const int32_
                       uint32_t ts, uint32_t tw, uint32_t th, int32_t *tpg, int64_t *cum, int64_t *n_isects_dev,
                       uint32_t *range_dev, void *workspace, size_t ws_bytes, void *stream,
                       const int64_t *status_in, int64_t *status_out, int64_t *dkeys = nullptr,
-                      int32_t *dvals = nullptr);
+                      int32_t *dvals = nullptr, float4 *prec = nullptr);
 
 int ubs_isect_tiles_count_range(uint32_t C, uint32_t N, const float *means2d, const int32_t *radii,
                                 const float *depths, uint32_t ts, uint32_t tw, uint32_t th, int32_t *tpg,
@@ -361,18 +370,20 @@ int ubs_isect_tiles_count_range(uint32_t C, uint32_t N, const float *means2d, co
 int ubs_isect_tiles_count_info(uint32_t C, uint32_t N, const float *means2d, const int32_t *radii,
                                const float *depths, uint32_t ts, uint32_t tw, uint32_t th, int32_t *tpg,
                                int64_t *cum, int64_t *info, const int64_t *status_in, int64_t *depth_keys,
-                               int32_t *depth_vals, void *workspace, size_t ws_bytes, void *stream) {
+                               int32_t *depth_vals, void *pair_records, void *workspace, size_t ws_bytes,
+                               void *stream) {
     UBS_REQUIRE(info && depths, "null pointer");
     UBS_REQUIRE(!depth_keys == !depth_vals, "depth_keys and depth_vals go together");
     return count_impl(C, N, means2d, radii, depths, ts, tw, th, tpg, cum, info,
                       reinterpret_cast<uint32_t *>(info + 1), workspace, ws_bytes, stream, status_in, info + 2,
-                      depth_keys, depth_vals);
+                      depth_keys, depth_vals, static_cast<float4 *>(pair_records));
 }
 
 static int count_impl(uint32_t C, uint32_t N, const float *means2d, const int32_t *radii, const float *depths,
                       uint32_t ts, uint32_t tw, uint32_t th, int32_t *tpg, int64_t *cum, int64_t *n_isects_dev,
                       uint32_t *range_dev, void *workspace, size_t ws_bytes, void *stream,
-                      const int64_t *status_in, int64_t *status_out, int64_t *dkeys, int32_t *dvals) {
+                      const int64_t *status_in, int64_t *status_out, int64_t *dkeys, int32_t *dvals,
+                      float4 *prec) {
     hipStream_t st = as_stream(stream);
     const uint64_t total = (uint64_t)C * N;
     UBS_REQUIRE(n_isects_dev, "n_isects_dev is null");
@@ -395,9 +406,9 @@ static int count_impl(uint32_t C, uint32_t N, const float *means2d, const int32_
     int64_t *bs = reinterpret_cast<int64_t *>(workspace);
     uint32_t *br = reinterpret_cast<uint32_t *>(bs + nb);
     const float *dp = range_dev ? depths : nullptr;
-    UBS_REQUIRE(!dkeys || depths, "depth keys need depths");
+    UBS_REQUIRE(!(dkeys || prec) || depths, "depth keys and pair records need depths");
     isect_count_kernel<<<(unsigned)nb, kThreads, 0, st>>>(total, means2d, radii, dp ? dp : (dkeys ? depths : nullptr),
-                                                         ts, tw, th, tpg, bs, br, dkeys, dvals);
+                                                         ts, tw, th, tpg, bs, br, dkeys, dvals, prec);
     UBS_LAUNCH_CHECK("isect_count_kernel");
     isect_cum_kernel<<<(unsigned)nb, kThreads, 0, st>>>(total, tpg, bs, (int64_t)nb, cum, nullptr, n_isects_dev,
                                                         range_dev ? br : nullptr, range_dev, status_in, status_out);
@@ -460,7 +471,17 @@ int ubs_isect_scan_ordered(uint32_t C, uint32_t N, const int32_t *tpg, const int
 
 static int emit_ordered(uint32_t C, uint32_t N, const float *means2d, const int32_t *radii, const float *depths,
                         const int32_t *order, const int64_t *cum, uint32_t ts, uint32_t tw, uint32_t th,
-                        int64_t *isect_ids, int32_t *flatten_ids, int64_t cap, void *stream);
+                        int64_t *isect_ids, int32_t *flatten_ids, int64_t cap, void *stream,
+                        const float4 *prec = nullptr);
+
+int ubs_isect_tiles_emit_ordered_rec(uint32_t C, uint32_t N, const void *pair_records, const int32_t *order,
+                                     const int64_t *cum, uint32_t ts, uint32_t tw, uint32_t th, int64_t *isect_ids,
+                                     int32_t *flatten_ids, int64_t cap, void *stream) {
+    UBS_REQUIRE(cap >= 0, "cap must be >= 0");
+    UBS_REQUIRE(pair_records || (uint64_t)C * N == 0, "null pointer");
+    return emit_ordered(C, N, nullptr, nullptr, nullptr, order, cum, ts, tw, th, isect_ids, flatten_ids, cap, stream,
+                        static_cast<const float4 *>(pair_records));
+}
 
 int ubs_isect_tiles_emit_ordered(uint32_t C, uint32_t N, const float *means2d, const int32_t *radii,
                                  const float *depths, const int32_t *order, const int64_t *cum, uint32_t ts,
@@ -479,14 +500,14 @@ int ubs_isect_tiles_emit_ordered_cap(uint32_t C, uint32_t N, const float *means2
 
 static int emit_ordered(uint32_t C, uint32_t N, const float *means2d, const int32_t *radii, const float *depths,
                         const int32_t *order, const int64_t *cum, uint32_t ts, uint32_t tw, uint32_t th,
-                        int64_t *isect_ids, int32_t *flatten_ids, int64_t cap, void *stream) {
+                        int64_t *isect_ids, int32_t *flatten_ids, int64_t cap, void *stream, const float4 *prec) {
     const uint64_t total = (uint64_t)C * N;
     if (total == 0) return UBS_OK;
     UBS_REQUIRE(total < (1ull << 31), "C*N must be < 2^31");
-    UBS_REQUIRE(means2d && radii && depths && order && cum && isect_ids && flatten_ids, "null pointer");
+    UBS_REQUIRE((prec || (means2d && radii && depths)) && order && cum && isect_ids && flatten_ids, "null pointer");
     const uint32_t tile_bits = bit_width_u32(tw * th);
     isect_emit_kernel<<<grid_for((int64_t)total), kThreads, 0, as_stream(stream)>>>(
-        C, N, means2d, radii, depths, cum, ts, tw, th, tile_bits, isect_ids, flatten_ids, order, cap);
+        C, N, means2d, radii, depths, cum, ts, tw, th, tile_bits, isect_ids, flatten_ids, order, cap, prec);
     UBS_LAUNCH_CHECK("isect_emit_kernel");
     return UBS_OK;
 }

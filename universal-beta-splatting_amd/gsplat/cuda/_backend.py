@@ -70,8 +70,10 @@ _SIGNATURES = {
     "ubs_radix_sort_pairs_flag": (
         _c_int, [_c_i64, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_u32, _vp, _c_size, ctypes.POINTER(_c_int), _vp, _vp]),
     "ubs_isect_tiles_count_info": (
-        _c_int, [_c_u32, _c_u32, _vp, _vp, _vp, _c_u32, _c_u32, _c_u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_size,
-                 _vp]),
+        _c_int, [_c_u32, _c_u32, _vp, _vp, _vp, _c_u32, _c_u32, _c_u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                 _c_size, _vp]),
+    "ubs_isect_tiles_emit_ordered_rec": (
+        _c_int, [_c_u32, _c_u32, _vp, _vp, _vp, _c_u32, _c_u32, _c_u32, _vp, _vp, _c_i64, _vp]),
     "ubs_radix_sort_pairs_skip": (
         _c_int, [_c_i64, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_u32, _vp, _c_size, ctypes.POINTER(_c_int), _vp]),
     "ubs_beta_project_fwd_rows": (

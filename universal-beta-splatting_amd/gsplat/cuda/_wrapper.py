@@ -348,7 +348,7 @@ def isect_read_info(states) -> list:
 class IsectState:
     """The intersection stage up to the host read of n_isects (see isect_begin)."""
     __slots__ = ("means2d", "radii", "depths", "tile_size", "tile_width", "tile_height", "sort", "tpg", "cum",
-                 "info", "info_host", "ready", "order", "C", "N")
+                 "info", "info_host", "ready", "order", "prec", "C", "N")
 
 
 def isect_begin(means2d: Tensor, radii: Tensor, depths: Tensor, tile_size: int, tile_width: int, tile_height: int,
@@ -387,9 +387,11 @@ def isect_begin(means2d: Tensor, radii: Tensor, depths: Tensor, tile_size: int, 
     # the depth-first sort's input (depth bits, flat ids), written by the count pass
     dkeys = torch.empty((C * N,), dtype=torch.int64, device=dev) if depth_first else None
     dvals = torch.empty((C * N,), dtype=torch.int32, device=dev) if depth_first else None
+    # the emission's per-pair reads (centre, radius, depth) in one 16-byte record
+    S.prec = torch.empty((C * N, 4), dtype=torch.float32, device=dev) if depth_first else None
     _call("ubs_isect_tiles_count_info", C, N, ptr(means2d), ptr(radii), ptr(depths), tile_size, tile_width,
-          tile_height, ptr(S.tpg), ptr(S.cum), ptr(S.info), ptr(_late_status(dev)), ptr(dkeys), ptr(dvals), ptr(ws),
-          ws_bytes, st)
+          tile_height, ptr(S.tpg), ptr(S.cum), ptr(S.info), ptr(_late_status(dev)), ptr(dkeys), ptr(dvals),
+          ptr(S.prec), ptr(ws), ws_bytes, st)
     S.order = S.info_host = S.ready = None
     if depth_first:
         # the info leaves for the host first; the depth sort and its scan
@@ -417,8 +419,8 @@ def isect_finish(S: IsectState, n_isects: int, packed: int):
     if n_isects and S.order is not None:
         # capped at n_isects: the depth order may come from a sort whose
         # status the host has not read yet
-        _call("ubs_isect_tiles_emit_ordered_cap", C, S.N, ptr(S.means2d), ptr(S.radii), ptr(S.depths),
-              ptr(S.order), ptr(S.cum), ts, tw, th, ptr(isect_ids), ptr(flatten_ids), n_isects, st)
+        _call("ubs_isect_tiles_emit_ordered_rec", C, S.N, ptr(S.prec), ptr(S.order), ptr(S.cum), ts, tw, th,
+              ptr(isect_ids), ptr(flatten_ids), n_isects, st)
     elif n_isects:
         _call("ubs_isect_tiles_emit", C, S.N, ptr(S.means2d), ptr(S.radii), ptr(S.depths), ptr(S.cum), ts, tw, th,
               ptr(isect_ids), ptr(flatten_ids), st)
